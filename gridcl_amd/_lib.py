@@ -1,0 +1,110 @@
+"""ctypes binding of the C ABI declared in include/tetraloma_geom.h.
+
+The library is built in-tree (``gridcl_amd/lib/libtetraloma_geom.so``) by
+``gridcl_amd.build.build_library``. There is no fallback: importing a module that needs the
+library raises if it is missing, so no caller can silently run anything but the HIP kernels.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_char_p, c_double, c_int32, c_size_t, c_uint32, c_uint64, c_void_p
+
+LIB_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")
+LIB_PATH = os.path.join(LIB_DIR, "libtetraloma_geom.so")
+
+# Every symbol the header declares, with (restype, argtypes).
+SIGNATURES = {
+    "tg_last_error_message": (c_char_p, []),
+    "tg_version": (c_char_p, []),
+    "tg_field_elems": (c_uint64, [c_uint64, c_uint64, c_uint32, c_uint32]),
+    "tg_num_lagrange_nodes": (c_uint64, [c_int32, c_uint32]),
+    "tg_lagrange_points": (
+        c_int32,
+        [c_int32, c_uint32, c_int32, POINTER(c_double), c_size_t, POINTER(c_size_t)],
+    ),
+    "tg_geometry_create": (
+        c_int32,
+        [c_int32, c_int32, c_void_p, c_uint64, POINTER(c_double), c_void_p, c_int32,
+         POINTER(c_void_p)],
+    ),
+    "tg_geometry_destroy": (c_int32, [c_void_p]),
+    "tg_geometry_num_blocks": (c_int32, [c_void_p]),
+    "tg_geometry_block_cells": (c_uint64, [c_void_p, c_int32]),
+    "tg_geometry_set_fixup_threshold": (c_int32, [c_void_p, c_double]),
+    "tg_pointset_create": (
+        c_int32,
+        [c_int32, POINTER(c_double), POINTER(c_double), c_uint64, POINTER(c_double),
+         POINTER(c_void_p)],
+    ),
+    "tg_pointset_gauss": (c_int32, [c_int32, c_uint32, POINTER(c_void_p)]),
+    "tg_pointset_facet_gauss": (c_int32, [c_int32, c_uint32, POINTER(c_void_p)]),
+    "tg_pointset_tet_rule": (c_int32, [c_int32, POINTER(c_void_p)]),
+    "tg_pointset_destroy": (c_int32, [c_void_p]),
+    "tg_pointset_len": (c_uint64, [c_void_p]),
+    "tg_pointset_get": (
+        c_int32, [c_void_p, POINTER(c_double), POINTER(c_double), POINTER(c_double),
+                  POINTER(c_uint64)]
+    ),
+    "tg_pointset_tabulate": (
+        c_int32, [c_void_p, c_uint32, c_int32, POINTER(c_double), POINTER(c_double)]
+    ),
+    "tg_eval_block": (
+        c_int32, [c_void_p, c_int32, c_void_p, c_uint32, c_uint32, c_uint32, c_void_p, c_void_p]
+    ),
+    "tg_evector_build": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p]),
+    "tg_evector_elems": (c_uint64, [c_void_p, c_int32]),
+}
+
+
+class BlockDesc(ctypes.Structure):
+    """tg_block_desc."""
+
+    _fields_ = [
+        ("cell", c_int32),
+        ("degree", c_uint32),
+        ("family", c_int32),
+        ("dofmap", c_void_p),
+        ("dofmap_len", c_uint64),
+        ("dofmap_stride", c_uint32),
+        ("flags", c_void_p),
+    ]
+
+
+class Outputs(ctypes.Structure):
+    """tg_outputs."""
+
+    _fields_ = [
+        ("points", c_void_p),
+        ("jacobians", c_void_p),
+        ("detj", c_void_p),
+        ("pinv", c_void_p),
+        ("facet_normals", c_void_p),
+        ("normals", c_void_p),
+    ]
+
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """The loaded library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with gridcl_amd.build.build_library() "
+            "(the geometry path has no CPU fallback)"
+        )
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def last_error() -> str:
+    return load().tg_last_error_message().decode()

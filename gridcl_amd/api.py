@@ -1,0 +1,508 @@
+"""Python mirror of the reference's geometry API over the C ABI (include/tetraloma_geom.h).
+
+Names, argument meanings and errors follow tetraloma-geometry's public API (src/lib.rs:78-99):
+``Geometry`` (geometry.rs:49), ``GeometryBlock`` / ``CoordinateElement`` (element.rs:30-70),
+``PointSet`` (point_set.rs:57), ``QuadRule`` (rule.rs:49), ``FacetQuadRule`` (facet_rule.rs:204),
+``QuadField`` (field.rs:162), the block functions ``physical_points_block``, ``jacobians_block``
+(points_jac.rs:42, 90), ``detj_block``, ``pseudo_inverse_block`` (detj_pinv.rs:42, 90),
+``facet_normals_block``, ``normals_block`` (facets.rs:46, 107) and the mesh-level loops of
+``Mesh`` (tetraloma/src/mesh.rs:249-257, 434-964). ``geometric_factors_block`` is the fused
+evaluation of several factors in one kernel pass, which the reference does not have.
+
+Device memory is held in torch tensors on the current HIP device; every evaluation is enqueued
+on torch's current stream. Nothing here computes geometry on the host.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass
+from enum import IntEnum
+from typing import Iterable, Sequence
+
+import numpy as np
+import torch
+
+from . import _lib
+
+
+class CellType(IntEnum):
+    POINT = 0
+    INTERVAL = 1
+    TRIANGLE = 2
+    QUADRILATERAL = 3
+    TETRAHEDRON = 4
+    HEXAHEDRON = 5
+    PRISM = 6
+    PYRAMID = 7
+
+
+class NodeFamily(IntEnum):
+    EQUISPACED = 0
+    GLL = 1
+
+
+TDIM = {CellType.POINT: 0, CellType.INTERVAL: 1, CellType.TRIANGLE: 2,
+        CellType.QUADRILATERAL: 2, CellType.TETRAHEDRON: 3, CellType.HEXAHEDRON: 3,
+        CellType.PRISM: 3, CellType.PYRAMID: 3}
+
+# Factor bits of the C ABI and their names.
+FACTORS = {"points": 1, "jacobians": 2, "detj": 4, "pinv": 8, "facet_normals": 16, "normals": 32}
+
+OPT_NO_CELL_SHIFT = 1
+OPT_NO_FP64_FIXUP = 2
+OPT_FORCE_GENERIC = 4
+
+
+# ---- Errors: one class per tetraloma_device::GridError variant of the path -------------------
+
+class GridError(Exception):
+    code = -1
+
+
+class LengthMismatch(GridError):
+    code = 1
+
+
+class TooLarge(GridError):
+    code = 2
+
+
+class AllocationTooLarge(GridError):
+    code = 3
+
+
+class UnsupportedDegree(GridError):
+    code = 4
+
+
+class UnsupportedNodeFamily(GridError):
+    code = 5
+
+
+class CellNotInRule(GridError):
+    code = 6
+
+
+class UnsupportedPrecision(GridError):
+    code = 7
+
+
+class InvalidArgument(GridError):
+    code = 8
+
+
+class ServerError(GridError):
+    code = 9
+
+
+_ERRORS = {c.code: c for c in (LengthMismatch, TooLarge, AllocationTooLarge, UnsupportedDegree,
+                               UnsupportedNodeFamily, CellNotInRule, UnsupportedPrecision,
+                               InvalidArgument, ServerError)}
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise _ERRORS.get(rc, GridError)(_lib.last_error())
+
+
+def _dptr(t: torch.Tensor | None) -> int | None:
+    return None if t is None else t.data_ptr()
+
+
+def _stream() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _dbl(a: np.ndarray | None):
+    if a is None:
+        return None
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), a
+
+
+# ---- Elements, blocks, geometry -----------------------------------------------------------------
+
+@dataclass(frozen=True)
+class CoordinateElement:
+    """The coordinate element of a block (element.rs:30-62)."""
+
+    cell: CellType
+    degree: int
+    family: NodeFamily = NodeFamily.EQUISPACED
+
+    @property
+    def tdim(self) -> int:
+        return TDIM[CellType(self.cell)]
+
+    @property
+    def nodes_per_cell(self) -> int:
+        return int(_lib.load().tg_num_lagrange_nodes(int(self.cell), int(self.degree)))
+
+
+@dataclass
+class GeometryBlock:
+    """One cell block: its element, its dofmap (u32 [cells][k], held as a device int32 tensor of
+    the same bits) and optional cell flags (bit 0 flips normals)."""
+
+    element: CoordinateElement
+    dofmap: torch.Tensor
+    flags: torch.Tensor | None = None
+    stride: int | None = None  # defaults to the element's node count
+
+
+def _u32_device(a: np.ndarray | torch.Tensor, device) -> torch.Tensor:
+    if isinstance(a, torch.Tensor):
+        if a.dtype not in (torch.int32,):
+            raise TypeError("dofmaps and flags are int32 tensors holding u32 bits")
+        return a.contiguous()
+    arr = np.ascontiguousarray(a, dtype=np.uint32).view(np.int32)
+    return torch.from_numpy(arr.copy()).to(device)
+
+
+class Geometry:
+    """Node coordinates relative to an origin, plus cell blocks (geometry.rs:49-150).
+
+    ``nodes`` is a device tensor [num_nodes][gdim] of float32 or float64 (the geometry's
+    precision); ``origin`` the gdim f64 values that X adds back."""
+
+    def __init__(self, blocks: Sequence[GeometryBlock], nodes: torch.Tensor,
+                 origin: Sequence[float]):
+        lib = _lib.load()
+        if nodes.dim() != 2:
+            raise LengthMismatch("nodes must be [num_nodes][gdim]")
+        if nodes.dtype == torch.float32:
+            self.dtype = 0
+        elif nodes.dtype == torch.float64:
+            self.dtype = 1
+        else:
+            raise UnsupportedPrecision(f"no geometry of {nodes.dtype}")
+        self.nodes = nodes.contiguous()
+        self.gdim = int(nodes.shape[1])
+        self.origin = [float(o) for o in origin]
+        self.blocks = list(blocks)
+        descs = (_lib.BlockDesc * max(1, len(self.blocks)))()
+        for i, b in enumerate(self.blocks):
+            k = b.element.nodes_per_cell
+            descs[i].cell = int(b.element.cell)
+            descs[i].degree = int(b.element.degree)
+            descs[i].family = int(b.element.family)
+            descs[i].dofmap = _dptr(b.dofmap) if b.dofmap.numel() else None
+            descs[i].dofmap_len = int(b.dofmap.numel())
+            descs[i].dofmap_stride = int(b.stride if b.stride is not None else k)
+            descs[i].flags = _dptr(b.flags) if b.flags is not None else None
+        org = (ctypes.c_double * 3)(*(self.origin + [0.0] * (3 - len(self.origin))))
+        handle = ctypes.c_void_p()
+        _check(lib.tg_geometry_create(self.dtype, self.gdim,
+                                      _dptr(self.nodes) if self.nodes.numel() else None,
+                                      int(self.nodes.shape[0]), org, descs, len(self.blocks),
+                                      ctypes.byref(handle)))
+        self._h = handle
+
+    @property
+    def handle(self):
+        return self._h
+
+    def num_blocks(self) -> int:
+        return len(self.blocks)
+
+    def block_cells(self, b: int) -> int:
+        return int(_lib.load().tg_geometry_block_cells(self._h, b))
+
+    @property
+    def torch_dtype(self):
+        return torch.float32 if self.dtype == 0 else torch.float64
+
+    def set_fixup_threshold(self, tau: float) -> None:
+        _check(_lib.load().tg_geometry_set_fixup_threshold(self._h, float(tau)))
+
+    def evector(self, b: int) -> torch.Tensor:
+        """The E-vector of block b, [k][gdim][cells padded to 16] (evector.rs:197-236)."""
+        lib = _lib.load()
+        n = int(lib.tg_evector_elems(self._h, b))
+        out = torch.empty(max(n, 1), dtype=self.torch_dtype, device=self.nodes.device)
+        _check(lib.tg_evector_build(self._h, b, _dptr(out), _stream()))
+        return out[:n]
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            _lib.load().tg_geometry_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---- Point sets and rules -----------------------------------------------------------------------
+
+class PointSet:
+    """Points [nq][tdim] and weights of one cell type (point_set.rs:57-110), optionally with the
+    outward reference normal of each point's facet (a facet point set, facet_rule.rs:39)."""
+
+    def __init__(self, handle, cell: CellType, facet: bool = False):
+        self._h = handle
+        self.cell = CellType(cell)
+        self.is_facet = facet
+        lib = _lib.load()
+        nq = int(lib.tg_pointset_len(handle))
+        td = TDIM[self.cell]
+        pts = np.zeros(max(1, nq * td))
+        wts = np.zeros(max(1, nq))
+        nrm = np.zeros(max(1, nq * td))
+        count = ctypes.c_uint64()
+        dp = ctypes.POINTER(ctypes.c_double)
+        _check(lib.tg_pointset_get(handle, pts.ctypes.data_as(dp), wts.ctypes.data_as(dp),
+                                   nrm.ctypes.data_as(dp), ctypes.byref(count)))
+        self.points = pts[: nq * td].reshape(nq, td)
+        self.weights = wts[:nq]
+        self.facet_normals = nrm[: nq * td].reshape(nq, td) if count.value else None
+
+    @classmethod
+    def custom(cls, cell: CellType, points, weights, facet_normals=None) -> "PointSet":
+        lib = _lib.load()
+        p = _dbl(np.asarray(points, dtype=np.float64).reshape(-1))
+        w = _dbl(np.asarray(weights, dtype=np.float64).reshape(-1))
+        n = _dbl(None if facet_normals is None else np.asarray(facet_normals).reshape(-1))
+        h = ctypes.c_void_p()
+        _check(lib.tg_pointset_create(int(cell), p[0], w[0], len(w[1]),
+                                      n[0] if n else None, ctypes.byref(h)))
+        return cls(h, cell, facet=facet_normals is not None)
+
+    @classmethod
+    def gauss(cls, cell: CellType, degree: int) -> "PointSet":
+        h = ctypes.c_void_p()
+        _check(_lib.load().tg_pointset_gauss(int(cell), int(degree), ctypes.byref(h)))
+        return cls(h, cell)
+
+    @classmethod
+    def facet_gauss(cls, cell: CellType, degree: int) -> "PointSet":
+        h = ctypes.c_void_p()
+        _check(_lib.load().tg_pointset_facet_gauss(int(cell), int(degree), ctypes.byref(h)))
+        return cls(h, cell, facet=True)
+
+    @classmethod
+    def tet_keast4(cls) -> "PointSet":
+        h = ctypes.c_void_p()
+        _check(_lib.load().tg_pointset_tet_rule(0, ctypes.byref(h)))
+        return cls(h, CellType.TETRAHEDRON)
+
+    @classmethod
+    def tet_14(cls) -> "PointSet":
+        h = ctypes.c_void_p()
+        _check(_lib.load().tg_pointset_tet_rule(1, ctypes.byref(h)))
+        return cls(h, CellType.TETRAHEDRON)
+
+    def __len__(self) -> int:
+        return len(self.weights)
+
+    def tabulate(self, e: CoordinateElement) -> np.ndarray:
+        """phi [q][k] in f64 (point_set.rs:312)."""
+        return self._tab(e)[0]
+
+    def tabulate_derivatives(self, e: CoordinateElement) -> np.ndarray:
+        """dphi [q][k][tdim] in f64 (point_set.rs:350)."""
+        return self._tab(e)[1]
+
+    def _tab(self, e: CoordinateElement):
+        if CellType(e.cell) != self.cell:
+            raise CellNotInRule(self.cell.name)
+        k = e.nodes_per_cell
+        nq, td = len(self), TDIM[self.cell]
+        v = np.zeros(max(1, nq * k))
+        d = np.zeros(max(1, nq * k * td))
+        _check(_lib.load().tg_pointset_tabulate(
+            self._h, int(e.degree), int(e.family),
+            v.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+            d.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return v[: nq * k].reshape(nq, k), d[: nq * k * td].reshape(nq, k, td)
+
+    def __del__(self):
+        try:
+            if self._h:
+                _lib.load().tg_pointset_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+class QuadRule:
+    """One point set per cell type (rule.rs:49-172)."""
+
+    def __init__(self, sets: Iterable[PointSet]):
+        self.sets = list(sets)
+
+    @classmethod
+    def gauss(cls, cells: Iterable[CellType], degree: int) -> "QuadRule":
+        seen = []
+        for c in cells:
+            if CellType(c) not in seen:
+                seen.append(CellType(c))
+        return cls(PointSet.gauss(c, degree) for c in seen)
+
+    @classmethod
+    def for_grid(cls, g: Geometry, degree: int) -> "QuadRule":
+        return cls.gauss([b.element.cell for b in g.blocks], degree)
+
+    def get(self, cell: CellType) -> PointSet:
+        for s in self.sets:
+            if s.cell == CellType(cell):
+                return s
+        raise CellNotInRule(CellType(cell).name)
+
+
+class FacetQuadRule(QuadRule):
+    """Facet point sets per cell type (facet_rule.rs:204-310)."""
+
+    @classmethod
+    def gauss(cls, cells: Iterable[CellType], degree: int) -> "FacetQuadRule":
+        seen = []
+        for c in cells:
+            if CellType(c) not in seen:
+                seen.append(CellType(c))
+        return cls(PointSet.facet_gauss(c, degree) for c in seen)
+
+
+# ---- Fields ------------------------------------------------------------------------------------
+
+class QuadField:
+    """Values at the points of one block in tiles of `tile` cells (field.rs:126-168): component
+    k at point q of cell c at ((c // tile)·nq·LEN + q·LEN + k)·tile + c % tile."""
+
+    def __init__(self, data: torch.Tensor, num_cells: int, num_points: int, length: int,
+                 tile: int):
+        self.data = data
+        self.num_cells = num_cells
+        self.num_points = num_points
+        self.len = length
+        self.tile = tile
+
+    def to_host_cell_major(self) -> np.ndarray:
+        """[cells][points][LEN] on the host, tiling undone (field.rs:380)."""
+        t = self.tile
+        a = self.data.detach().cpu().numpy()
+        padded = (self.num_cells + t - 1) // t * t
+        a = a[: padded * self.num_points * self.len]
+        a = a.reshape(padded // t, self.num_points, self.len, t).transpose(0, 3, 1, 2)
+        return a.reshape(padded, self.num_points, self.len)[: self.num_cells]
+
+
+def factor_len(name: str, tdim: int, gdim: int) -> int:
+    return {"points": gdim, "jacobians": tdim * gdim, "detj": 1, "pinv": tdim * gdim,
+            "facet_normals": gdim, "normals": gdim}[name]
+
+
+def geometric_factors_block(g: Geometry, b: int, pts: PointSet, factors: Sequence[str],
+                            tile: int = 16, options: int = 0,
+                            out: dict | None = None) -> dict[str, QuadField]:
+    """Every factor in `factors` of block b at `pts`, in one fused kernel pass. `out` may
+    supply preallocated device tensors by factor name."""
+    lib = _lib.load()
+    if b < 0 or b >= len(g.blocks):
+        raise InvalidArgument("block index out of range")
+    blk = g.blocks[b]
+    td = TDIM[CellType(blk.element.cell)]
+    ncells = g.block_cells(b)
+    nq = len(pts)
+    mask = 0
+    fields = {}
+    outs = _lib.Outputs()
+    for name in factors:
+        if name not in FACTORS:
+            raise InvalidArgument(f"unknown factor {name}")
+        mask |= FACTORS[name]
+        length = factor_len(name, td, g.gdim)
+        n = int(lib.tg_field_elems(ncells, nq, length, tile))
+        if tile not in (1, 2, 4, 8, 16):
+            raise InvalidArgument("tile must be 1, 2, 4, 8 or 16")
+        if out is not None and name in out:
+            data = out[name]
+            if data.numel() < n or data.dtype != g.torch_dtype:
+                raise LengthMismatch(f"output {name}: need {n} elements of {g.torch_dtype}")
+        else:
+            data = torch.empty(max(n, 1), dtype=g.torch_dtype, device=g.nodes.device)
+        setattr(outs, name, _dptr(data))
+        fields[name] = QuadField(data, ncells, nq, length, tile)
+    _check(lib.tg_eval_block(g.handle, b, pts._h, mask, tile, options, ctypes.byref(outs),
+                             _stream()))
+    return fields
+
+
+def physical_points_block(g: Geometry, b: int, pts: PointSet, tile: int = 16) -> QuadField:
+    return geometric_factors_block(g, b, pts, ["points"], tile)["points"]
+
+
+def jacobians_block(g: Geometry, b: int, pts: PointSet, tile: int = 16) -> QuadField:
+    return geometric_factors_block(g, b, pts, ["jacobians"], tile)["jacobians"]
+
+
+def detj_block(g: Geometry, b: int, pts: PointSet, tile: int = 16) -> QuadField:
+    return geometric_factors_block(g, b, pts, ["detj"], tile)["detj"]
+
+
+def pseudo_inverse_block(g: Geometry, b: int, pts: PointSet, tile: int = 16) -> QuadField:
+    return geometric_factors_block(g, b, pts, ["pinv"], tile)["pinv"]
+
+
+def facet_normals_block(g: Geometry, b: int, pts: PointSet, tile: int = 16) -> QuadField:
+    return geometric_factors_block(g, b, pts, ["facet_normals"], tile)["facet_normals"]
+
+
+def normals_block(g: Geometry, b: int, pts: PointSet, tile: int = 16) -> QuadField:
+    return geometric_factors_block(g, b, pts, ["normals"], tile)["normals"]
+
+
+class Mesh:
+    """The block loop of tetraloma::Mesh's geometric methods (mesh.rs:249-257, 434-964)."""
+
+    def __init__(self, geometry: Geometry):
+        self.geometry = geometry
+
+    def _per_block(self, rule: QuadRule, name: str, tile: int) -> list[QuadField]:
+        g = self.geometry
+        return [geometric_factors_block(g, b, rule.get(blk.element.cell), [name], tile)[name]
+                for b, blk in enumerate(g.blocks)]
+
+    def physical_points(self, rule: QuadRule, tile: int = 16):
+        return self._per_block(rule, "points", tile)
+
+    def jacobians(self, rule: QuadRule, tile: int = 16):
+        return self._per_block(rule, "jacobians", tile)
+
+    def detj(self, rule: QuadRule, tile: int = 16):
+        return self._per_block(rule, "detj", tile)
+
+    def pseudo_inverse_jacobians(self, rule: QuadRule, tile: int = 16):
+        return self._per_block(rule, "pinv", tile)
+
+    def normals(self, rule: QuadRule, tile: int = 16):
+        return self._per_block(rule, "normals", tile)
+
+    def facet_normals(self, rule: FacetQuadRule, tile: int = 16):
+        return self._per_block(rule, "facet_normals", tile)
+
+
+def geometry_from_host(mesh, dtype=torch.float32, device="cuda", origin=None):
+    """Uploads a HostMesh: nodes relative to the centre of their bounding box (the reference's
+    origin, geometry.rs:14-19), narrowed to `dtype`; dofmaps and flags as u32 bits."""
+    x = np.asarray(mesh.nodes, dtype=np.float64)
+    if origin is None:
+        origin = 0.5 * (x.min(axis=0) + x.max(axis=0)) if len(x) else np.zeros(mesh.gdim)
+    rel = x - np.asarray(origin)[None, :]
+    np_dtype = np.float32 if dtype == torch.float32 else np.float64
+    nodes = torch.from_numpy(rel.astype(np_dtype)).to(device)
+    blocks = []
+    for b in mesh.blocks:
+        e = CoordinateElement(CellType(b.cell), int(b.degree), NodeFamily(b.family))
+        dm = _u32_device(np.asarray(b.cells).reshape(-1), device)
+        fl = None if b.flags is None else _u32_device(np.asarray(b.flags), device)
+        blocks.append(GeometryBlock(e, dm, fl))
+    return Geometry(blocks, nodes, [float(o) for o in origin])
+
+
+def stored_nodes_f64(g: Geometry) -> np.ndarray:
+    """The coordinates the geometry stores, widened to f64 plus the origin: the node values a
+    host reference evaluates (tetraloma-testing/src/reference.rs:47-69)."""
+    x = g.nodes.detach().cpu().numpy().astype(np.float64)
+    return x + np.asarray(g.origin)[None, :]

@@ -1,0 +1,464 @@
+// C-ABI entry points declared in include/tetraloma_geom.h: handle management, argument checks
+// with the reference's error variants, cached basis tables, and kernel dispatch.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/tetraloma_geom.h"
+#include "geom_internal.hpp"
+#include "refcell.hpp"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+const char* cell_name(int c) {
+  static const char* names[] = {"Point",       "Interval",   "Triangle", "Quadrilateral",
+                                "Tetrahedron", "Hexahedron", "Prism",    "Pyramid"};
+  return tg::valid_cell(c) ? names[c] : "?";
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  return fail(TG_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+struct Block {
+  tg::Cell cell;
+  unsigned degree;
+  tg::Family family;
+  const uint32_t* dofmap;
+  uint64_t num_cells;
+  const uint32_t* flags;
+  uint32_t k;
+  int tdim;
+};
+
+// Device copies of one element's tables at one point set: interleaved [q][k][4] (value, then
+// the tdim derivatives, zero-padded) in the geometry's type, the same in f64 for the FP64
+// fix-up, and the reference facet normals [q][tdim] in the geometry's type.
+struct DeviceTables {
+  void* tab = nullptr;
+  double* tab64 = nullptr;
+  void* ref_normals = nullptr;
+  ~DeviceTables() {
+    if (tab) (void)hipFree(tab);
+    if (tab64) (void)hipFree(tab64);
+    if (ref_normals) (void)hipFree(ref_normals);
+  }
+};
+
+}  // namespace
+
+struct tg_geometry {
+  int dtype;
+  int gdim;
+  const void* nodes;
+  uint64_t num_nodes;
+  double origin[3];
+  std::vector<Block> blocks;
+  double tau = 1e-2;
+};
+
+struct tg_pointset {
+  tg::Cell cell;
+  int tdim;
+  uint64_t nq;
+  std::vector<double> points, weights, normals;
+  bool has_normals = false;
+  std::map<std::tuple<unsigned, int, int>, std::unique_ptr<DeviceTables>> cache;
+};
+
+namespace {
+
+// Host tables of (cell, degree, family) at the points, interleaved [q][k][4] in f64.
+int host_tables(const tg_pointset* p, unsigned degree, int family, std::vector<double>& inter,
+                size_t* k_out) {
+  tg::LagrangeElement e;
+  std::string err;
+  if (!e.init(p->cell, degree, static_cast<tg::Family>(family), &err)) {
+    if (degree == 0 || err == "unsupported degree")
+      return fail(TG_ERR_UNSUPPORTED_DEGREE, std::string("UnsupportedDegree { cell: ") +
+                                                 cell_name(p->cell) +
+                                                 ", degree: " + std::to_string(degree) + " }");
+    return fail(TG_ERR_UNSUPPORTED_NODE_FAMILY,
+                std::string("UnsupportedNodeFamily { cell: ") + cell_name(p->cell) + " }: " + err);
+  }
+  const size_t k = e.dim(), nq = p->nq;
+  std::vector<double> t;
+  e.tabulate(p->points.data(), nq, 1, t);
+  inter.assign(nq * k * 4, 0.0);
+  for (size_t q = 0; q < nq; ++q)
+    for (size_t i = 0; i < k; ++i) {
+      inter[(q * k + i) * 4] = t[q * k + i];
+      for (int d = 0; d < p->tdim; ++d) inter[(q * k + i) * 4 + 1 + d] = t[((1 + d) * nq + q) * k + i];
+    }
+  *k_out = k;
+  return TG_OK;
+}
+
+template <typename T>
+int upload(const std::vector<double>& src, void** dst) {
+  std::vector<T> narrowed(src.begin(), src.end());
+  const size_t bytes = std::max<size_t>(narrowed.size(), 1) * sizeof(T);
+  hipError_t e = hipMalloc(dst, bytes);
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc(tables)");
+  if (!narrowed.empty()) {
+    e = hipMemcpy(*dst, narrowed.data(), narrowed.size() * sizeof(T), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpy(tables)");
+  }
+  return TG_OK;
+}
+
+// The cached device tables of (degree, family, dtype), computed and uploaded on first use
+// (point_set.rs:355-365).
+int device_tables(tg_pointset* p, unsigned degree, int family, int dtype, DeviceTables** out) {
+  auto key = std::make_tuple(degree, family, dtype);
+  auto it = p->cache.find(key);
+  if (it != p->cache.end()) {
+    *out = it->second.get();
+    return TG_OK;
+  }
+  std::vector<double> inter;
+  size_t k = 0;
+  int rc = host_tables(p, degree, family, inter, &k);
+  if (rc) return rc;
+  auto t = std::make_unique<DeviceTables>();
+  rc = dtype == TG_F32 ? upload<float>(inter, &t->tab) : upload<double>(inter, &t->tab);
+  if (rc) return rc;
+  rc = upload<double>(inter, reinterpret_cast<void**>(&t->tab64));
+  if (rc) return rc;
+  if (p->has_normals) {
+    rc = dtype == TG_F32 ? upload<float>(p->normals, &t->ref_normals)
+                         : upload<double>(p->normals, &t->ref_normals);
+    if (rc) return rc;
+  }
+  *out = t.get();
+  p->cache.emplace(key, std::move(t));
+  return TG_OK;
+}
+
+int make_pointset(tg::Cell cell, std::vector<double> pts, std::vector<double> wts,
+                  std::vector<double> normals, bool has_normals, tg_pointset** out) {
+  auto* p = new tg_pointset;
+  p->cell = cell;
+  p->tdim = tg::tdim(cell);
+  p->nq = wts.size();
+  p->points = std::move(pts);
+  p->weights = std::move(wts);
+  p->normals = std::move(normals);
+  p->has_normals = has_normals;
+  *out = p;
+  return TG_OK;
+}
+
+bool valid_tile(uint32_t tile) {
+  return tile == 1 || tile == 2 || tile == 4 || tile == 8 || tile == 16;
+}
+
+uint32_t log2u(uint32_t v) {
+  uint32_t r = 0;
+  while ((1u << r) < v) ++r;
+  return r;
+}
+
+template <typename T>
+int eval_typed(const tg_geometry* g, const Block& b, tg_pointset* pts, DeviceTables* tabs,
+               uint32_t factors, uint32_t tile, uint32_t options, const tg_outputs* out,
+               hipStream_t stream) {
+  tg::FactorParams<T> p{};
+  p.num_cells = b.num_cells;
+  p.padded_cells = (b.num_cells + tile - 1) / tile * tile;
+  p.last_node = static_cast<uint32_t>(g->num_nodes - 1);
+  p.nq = static_cast<uint32_t>(pts->nq);
+  p.k = b.k;
+  p.tile_log2 = log2u(tile);
+  p.mask = factors;
+  const bool f32 = sizeof(T) == 4;
+  p.fixup = (f32 && !(options & TG_OPT_NO_FP64_FIXUP) && !(options & TG_OPT_NO_CELL_SHIFT)) ? 1u
+                                                                                            : 0u;
+  p.tau = g->tau;
+  for (int i = 0; i < 3; ++i) p.origin[i] = static_cast<T>(i < g->gdim ? g->origin[i] : 0.0);
+  tg::FactorOutputs o{out->points, out->jacobians, out->detj, out->pinv, out->facet_normals,
+                      out->normals};
+  const bool shift = f32 && !(options & TG_OPT_NO_CELL_SHIFT);
+  hipError_t e = tg::launch_factor<T>(
+      b.tdim, g->gdim, shift, (options & TG_OPT_FORCE_GENERIC) != 0,
+      static_cast<const T*>(g->nodes), b.dofmap, b.flags, static_cast<const T*>(tabs->tab),
+      tabs->tab64, static_cast<const T*>(tabs->ref_normals), p, o, stream);
+  if (e == hipErrorNotSupported)
+    return fail(TG_ERR_UNSUPPORTED_DEGREE, "no kernel for this (tdim, gdim)");
+  if (e != hipSuccess) return hip_fail(e, "factor kernel launch");
+  return TG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* tg_last_error_message(void) { return g_last_error.c_str(); }
+const char* tg_version(void) { return "gridcl_amd 0.1 (gfx950)"; }
+
+uint64_t tg_field_elems(uint64_t num_cells, uint64_t nq, uint32_t len, uint32_t tile) {
+  if (!valid_tile(tile)) return 0;
+  return (num_cells + tile - 1) / tile * tile * nq * len;
+}
+
+uint64_t tg_num_lagrange_nodes(int32_t cell, uint32_t degree) {
+  if (!tg::valid_cell(cell)) return 0;
+  return tg::num_lagrange_nodes(static_cast<tg::Cell>(cell), degree);
+}
+
+int tg_lagrange_points(int32_t cell, uint32_t degree, int32_t family, double* out, size_t cap,
+                       size_t* len) {
+  if (!tg::valid_cell(cell) || (family != TG_EQUISPACED && family != TG_GLL))
+    return fail(TG_ERR_INVALID_ARGUMENT, "bad cell or family");
+  std::vector<double> pts;
+  if (!tg::lagrange_points(static_cast<tg::Cell>(cell), degree, static_cast<tg::Family>(family),
+                           pts)) {
+    if (degree == 0)
+      return fail(TG_ERR_UNSUPPORTED_DEGREE, std::string("UnsupportedDegree { cell: ") +
+                                                 cell_name(cell) + ", degree: 0 }");
+    return fail(TG_ERR_UNSUPPORTED_NODE_FAMILY,
+                std::string("UnsupportedNodeFamily { cell: ") + cell_name(cell) + " }");
+  }
+  if (len) *len = pts.size();
+  if (out) {
+    if (cap < pts.size()) return fail(TG_ERR_LENGTH_MISMATCH, "output capacity too small");
+    std::memcpy(out, pts.data(), pts.size() * sizeof(double));
+  }
+  return TG_OK;
+}
+
+int tg_geometry_create(int32_t dtype, int32_t gdim, const void* nodes, uint64_t num_nodes,
+                       const double* origin, const tg_block_desc* blocks, int32_t num_blocks,
+                       tg_geometry** out) {
+  if (!out) return fail(TG_ERR_INVALID_ARGUMENT, "null output handle");
+  *out = nullptr;
+  if (dtype != TG_F32 && dtype != TG_F64)
+    return fail(TG_ERR_UNSUPPORTED_PRECISION, "UnsupportedPrecision: dtype must be F32 or F64");
+  if (gdim < 1 || gdim > 3) return fail(TG_ERR_INVALID_ARGUMENT, "gdim must be 1, 2 or 3");
+  if (num_blocks < 0 || (num_blocks > 0 && !blocks))
+    return fail(TG_ERR_INVALID_ARGUMENT, "bad block list");
+  if (num_nodes > 0 && !nodes) return fail(TG_ERR_INVALID_ARGUMENT, "null nodes");
+  if (num_nodes > 0xffffffffull) return fail(TG_ERR_TOO_LARGE, "TooLarge: more than 2^32 nodes");
+  auto g = std::make_unique<tg_geometry>();
+  g->dtype = dtype;
+  g->gdim = gdim;
+  g->nodes = nodes;
+  g->num_nodes = num_nodes;
+  for (int i = 0; i < 3; ++i) g->origin[i] = (origin && i < gdim) ? origin[i] : 0.0;
+  for (int bi = 0; bi < num_blocks; ++bi) {
+    const tg_block_desc& d = blocks[bi];
+    if (!tg::valid_cell(d.cell) || d.cell == TG_POINT)
+      return fail(TG_ERR_INVALID_ARGUMENT, "block " + std::to_string(bi) + ": bad cell type");
+    const tg::Cell cell = static_cast<tg::Cell>(d.cell);
+    if (tg::tdim(cell) > gdim)
+      return fail(TG_ERR_INVALID_ARGUMENT, "block " + std::to_string(bi) + ": tdim > gdim");
+    if (d.degree == 0)
+      return fail(TG_ERR_UNSUPPORTED_DEGREE, std::string("UnsupportedDegree { cell: ") +
+                                                 cell_name(d.cell) + ", degree: 0 }");
+    if (d.family != TG_EQUISPACED && d.family != TG_GLL)
+      return fail(TG_ERR_INVALID_ARGUMENT, "bad node family");
+    if (d.family == TG_GLL && cell != tg::kInterval && cell != tg::kQuadrilateral &&
+        cell != tg::kHexahedron)
+      return fail(TG_ERR_UNSUPPORTED_NODE_FAMILY, std::string("UnsupportedNodeFamily { cell: ") +
+                                                      cell_name(d.cell) + ", family: Gll }");
+    const uint64_t k = tg::num_lagrange_nodes(cell, d.degree);
+    // geometry.rs:122-140: the stride must equal k and the length be a multiple of it.
+    if (d.dofmap_stride != k || d.dofmap_len % k != 0)
+      return fail(TG_ERR_LENGTH_MISMATCH, "LengthMismatch { expected: " + std::to_string(k) +
+                                              ", found: " + std::to_string(d.dofmap_stride) + " }");
+    if (num_nodes == 0 && d.dofmap_len != 0)
+      return fail(TG_ERR_LENGTH_MISMATCH, "LengthMismatch { expected: 0, found: " +
+                                              std::to_string(d.dofmap_len) + " }");
+    if (d.dofmap_len > 0 && !d.dofmap) return fail(TG_ERR_INVALID_ARGUMENT, "null dofmap");
+    Block b{cell, d.degree, static_cast<tg::Family>(d.family), d.dofmap, d.dofmap_len / k,
+            d.flags, static_cast<uint32_t>(k), tg::tdim(cell)};
+    g->blocks.push_back(b);
+  }
+  *out = g.release();
+  return TG_OK;
+}
+
+int tg_geometry_destroy(tg_geometry* g) {
+  delete g;
+  return TG_OK;
+}
+
+int tg_geometry_num_blocks(const tg_geometry* g) {
+  return g ? static_cast<int>(g->blocks.size()) : -1;
+}
+
+uint64_t tg_geometry_block_cells(const tg_geometry* g, int32_t block) {
+  if (!g || block < 0 || block >= static_cast<int>(g->blocks.size())) return 0;
+  return g->blocks[block].num_cells;
+}
+
+int tg_geometry_set_fixup_threshold(tg_geometry* g, double tau) {
+  if (!g || !(tau >= 0.0)) return fail(TG_ERR_INVALID_ARGUMENT, "bad geometry or threshold");
+  g->tau = tau;
+  return TG_OK;
+}
+
+int tg_pointset_create(int32_t cell, const double* points, const double* weights, uint64_t nq,
+                       const double* facet_normals, tg_pointset** out) {
+  if (!out) return fail(TG_ERR_INVALID_ARGUMENT, "null output handle");
+  *out = nullptr;
+  if (!tg::valid_cell(cell)) return fail(TG_ERR_INVALID_ARGUMENT, "bad cell type");
+  const int td = tg::tdim(static_cast<tg::Cell>(cell));
+  if (nq > 0 && (!weights || (td > 0 && !points)))
+    return fail(TG_ERR_INVALID_ARGUMENT, "null points or weights");
+  std::vector<double> p(points, points + (points ? nq * td : 0));
+  std::vector<double> w(weights, weights + (weights ? nq : 0));
+  std::vector<double> n;
+  if (facet_normals) n.assign(facet_normals, facet_normals + nq * td);
+  return make_pointset(static_cast<tg::Cell>(cell), std::move(p), std::move(w), std::move(n),
+                       facet_normals != nullptr, out);
+}
+
+int tg_pointset_gauss(int32_t cell, uint32_t degree, tg_pointset** out) {
+  if (!out) return fail(TG_ERR_INVALID_ARGUMENT, "null output handle");
+  if (!tg::valid_cell(cell)) return fail(TG_ERR_INVALID_ARGUMENT, "bad cell type");
+  std::vector<double> p, w;
+  tg::gauss_rule(static_cast<tg::Cell>(cell), degree, p, w);
+  return make_pointset(static_cast<tg::Cell>(cell), std::move(p), std::move(w), {}, false, out);
+}
+
+int tg_pointset_facet_gauss(int32_t cell, uint32_t degree, tg_pointset** out) {
+  if (!out) return fail(TG_ERR_INVALID_ARGUMENT, "null output handle");
+  if (!tg::valid_cell(cell) || cell == TG_POINT)
+    return fail(TG_ERR_INVALID_ARGUMENT, "bad cell type");
+  const tg::Cell c = static_cast<tg::Cell>(cell);
+  const int td = tg::tdim(c);
+  const std::vector<double> fn = tg::reference_facet_normals(c);
+  std::vector<double> pts, wts, nrm;
+  for (int f = 0; f < tg::num_sub_entities(c, td - 1); ++f) {
+    std::vector<double> fp, fw;
+    tg::gauss_rule(tg::sub_entity_type(c, td - 1, f), degree, fp, fw);
+    std::vector<double> mapped = tg::facet_points_to_cell(c, f, fp, fw.size());
+    pts.insert(pts.end(), mapped.begin(), mapped.end());
+    for (size_t q = 0; q < fw.size(); ++q)
+      nrm.insert(nrm.end(), fn.begin() + f * td, fn.begin() + (f + 1) * td);
+    wts.insert(wts.end(), fw.begin(), fw.end());
+  }
+  return make_pointset(c, std::move(pts), std::move(wts), std::move(nrm), true, out);
+}
+
+int tg_pointset_tet_rule(int32_t rule, tg_pointset** out) {
+  if (!out) return fail(TG_ERR_INVALID_ARGUMENT, "null output handle");
+  std::vector<double> p, w;
+  if (rule == 0)
+    tg::tet_rule_keast4(p, w);
+  else if (rule == 1)
+    tg::tet_rule_14(p, w);
+  else
+    return fail(TG_ERR_INVALID_ARGUMENT, "unknown tetrahedron rule");
+  return make_pointset(tg::kTetrahedron, std::move(p), std::move(w), {}, false, out);
+}
+
+int tg_pointset_destroy(tg_pointset* p) {
+  delete p;
+  return TG_OK;
+}
+
+uint64_t tg_pointset_len(const tg_pointset* p) { return p ? p->nq : 0; }
+
+int tg_pointset_get(const tg_pointset* p, double* points, double* weights, double* facet_normals,
+                    uint64_t* num_normals) {
+  if (!p) return fail(TG_ERR_INVALID_ARGUMENT, "null point set");
+  if (points && !p->points.empty())
+    std::memcpy(points, p->points.data(), p->points.size() * sizeof(double));
+  if (weights && !p->weights.empty())
+    std::memcpy(weights, p->weights.data(), p->weights.size() * sizeof(double));
+  const size_t nn = p->has_normals ? p->normals.size() : 0;
+  if (facet_normals && nn) std::memcpy(facet_normals, p->normals.data(), nn * sizeof(double));
+  if (num_normals) *num_normals = nn;
+  return TG_OK;
+}
+
+int tg_pointset_tabulate(const tg_pointset* p, uint32_t degree, int32_t family, double* values,
+                         double* derivatives) {
+  if (!p) return fail(TG_ERR_INVALID_ARGUMENT, "null point set");
+  std::vector<double> inter;
+  size_t k = 0;
+  int rc = host_tables(p, degree, family, inter, &k);
+  if (rc) return rc;
+  for (size_t q = 0; q < p->nq; ++q)
+    for (size_t i = 0; i < k; ++i) {
+      if (values) values[q * k + i] = inter[(q * k + i) * 4];
+      if (derivatives)
+        for (int d = 0; d < p->tdim; ++d)
+          derivatives[(q * k + i) * p->tdim + d] = inter[(q * k + i) * 4 + 1 + d];
+    }
+  return TG_OK;
+}
+
+int tg_eval_block(const tg_geometry* g, int32_t block, tg_pointset* pts, uint32_t factors,
+                  uint32_t tile, uint32_t options, const tg_outputs* out, void* stream) {
+  if (!g || !pts || !out) return fail(TG_ERR_INVALID_ARGUMENT, "null geometry, point set or outputs");
+  if (block < 0 || block >= static_cast<int>(g->blocks.size()))
+    return fail(TG_ERR_INVALID_ARGUMENT, "block index out of range");
+  const Block& b = g->blocks[block];
+  if (pts->cell != b.cell)
+    return fail(TG_ERR_CELL_NOT_IN_RULE, std::string("CellNotInRule(\"") + cell_name(b.cell) + "\")");
+  if (!valid_tile(tile)) return fail(TG_ERR_INVALID_ARGUMENT, "tile must be 1, 2, 4, 8 or 16");
+  const uint32_t all = TG_POINTS | TG_JACOBIAN | TG_DETJ | TG_PINV | TG_FACET_NORMALS | TG_NORMALS;
+  if (factors == 0 || (factors & ~all)) return fail(TG_ERR_INVALID_ARGUMENT, "bad factor mask");
+  if ((factors & TG_NORMALS) && b.tdim + 1 != g->gdim)
+    return fail(TG_ERR_INVALID_ARGUMENT, "normals need a block of codimension 1");
+  if ((factors & TG_FACET_NORMALS) && (b.tdim != g->gdim || !pts->has_normals))
+    return fail(TG_ERR_INVALID_ARGUMENT,
+                "facet normals need a block of codimension 0 and a facet point set");
+  const void* ptrs[6] = {out->points, out->jacobians, out->detj, out->pinv, out->facet_normals,
+                         out->normals};
+  for (int i = 0; i < 6; ++i)
+    if ((factors & (1u << i)) && !ptrs[i])
+      return fail(TG_ERR_INVALID_ARGUMENT, "missing output for a requested factor");
+  if (b.num_cells == 0 || pts->nq == 0) return TG_OK;  // layout.rs:509-511: nothing to launch
+  if (pts->nq > 0xffffffffull) return fail(TG_ERR_TOO_LARGE, "TooLarge: too many points");
+  DeviceTables* tabs = nullptr;
+  int rc = device_tables(pts, b.degree, b.family, g->dtype, &tabs);
+  if (rc) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  return g->dtype == TG_F32 ? eval_typed<float>(g, b, pts, tabs, factors, tile, options, out, s)
+                            : eval_typed<double>(g, b, pts, tabs, factors, tile, options, out, s);
+}
+
+uint64_t tg_evector_elems(const tg_geometry* g, int32_t block) {
+  if (!g || block < 0 || block >= static_cast<int>(g->blocks.size())) return 0;
+  const Block& b = g->blocks[block];
+  return static_cast<uint64_t>(b.k) * g->gdim * ((b.num_cells + 15) / 16 * 16);
+}
+
+int tg_evector_build(const tg_geometry* g, int32_t block, void* out, void* stream) {
+  if (!g || !out) return fail(TG_ERR_INVALID_ARGUMENT, "null geometry or output");
+  if (block < 0 || block >= static_cast<int>(g->blocks.size()))
+    return fail(TG_ERR_INVALID_ARGUMENT, "block index out of range");
+  const Block& b = g->blocks[block];
+  if (b.num_cells == 0) return TG_OK;
+  const uint64_t padded = (b.num_cells + 15) / 16 * 16;
+  const uint32_t last = static_cast<uint32_t>(g->num_nodes - 1);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipError_t e =
+      g->dtype == TG_F32
+          ? tg::launch_evector<float>(static_cast<const float*>(g->nodes), b.dofmap, b.num_cells,
+                                      padded, b.k, g->gdim, last, static_cast<float*>(out), s)
+          : tg::launch_evector<double>(static_cast<const double*>(g->nodes), b.dofmap,
+                                       b.num_cells, padded, b.k, g->gdim, last,
+                                       static_cast<double*>(out), s);
+  if (e != hipSuccess) return hip_fail(e, "evector kernel launch");
+  return TG_OK;
+}
+
+}  // extern "C"

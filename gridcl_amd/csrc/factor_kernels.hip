@@ -1,0 +1,456 @@
+// Factor kernels for gfx950: X, J, det J, K, facet normals and normals of one cell block at a
+// point set, all requested factors in one fused pass.
+//
+// Arithmetic follows the reference's factor kernel (tetraloma-geometry/src/layout.rs:138-369):
+// J(g, d) = sum_k x(k, g) dphi_k/dxi_d, component d * gdim + g, sums starting from the first
+// product; det J by first-row cofactors; the pseudo-determinant, pseudo-inverse, facet normals
+// and normals by the same formulas. The layout of each output is the QuadField tile layout
+// (field.rs:126-134).
+//
+// Mapping: one lane per cell. The lane gathers its cell's node coordinates once through the
+// dofmap into registers (FP32: shifted to the cell's first node, so that the contraction works on
+// offsets of the cell's own size); the basis tables are indexed uniformly across the wavefront,
+// so they are read through the scalar cache; stores are non-temporal. With tiles of 16 cells a
+// wavefront writes four 64-byte runs per component, and consecutive components fill the rest of
+// each tile row. Blocks are remapped so that each XCD walks one contiguous cell range, keeping
+// the nodes shared by neighbouring cells in that XCD's L2.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "geom_internal.hpp"
+
+namespace tg {
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ float tsqrt(float x) { return __builtin_sqrtf(x); }
+__device__ __forceinline__ double tsqrt(double x) { return __builtin_sqrt(x); }
+__device__ __forceinline__ float tfma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double tfma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// Hardware dispatch places workgroup b on XCD b % 8. Give XCD x the contiguous range of logical
+// blocks [x * per + min(x, rem), ...): a bijection of [0, nb).
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
+  const uint32_t x = b & 7u, i = b >> 3;
+  const uint32_t per = nb >> 3, rem = nb & 7u;
+  return x * per + (x < rem ? x : rem) + i;
+}
+
+template <typename T>
+__device__ __forceinline__ void st(void* base, uint64_t idx, T v) {
+  __builtin_nontemporal_store(v, reinterpret_cast<T*>(base) + idx);
+}
+
+// det J (tdim = gdim) or sqrt(det(J^T J)) (layout.rs:186-213).
+template <typename T, int TD, int GD>
+__device__ __forceinline__ T col_dot(const T* j, int a, int b) {
+  T acc = j[a * GD] * j[b * GD];
+#pragma unroll
+  for (int g = 1; g < GD; ++g) acc += j[a * GD + g] * j[b * GD + g];
+  return acc;
+}
+
+template <typename T, int TD, int GD>
+__device__ __forceinline__ T det_of(const T* j) {
+  if constexpr (TD == GD) {
+    if constexpr (TD == 1) {
+      return j[0];
+    } else if constexpr (TD == 2) {
+      return j[0] * j[3] - j[2] * j[1];
+    } else {
+      return j[0] * (j[4] * j[8] - j[7] * j[5]) - j[3] * (j[1] * j[8] - j[7] * j[2]) +
+             j[6] * (j[1] * j[5] - j[4] * j[2]);
+    }
+  } else if constexpr (TD == 1) {
+    return tsqrt(col_dot<T, TD, GD>(j, 0, 0));
+  } else {
+    const T a = col_dot<T, TD, GD>(j, 0, 0), b = col_dot<T, TD, GD>(j, 0, 1),
+            c = col_dot<T, TD, GD>(j, 1, 1);
+    return tsqrt(a * c - b * b);
+  }
+}
+
+// K = (J^T J)^-1 J^T, column-major tdim x gdim (layout.rs:215-268).
+template <typename T, int TD, int GD>
+__device__ __forceinline__ void pinv_of(const T* j, T* k) {
+  if constexpr (TD == GD) {
+    const T inv = T(1) / det_of<T, TD, GD>(j);
+    if constexpr (TD == 1) {
+      k[0] = inv;
+    } else if constexpr (TD == 2) {
+      k[0] = j[3] * inv;
+      k[2] = -j[2] * inv;
+      k[1] = -j[1] * inv;
+      k[3] = j[0] * inv;
+    } else {
+      k[0] = (j[4] * j[8] - j[7] * j[5]) * inv;
+      k[3] = (j[6] * j[5] - j[3] * j[8]) * inv;
+      k[6] = (j[3] * j[7] - j[6] * j[4]) * inv;
+      k[1] = (j[7] * j[2] - j[1] * j[8]) * inv;
+      k[4] = (j[0] * j[8] - j[6] * j[2]) * inv;
+      k[7] = (j[6] * j[1] - j[0] * j[7]) * inv;
+      k[2] = (j[1] * j[5] - j[4] * j[2]) * inv;
+      k[5] = (j[3] * j[2] - j[0] * j[5]) * inv;
+      k[8] = (j[0] * j[4] - j[3] * j[1]) * inv;
+    }
+  } else if constexpr (TD == 1) {
+    const T inv = T(1) / col_dot<T, TD, GD>(j, 0, 0);
+#pragma unroll
+    for (int g = 0; g < GD; ++g) k[g] = j[g] * inv;
+  } else {
+    const T a = col_dot<T, TD, GD>(j, 0, 0), b = col_dot<T, TD, GD>(j, 0, 1),
+            c = col_dot<T, TD, GD>(j, 1, 1);
+    const T inv = T(1) / (a * c - b * b);
+    const T g00 = c * inv, g01 = -b * inv, g11 = a * inv;
+#pragma unroll
+    for (int g = 0; g < GD; ++g) {
+      k[g * TD] = g00 * j[g] + g01 * j[GD + g];
+      k[g * TD + 1] = g01 * j[g] + g11 * j[GD + g];
+    }
+  }
+}
+
+// Stores every factor of J that the mask asks for at point q (layout.rs:275-369).
+template <typename T, int TD, int GD>
+__device__ __forceinline__ void store_from_j(const T* j, T detj, const T* __restrict__ ref_normals,
+                                             const uint32_t* __restrict__ flags, uint64_t c,
+                                             uint32_t q, uint32_t mask, uint64_t row,
+                                             uint32_t tl, uint32_t lane, const FactorOutputs& o) {
+  constexpr int JL = TD * GD;
+  if (mask & kFJac) {
+#pragma unroll
+    for (int i = 0; i < JL; ++i) st<T>(o.j, (((row * JL) + i) << tl) | lane, j[i]);
+  }
+  if (mask & kFDetJ) st<T>(o.detj, (row << tl) | lane, detj);
+  if (mask & (kFPinv | kFFacetNormals)) {
+    T k[JL];
+    pinv_of<T, TD, GD>(j, k);
+    if (mask & kFPinv) {
+#pragma unroll
+      for (int i = 0; i < JL; ++i) st<T>(o.pinv, (((row * JL) + i) << tl) | lane, k[i]);
+    }
+    if constexpr (TD == GD) {
+      if (mask & kFFacetNormals) {
+        T m[GD];
+        T s2 = T(0);
+#pragma unroll
+        for (int g = 0; g < GD; ++g) {
+          T acc = k[g * TD] * ref_normals[q * TD];
+#pragma unroll
+          for (int t = 1; t < TD; ++t) acc += k[g * TD + t] * ref_normals[q * TD + t];
+          m[g] = acc;
+          s2 = g == 0 ? acc * acc : s2 + acc * acc;
+        }
+        const T nrm = tsqrt(s2);
+#pragma unroll
+        for (int g = 0; g < GD; ++g) st<T>(o.fnormals, (((row * GD) + g) << tl) | lane, m[g] / nrm);
+      }
+    }
+  }
+  if constexpr (TD + 1 == GD) {
+    if (mask & kFNormals) {
+      T n[GD];
+      if constexpr (GD == 2) {
+        n[0] = j[1];
+        n[1] = -j[0];
+      } else {
+        n[0] = j[1] * j[5] - j[2] * j[4];
+        n[1] = j[2] * j[3] - j[0] * j[5];
+        n[2] = j[0] * j[4] - j[1] * j[3];
+      }
+      T s2 = n[0] * n[0];
+#pragma unroll
+      for (int g = 1; g < GD; ++g) s2 += n[g] * n[g];
+      const T nrm = tsqrt(s2);
+      const T s = (flags != nullptr && (flags[c] & 1u)) ? T(-1) : T(1);
+#pragma unroll
+      for (int g = 0; g < GD; ++g) st<T>(o.normals, (((row * GD) + g) << tl) | lane, (s * n[g]) / nrm);
+    }
+  }
+}
+
+// FP64 det J of cell c at point q from the stored (FP32) coordinates, for near-degenerate points.
+template <typename T, int TD, int GD>
+__device__ __noinline__ double detj_fp64(const T* __restrict__ nodes,
+                                         const uint32_t* __restrict__ row, uint32_t k,
+                                         uint32_t last_node, const double* __restrict__ tab64,
+                                         uint32_t q) {
+  double x0[GD];
+  uint32_t n0 = row[0] < last_node ? row[0] : last_node;
+#pragma unroll
+  for (int g = 0; g < GD; ++g) x0[g] = static_cast<double>(nodes[static_cast<uint64_t>(n0) * GD + g]);
+  double j[TD * GD];
+#pragma unroll
+  for (int i = 0; i < TD * GD; ++i) j[i] = 0.0;
+  for (uint32_t kk = 1; kk < k; ++kk) {
+    const uint32_t n = row[kk] < last_node ? row[kk] : last_node;
+    const double* t = tab64 + (static_cast<uint64_t>(q) * k + kk) * 4;
+#pragma unroll
+    for (int g = 0; g < GD; ++g) {
+      const double d = static_cast<double>(nodes[static_cast<uint64_t>(n) * GD + g]) - x0[g];
+#pragma unroll
+      for (int dd = 0; dd < TD; ++dd) j[dd * GD + g] = __builtin_fma(d, t[1 + dd], j[dd * GD + g]);
+    }
+  }
+  return det_of<double, TD, GD>(j);
+}
+
+// One lane per cell, K nodes held in registers.
+template <typename T, int TD, int GD, int K, bool SHIFT>
+__global__ void __launch_bounds__(kBlock)
+    factor_direct(const T* __restrict__ nodes, const uint32_t* __restrict__ dofmap,
+                  const uint32_t* __restrict__ flags, const T* __restrict__ tab,
+                  const double* __restrict__ tab64, const T* __restrict__ ref_normals,
+                  FactorParams<T> p, FactorOutputs o) {
+  const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
+  const uint64_t cl = static_cast<uint64_t>(b) * kBlock + threadIdx.x;
+  if (cl >= p.padded_cells) return;
+  const uint64_t c = cl < p.num_cells ? cl : p.num_cells - 1;
+  const uint32_t* __restrict__ row = dofmap + c * K;
+
+  T x[K][GD];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    uint32_t n = row[k];
+    n = n < p.last_node ? n : p.last_node;
+#pragma unroll
+    for (int g = 0; g < GD; ++g) x[k][g] = nodes[static_cast<uint64_t>(n) * GD + g];
+  }
+  if constexpr (SHIFT) {
+#pragma unroll
+    for (int k = 1; k < K; ++k)
+#pragma unroll
+      for (int g = 0; g < GD; ++g) x[k][g] -= x[0][g];
+  }
+  constexpr int K0 = SHIFT ? 1 : 0;  // first node whose offset is not identically zero
+  const uint32_t tl = p.tile_log2;
+  const uint64_t tile = cl >> tl;
+  const uint32_t lane = static_cast<uint32_t>(cl) & ((1u << tl) - 1u);
+  const uint32_t mask = p.mask;
+  const bool need_j = (mask & ~kFPoints) != 0;
+
+  for (uint32_t q = 0; q < p.nq; ++q) {
+    const T* __restrict__ t = tab + static_cast<uint64_t>(q) * K * 4;
+    const uint64_t row_q = tile * p.nq + q;  // (tile * nq + q): times LEN and the component next
+    if (mask & kFPoints) {
+#pragma unroll
+      for (int g = 0; g < GD; ++g) {
+        T acc = x[K0][g] * t[4 * K0];
+#pragma unroll
+        for (int k = K0 + 1; k < K; ++k) acc = tfma(x[k][g], t[4 * k], acc);
+        const T v = SHIFT ? (x[0][g] + acc) + p.origin[g] : acc + p.origin[g];
+        st<T>(o.x, (((row_q * GD) + g) << tl) | lane, v);
+      }
+    }
+    if (need_j) {
+      T j[TD * GD];
+#pragma unroll
+      for (int d = 0; d < TD; ++d)
+#pragma unroll
+        for (int g = 0; g < GD; ++g) {
+          T acc = x[K0][g] * t[4 * K0 + 1 + d];
+#pragma unroll
+          for (int k = K0 + 1; k < K; ++k) acc = tfma(x[k][g], t[4 * k + 1 + d], acc);
+          j[d * GD + g] = acc;
+        }
+      T detj = T(0);
+      if (mask & kFDetJ) {
+        detj = det_of<T, TD, GD>(j);
+        if constexpr (TD == GD && sizeof(T) == 4) {
+          if (p.fixup) {
+            T n2 = T(0);
+#pragma unroll
+            for (int i = 0; i < TD * GD; ++i) n2 = tfma(j[i], j[i], n2);
+            if (__builtin_fabsf(detj) < static_cast<T>(p.tau) * n2 * tsqrt(n2))
+              detj = static_cast<T>(
+                  detj_fp64<T, TD, GD>(nodes, row, K, p.last_node, tab64, q));
+          }
+        }
+      }
+      store_from_j<T, TD, GD>(j, detj, ref_normals, flags, c, q, mask, row_q, tl, lane, o);
+    }
+  }
+}
+
+// Runtime node count (the reference's `loops` strategy, layout_loops.rs:68-108): node
+// coordinates are re-read through the dofmap at every point. Any element.
+template <typename T, int TD, int GD, bool SHIFT>
+__global__ void __launch_bounds__(kBlock)
+    factor_generic(const T* __restrict__ nodes, const uint32_t* __restrict__ dofmap,
+                   const uint32_t* __restrict__ flags, const T* __restrict__ tab,
+                   const double* __restrict__ tab64, const T* __restrict__ ref_normals,
+                   FactorParams<T> p, FactorOutputs o) {
+  const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
+  const uint64_t cl = static_cast<uint64_t>(b) * kBlock + threadIdx.x;
+  if (cl >= p.padded_cells) return;
+  const uint64_t c = cl < p.num_cells ? cl : p.num_cells - 1;
+  const uint32_t K = p.k;
+  const uint32_t* __restrict__ row = dofmap + c * K;
+  T x0[GD];
+  {
+    const uint32_t n = row[0] < p.last_node ? row[0] : p.last_node;
+#pragma unroll
+    for (int g = 0; g < GD; ++g) x0[g] = nodes[static_cast<uint64_t>(n) * GD + g];
+  }
+  const uint32_t k0 = SHIFT ? 1u : 0u;
+  const uint32_t tl = p.tile_log2;
+  const uint64_t tile = cl >> tl;
+  const uint32_t lane = static_cast<uint32_t>(cl) & ((1u << tl) - 1u);
+  const uint32_t mask = p.mask;
+  const bool need_j = (mask & ~kFPoints) != 0;
+
+  for (uint32_t q = 0; q < p.nq; ++q) {
+    const T* __restrict__ tq = tab + static_cast<uint64_t>(q) * K * 4;
+    T xs[GD];
+    T j[TD * GD];
+    for (uint32_t k = k0; k < K; ++k) {
+      const uint32_t n = row[k] < p.last_node ? row[k] : p.last_node;
+      T xk[GD];
+#pragma unroll
+      for (int g = 0; g < GD; ++g) {
+        xk[g] = nodes[static_cast<uint64_t>(n) * GD + g];
+        if (SHIFT) xk[g] -= x0[g];
+      }
+      const T* t = tq + 4 * k;
+      if (k == k0) {
+#pragma unroll
+        for (int g = 0; g < GD; ++g) {
+          xs[g] = xk[g] * t[0];
+#pragma unroll
+          for (int d = 0; d < TD; ++d) j[d * GD + g] = xk[g] * t[1 + d];
+        }
+      } else {
+#pragma unroll
+        for (int g = 0; g < GD; ++g) {
+          xs[g] = tfma(xk[g], t[0], xs[g]);
+#pragma unroll
+          for (int d = 0; d < TD; ++d) j[d * GD + g] = tfma(xk[g], t[1 + d], j[d * GD + g]);
+        }
+      }
+    }
+    const uint64_t row_q = tile * p.nq + q;
+    if (mask & kFPoints) {
+#pragma unroll
+      for (int g = 0; g < GD; ++g) {
+        const T v = SHIFT ? (x0[g] + xs[g]) + p.origin[g] : xs[g] + p.origin[g];
+        st<T>(o.x, (((row_q * GD) + g) << tl) | lane, v);
+      }
+    }
+    if (need_j) {
+      T detj = T(0);
+      if (mask & kFDetJ) {
+        detj = det_of<T, TD, GD>(j);
+        if constexpr (TD == GD && sizeof(T) == 4) {
+          if (p.fixup) {
+            T n2 = T(0);
+#pragma unroll
+            for (int i = 0; i < TD * GD; ++i) n2 = tfma(j[i], j[i], n2);
+            if (__builtin_fabsf(detj) < static_cast<T>(p.tau) * n2 * tsqrt(n2))
+              detj = static_cast<T>(detj_fp64<T, TD, GD>(nodes, row, K, p.last_node, tab64, q));
+          }
+        }
+      }
+      store_from_j<T, TD, GD>(j, detj, ref_normals, flags, c, q, mask, row_q, tl, lane, o);
+    }
+  }
+}
+
+// E-vector gather (evector.rs:197-236): out[(k * gdim + g) * padded + cell].
+template <typename T>
+__global__ void __launch_bounds__(kBlock)
+    evector_kernel(const T* __restrict__ nodes, const uint32_t* __restrict__ dofmap,
+                   uint64_t num_cells, uint64_t padded, uint32_t k, int gdim, uint32_t last_node,
+                   T* __restrict__ out) {
+  const uint64_t cl = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (cl >= padded) return;
+  const uint64_t c = cl < num_cells ? cl : num_cells - 1;
+  for (uint32_t kk = 0; kk < k; ++kk) {
+    uint32_t n = dofmap[c * k + kk];
+    n = n < last_node ? n : last_node;
+    for (int g = 0; g < gdim; ++g)
+      out[(static_cast<uint64_t>(kk) * gdim + g) * padded + cl] =
+          nodes[static_cast<uint64_t>(n) * gdim + g];
+  }
+}
+
+template <typename T>
+using FactorFn = void (*)(const T*, const uint32_t*, const uint32_t*, const T*, const double*,
+                          const T*, FactorParams<T>, FactorOutputs);
+
+// Register-resident kernels exist for these (tdim, gdim, k): degrees 1-3 of every cell (tets,
+// hexes up to Q3, prisms and pyramids up to degree 2, surfaces and curves up to degree 3).
+#define TG_DIRECT_SHAPES(X)                                                                     \
+  X(3, 3, 4) X(3, 3, 10) X(3, 3, 20) X(3, 3, 8) X(3, 3, 27) X(3, 3, 64) X(3, 3, 6) X(3, 3, 18) \
+  X(3, 3, 5) X(3, 3, 14) X(2, 2, 3) X(2, 2, 6) X(2, 2, 10) X(2, 2, 4) X(2, 2, 9) X(2, 2, 16)   \
+  X(2, 3, 3) X(2, 3, 6) X(2, 3, 10) X(2, 3, 4) X(2, 3, 9) X(2, 3, 16) X(1, 1, 2) X(1, 1, 3)    \
+  X(1, 1, 4) X(1, 2, 2) X(1, 2, 3) X(1, 2, 4) X(1, 3, 2) X(1, 3, 3) X(1, 3, 4)
+
+template <typename T>
+FactorFn<T> direct_kernel(int td, int gd, uint32_t k, bool shift) {
+  // FP64 keeps at most 27 nodes in registers; larger elements use the runtime-K kernel.
+  if (sizeof(T) == 8 && k > 27) return nullptr;
+#define TG_CASE(TD_, GD_, K_)                                                   \
+  if (td == TD_ && gd == GD_ && k == K_)                                        \
+    return shift ? static_cast<FactorFn<T>>(factor_direct<T, TD_, GD_, K_, true>) \
+                 : static_cast<FactorFn<T>>(factor_direct<T, TD_, GD_, K_, false>);
+  if (sizeof(T) == 4 && !shift) return nullptr;  // F32 without the shift runs generic
+  TG_DIRECT_SHAPES(TG_CASE)
+#undef TG_CASE
+  return nullptr;
+}
+
+template <typename T>
+FactorFn<T> generic_kernel(int td, int gd, bool shift) {
+#define TG_G(TD_, GD_)                                                              \
+  if (td == TD_ && gd == GD_)                                                       \
+    return shift ? static_cast<FactorFn<T>>(factor_generic<T, TD_, GD_, true>)      \
+                 : static_cast<FactorFn<T>>(factor_generic<T, TD_, GD_, false>);
+  TG_G(1, 1) TG_G(1, 2) TG_G(1, 3) TG_G(2, 2) TG_G(2, 3) TG_G(3, 3)
+#undef TG_G
+  return nullptr;
+}
+
+}  // namespace
+
+template <typename T>
+hipError_t launch_factor(int tdim, int gdim, bool shift, bool generic, const T* nodes,
+                         const uint32_t* dofmap, const uint32_t* flags, const T* tab,
+                         const double* tab64, const T* ref_normals, const FactorParams<T>& p,
+                         const FactorOutputs& out, hipStream_t stream) {
+  FactorFn<T> fn = generic ? nullptr : direct_kernel<T>(tdim, gdim, p.k, shift);
+  if (fn == nullptr) fn = generic_kernel<T>(tdim, gdim, shift);
+  if (fn == nullptr) return hipErrorNotSupported;
+  const uint64_t blocks = (p.padded_cells + kBlock - 1) / kBlock;
+  if (blocks > 0x7fffffffu) return hipErrorInvalidConfiguration;
+  hipLaunchKernelGGL(fn, dim3(static_cast<uint32_t>(blocks)), dim3(kBlock), 0, stream, nodes,
+                     dofmap, flags, tab, tab64, ref_normals, p, out);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_evector(const T* nodes, const uint32_t* dofmap, uint64_t num_cells,
+                          uint64_t padded, uint32_t k, int gdim, uint32_t last_node, T* out,
+                          hipStream_t stream) {
+  const uint64_t blocks = (padded + kBlock - 1) / kBlock;
+  if (blocks > 0x7fffffffu) return hipErrorInvalidConfiguration;
+  hipLaunchKernelGGL(evector_kernel<T>, dim3(static_cast<uint32_t>(blocks)), dim3(kBlock), 0,
+                     stream, nodes, dofmap, num_cells, padded, k, gdim, last_node, out);
+  return hipGetLastError();
+}
+
+template hipError_t launch_factor<float>(int, int, bool, bool, const float*, const uint32_t*,
+                                         const uint32_t*, const float*, const double*,
+                                         const float*, const FactorParams<float>&,
+                                         const FactorOutputs&, hipStream_t);
+template hipError_t launch_factor<double>(int, int, bool, bool, const double*, const uint32_t*,
+                                          const uint32_t*, const double*, const double*,
+                                          const double*, const FactorParams<double>&,
+                                          const FactorOutputs&, hipStream_t);
+template hipError_t launch_evector<float>(const float*, const uint32_t*, uint64_t, uint64_t,
+                                          uint32_t, int, uint32_t, float*, hipStream_t);
+template hipError_t launch_evector<double>(const double*, const uint32_t*, uint64_t, uint64_t,
+                                           uint32_t, int, uint32_t, double*, hipStream_t);
+
+}  // namespace tg

@@ -1,0 +1,190 @@
+/* C ABI of the MI355X geometry library: the degree-p Lagrange geometry map of a mesh's cell
+ * blocks evaluated at a point set (physical points X, Jacobians J, det J, pseudo-inverses K,
+ * normals, facet normals), with the field layout, error variants and index-clamping semantics of
+ * the reference's tetraloma-geometry crate.
+ *
+ * Conventions kept from the reference (paths relative to the reference repository):
+ *   - node coordinates: [node][gdim] in the geometry's scalar type, relative to an f64 origin that
+ *     the X outputs add back (tetraloma-geometry/src/geometry.rs:14-19, 139-142; layout.rs:425);
+ *   - dofmaps: u32 [cell][k], node order A.5 (tetraloma-refcell/src/lagrange.rs:134-266); indices
+ *     past the last node are clamped to it, not rejected (evector.rs:187-191, tests/bounds.rs);
+ *   - cell flags: u32 per cell, bit 0 flips the normal of a codimension-1 cell (layout.rs:351-366);
+ *   - outputs: QuadField tile layout, component k at point q of cell c at
+ *       ((c / tile) * nq * LEN + q * LEN + k) * tile + c % tile,
+ *     cells padded to whole tiles, padding lanes holding the last cell's values
+ *     (field.rs:126-134, evector.rs:197-236). J is column-major (component d * gdim + g), K is
+ *     column-major tdim x gdim (component g * tdim + t) (shape.rs; tetraloma-testing reference.rs:12-13).
+ *
+ * All device pointers are HIP device allocations owned by the caller unless stated otherwise.
+ * Every evaluation call is asynchronous on the given stream (a hipStream_t; NULL = the null
+ * stream) and synchronises nothing. Handles are thread-compatible, not thread-safe, like the
+ * reference's Geometry and PointSet (tetraloma-geometry/src/testing.rs:5-6). */
+#ifndef TETRALOMA_GEOM_H
+#define TETRALOMA_GEOM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes; one per tetraloma_device::GridError variant this path can return
+ * (tetraloma-device/src/error.rs:80-150), plus argument and HIP runtime errors. */
+typedef enum tg_status {
+  TG_OK = 0,
+  TG_ERR_LENGTH_MISMATCH = 1,      /* GridError::LengthMismatch {expected, found} */
+  TG_ERR_TOO_LARGE = 2,            /* GridError::TooLarge */
+  TG_ERR_ALLOCATION_TOO_LARGE = 3, /* GridError::AllocationTooLarge {bytes, max} */
+  TG_ERR_UNSUPPORTED_DEGREE = 4,   /* GridError::UnsupportedDegree {cell, degree} */
+  TG_ERR_UNSUPPORTED_NODE_FAMILY = 5, /* GridError::UnsupportedNodeFamily {cell, family} */
+  TG_ERR_CELL_NOT_IN_RULE = 6,     /* GridError::CellNotInRule(cell) */
+  TG_ERR_UNSUPPORTED_PRECISION = 7, /* GridError::UnsupportedPrecision {scalar, device} */
+  TG_ERR_INVALID_ARGUMENT = 8,     /* null handle or pointer, bad tile, block index out of range */
+  TG_ERR_HIP = 9                   /* GridError::Server(..): a HIP runtime call failed */
+} tg_status;
+
+/* tetraloma_dims::CellType, in its order (tetraloma-dims/src/lib.rs). */
+typedef enum tg_cell {
+  TG_POINT = 0,
+  TG_INTERVAL = 1,
+  TG_TRIANGLE = 2,
+  TG_QUADRILATERAL = 3,
+  TG_TETRAHEDRON = 4,
+  TG_HEXAHEDRON = 5,
+  TG_PRISM = 6,
+  TG_PYRAMID = 7
+} tg_cell;
+
+/* Scalar type of a geometry: its node coordinates and every field it produces. */
+typedef enum tg_dtype { TG_F32 = 0, TG_F64 = 1 } tg_dtype;
+
+/* tetraloma_refcell::NodeFamily (lagrange.rs:15-27). */
+typedef enum tg_family { TG_EQUISPACED = 0, TG_GLL = 1 } tg_family;
+
+/* Factors (layout.rs:86-101, Factor). Several may be requested in one call: they are computed in
+ * one fused pass. */
+enum {
+  TG_POINTS = 1u << 0,        /* X, LEN gdim */
+  TG_JACOBIAN = 1u << 1,      /* J, LEN tdim*gdim */
+  TG_DETJ = 1u << 2,          /* det J, or sqrt(det(J^T J)) when tdim < gdim; LEN 1 */
+  TG_PINV = 1u << 3,          /* K = (J^T J)^-1 J^T; LEN tdim*gdim */
+  TG_FACET_NORMALS = 1u << 4, /* K^T n_ref / |K^T n_ref| (codimension 0); LEN gdim */
+  TG_NORMALS = 1u << 5        /* s (J0 x J1) / |J0 x J1| (codimension 1); LEN gdim */
+};
+
+/* Evaluation options. */
+enum {
+  /* F32 geometries: contract node coordinates in the reference's order from node 0 instead of
+   * shifting them to the cell's first node (the shift is the default). */
+  TG_OPT_NO_CELL_SHIFT = 1u << 0,
+  /* F32 geometries: skip the FP64 recomputation of det J at near-degenerate points. */
+  TG_OPT_NO_FP64_FIXUP = 1u << 1,
+  /* Force the generic runtime-K kernel (the reference's `loops` strategy) for tests. */
+  TG_OPT_FORCE_GENERIC = 1u << 2
+};
+
+typedef struct tg_geometry tg_geometry;
+typedef struct tg_pointset tg_pointset;
+
+/* One cell block of a geometry (tetraloma-geometry/src/element.rs:30-70, GeometryBlock). */
+typedef struct tg_block_desc {
+  int32_t cell;            /* tg_cell */
+  uint32_t degree;         /* Lagrange degree p >= 1 */
+  int32_t family;          /* tg_family */
+  const uint32_t* dofmap;  /* device, [num_cells][k], k = number of Lagrange nodes */
+  uint64_t dofmap_len;     /* entries in dofmap; must be a multiple of k */
+  uint32_t dofmap_stride;  /* must equal k (geometry.rs:106-150) */
+  const uint32_t* flags;   /* device, [num_cells] or NULL (all zero) */
+} tg_block_desc;
+
+/* Output pointers of one evaluation; NULL for a factor not requested. Each must hold
+ * tg_field_elems(num_cells, nq, LEN, tile) elements of the geometry's scalar type. */
+typedef struct tg_outputs {
+  void* points;
+  void* jacobians;
+  void* detj;
+  void* pinv;
+  void* facet_normals;
+  void* normals;
+} tg_outputs;
+
+/* Text of the last error on this thread ("" if none). */
+const char* tg_last_error_message(void);
+/* Library version string. */
+const char* tg_version(void);
+
+/* Elements of a QuadField of num_cells cells, nq points, len components, in tiles of `tile`
+ * cells (field.rs:234-253: cells padded to whole tiles). 0 if tile is not 1, 2, 4, 8 or 16. */
+uint64_t tg_field_elems(uint64_t num_cells, uint64_t nq, uint32_t len, uint32_t tile);
+
+/* Number of Lagrange nodes of `cell` at `degree` (tetraloma-refcell/src/lagrange.rs:59). */
+uint64_t tg_num_lagrange_nodes(int32_t cell, uint32_t degree);
+
+/* Lagrange nodes in A.5 order, [k][tdim], into `out` (host) of capacity `cap` doubles
+ * (lagrange.rs:163). Writes the number of doubles to *len. */
+int tg_lagrange_points(int32_t cell, uint32_t degree, int32_t family, double* out, size_t cap,
+                       size_t* len);
+
+/* Geometry over caller-owned device arrays (geometry.rs:106 Geometry::new). `nodes` is
+ * [num_nodes][gdim] in `dtype`, relative to `origin` (gdim f64 values, host). The geometry keeps
+ * the pointers; they must outlive it. Checks: every block's dofmap_stride equals its node count
+ * and dofmap_len is a multiple of it (LengthMismatch), supported cell/degree/family. */
+int tg_geometry_create(int32_t dtype, int32_t gdim, const void* nodes, uint64_t num_nodes,
+                       const double* origin, const tg_block_desc* blocks, int32_t num_blocks,
+                       tg_geometry** out);
+int tg_geometry_destroy(tg_geometry* g);
+int tg_geometry_num_blocks(const tg_geometry* g);
+uint64_t tg_geometry_block_cells(const tg_geometry* g, int32_t block);
+/* Threshold tau of the FP64 fix-up: points with |det J| / |J|_F^3 < tau get det J recomputed in
+ * FP64 (F32 geometries). Default 1e-2. */
+int tg_geometry_set_fixup_threshold(tg_geometry* g, double tau);
+
+/* A point set of `cell` with points [nq][tdim] and weights [nq] (host f64), optionally with the
+ * outward reference normal of each point's facet [nq][tdim] (a FacetPointSet,
+ * facet_rule.rs:39-92). Tables are tabulated in f64 and uploaded on first use per element. */
+int tg_pointset_create(int32_t cell, const double* points, const double* weights, uint64_t nq,
+                       const double* facet_normals, tg_pointset** out);
+/* QuadRule::gauss's point set for `cell` at `degree` (rule.rs:79; quadrature.rs:77). */
+int tg_pointset_gauss(int32_t cell, uint32_t degree, tg_pointset** out);
+/* FacetQuadRule::gauss's point set (facet_rule.rs:51): a Gauss rule of `degree` on every facet,
+ * mapped into the cell, facet by facet, with the facets' outward reference normals. */
+int tg_pointset_facet_gauss(int32_t cell, uint32_t degree, tg_pointset** out);
+/* Tetrahedron rules not in the reference: rule 0 = 4-point degree 2 (Keast), 1 = 14-point
+ * degree 5 (Walkington). */
+int tg_pointset_tet_rule(int32_t rule, tg_pointset** out);
+int tg_pointset_destroy(tg_pointset* p);
+uint64_t tg_pointset_len(const tg_pointset* p);
+/* Copies points ([nq][tdim]), weights ([nq]) and, for a facet point set, the reference facet
+ * normals ([nq][tdim]) to host arrays (any may be NULL). Returns the number of facet-normal
+ * values copied through *num_normals if that is not NULL (0 for a plain point set). */
+int tg_pointset_get(const tg_pointset* p, double* points, double* weights, double* facet_normals,
+                    uint64_t* num_normals);
+/* Host copies of the tables of element (cell of p, degree, family) at p's points, in f64:
+ * values [nq][k] and first derivatives [nq][k][tdim] (point_set.rs:312, 350). */
+int tg_pointset_tabulate(const tg_pointset* p, uint32_t degree, int32_t family, double* values,
+                         double* derivatives);
+
+/* The fused factor evaluation of block `block` at the points of `pts`: every factor in
+ * `factors` (TG_POINTS | TG_JACOBIAN | ...) written to the matching pointer of `out` in the
+ * QuadField layout with tiles of `tile` cells. Replaces one launch_factor call per factor
+ * (tetraloma-geometry/src/layout.rs:488-772) and so physical_points_block,
+ * jacobians_block (points_jac.rs:42, 90), detj_block, pseudo_inverse_block (detj_pinv.rs:42,
+ * 90), facet_normals_block and normals_block (facets.rs:46, 107). An empty block or point set
+ * launches nothing (layout.rs:509-511). Errors: CellNotInRule if pts is of another cell type,
+ * InvalidArgument for a missing output, a factor that does not apply to the block's dimensions
+ * (normals need tdim = gdim - 1, facet normals tdim = gdim and a facet point set) or a bad tile. */
+int tg_eval_block(const tg_geometry* g, int32_t block, tg_pointset* pts, uint32_t factors,
+                  uint32_t tile, uint32_t options, const tg_outputs* out, void* stream);
+
+/* Gathers the E-vector of a block: [k][gdim][padded cells] with padded = cells rounded up to 16,
+ * padding columns copying the last cell, coordinates relative to the origin (evector.rs:197-236,
+ * Geometry::evector geometry.rs:324). `out` holds k * gdim * padded scalars. */
+int tg_evector_build(const tg_geometry* g, int32_t block, void* out, void* stream);
+uint64_t tg_evector_elems(const tg_geometry* g, int32_t block);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TETRALOMA_GEOM_H */

@@ -1,0 +1,75 @@
+/* FP64 CPU oracle of the geometry hot path. TEST INFRASTRUCTURE ONLY: tests, the smoke check and
+ * the CPU-baseline leg of the benchmark call it as a checker or as the timed CPU port; the
+ * library under gridcl_amd/ never links, loads or calls it.
+ *
+ * It restates, in plain C, the reference's f64 host reference and CPU factor path:
+ *   - reference cells and the A.5 Lagrange node order
+ *     (tetraloma-refcell/src/cell.rs:53-193, lagrange.rs:163-266);
+ *   - Lagrange bases in closed form (independent of the library's Vandermonde construction):
+ *     Silvester's product formula on equispaced simplices, products of 1D Lagrange polynomials on
+ *     intervals, quadrilaterals, hexahedra (equispaced or GLL) and prisms (triangle x interval),
+ *     the closed-form rational basis of the degree-1 pyramid, and a long-double Vandermonde solve
+ *     in the pyramid's rational space x^a y^b z^c/(1-z)^min(a,b) for higher pyramids
+ *     (polynomials/pyramid.rs:1-20);
+ *   - X, J, det J, K, normals, facet normals per cell and point with the reference kernel's
+ *     formulas and operation order: sums start from the node-0 product and run in ascending
+ *     node order (tetraloma-testing/src/reference.rs:117-193, 258-354;
+ *     tetraloma-geometry/src/layout.rs:138-369; tetraloma/tests/common/reference.rs:8-22);
+ *   - the E-vector gather with index clamping (tetraloma-geometry/src/evector.rs:197-236);
+ *   - the block error metric rel_err = max|got - want| / max|want| (reference.rs:776-817);
+ *   - the mesh functional of the adjoint and its FP64 gradient with respect to node
+ *     coordinates (no reference counterpart; pinned by finite differences and closed forms).
+ *
+ * Pinning: node layouts against the reference's lagrange_points.json / tables.json fixtures;
+ * bases by the Kronecker property, partition of unity and polynomial reproduction; factors by
+ * affine closed forms, unit-volume sums, K J = I and the reference's doc-test examples; the
+ * pyramid of degree >= 2 (Vandermonde path) by Kronecker/partition/reproduction only. */
+#ifndef GRIDCL_ORACLE_H
+#define GRIDCL_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+int or_tdim(int cell);
+size_t or_num_nodes(int cell, int degree);
+/* A.5 nodes [k][tdim]; returns the node count or -1. family 0 = equispaced, 1 = GLL. */
+long or_lagrange_points(int cell, int degree, int family, double* out);
+/* Values [q][k] and derivatives [q][k][tdim] (either may be NULL). Returns 0 or -1. */
+int or_tabulate(int cell, int degree, int family, const double* pts, size_t nq, double* values,
+                double* derivs);
+
+/* Factors of one block, cell-major outputs [c][q][LEN] (any may be NULL):
+ * nodes [n][gdim] absolute f64 coordinates; dofmap [c][k] (indices clamped to num_nodes - 1);
+ * flags [c] or NULL; ref_normals [q][tdim] or NULL. threads <= 0: all OpenMP threads. */
+int or_eval_block(int cell, int degree, int family, int gdim, const double* nodes,
+                  size_t num_nodes, const uint32_t* dofmap, size_t num_cells,
+                  const uint32_t* flags, const double* pts, size_t nq, const double* ref_normals,
+                  double* x, double* j, double* detj, double* pinv, double* fnormals,
+                  double* normals, int threads);
+
+/* The CPU path timed as the baseline: E-vector gather (evector.rs:197-236) into [k][g][cells]
+ * then X, J and det J with the direct kernel's order into QuadField tiles of `tile` cells, all
+ * in f64, OpenMP over tiles. nodes are relative to `origin`. Returns 0. */
+int or_cpu_factors(int tdim, int gdim, int k, const double* nodes, size_t num_nodes,
+                   const double* origin, const uint32_t* dofmap, size_t num_cells,
+                   const double* values, const double* derivs, size_t nq, int tile,
+                   double* evector, double* x, double* j, double* detj, int threads);
+
+/* Functional F = sum_c sum_q w_q det J + beta sum_c sum_q w_q |J|_F^2 / det(J)^(2/3) over a
+ * 3D block, in f64, and its gradient dF/dX [n][3] (grad may be NULL). Returns F. */
+double or_functional(int cell, int degree, int family, const double* nodes, size_t num_nodes,
+                     const uint32_t* dofmap, size_t num_cells, const double* pts,
+                     const double* wts, size_t nq, double beta, double* grad, int threads);
+
+/* rel_err = max|got - want| / max|want| over n values (reference.rs:776-817). */
+double or_rel_err(const double* got, const double* want, size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

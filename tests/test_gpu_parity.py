@@ -1,0 +1,358 @@
+"""Parity of the HIP factor kernels with the FP64 oracle (needs the GPU).
+
+The oracle evaluates the coordinates the geometry stores, widened to f64 (the reference's host
+reference does the same: tetraloma-testing/src/reference.rs:47-69). Errors are the reference's
+block metric rel_err = max|got - want| / max|want| (reference.rs:776-817).
+
+Tolerances, written per test: FP64 1e-12 (the reference's f64 base tolerance,
+tetraloma-testing/src/lib.rs TOL_F64). FP32 with the per-cell shift: 1e-6 for X, J and det J of
+the register-resident kernels (degrees 1-3) - the accuracy target of this path - and the
+reference's f32 base tolerance 1e-5 for K, normals and the runtime-K kernel of higher degrees.
+"""
+import numpy as np
+import pytest
+
+from gridcl_amd import api
+from gridcl_amd import meshes as M
+from tests.helpers import gpu_factors, oracle_factors, pointwise_rel, rel_errs
+
+pytestmark = pytest.mark.gpu
+
+TET, HEX, PRISM, PYR, TRI, QUAD, INTERVAL = 4, 5, 6, 7, 2, 3, 1
+
+
+def line_mesh(n, p, gdim, seed=5):
+    """n interval cells of degree p along a smooth curve in gdim dimensions (A.5: the two
+    vertices, then the interior nodes)."""
+    t = np.linspace(0.0, 1.0, n * p + 1)
+    cells = []
+    for c in range(n):
+        v0, v1 = c * p, (c + 1) * p
+        cells.append([v0, v1] + list(range(v0 + 1, v1)))
+    x = np.zeros((len(t), gdim))
+    x[:, 0] = t
+    if gdim > 1:
+        x[:, 1] = 0.2 * np.sin(np.pi * t)
+    if gdim > 2:
+        x[:, 2] = 0.1 * np.cos(np.pi * t)
+    m = M.HostMesh(gdim, x, [M.HostBlock(INTERVAL, p, np.array(cells, dtype=np.uint32))])
+    return M.jitter(m, 0.05 / (n * p), seed, interior_only=False)
+
+
+def volume_mesh(cell, p, n=3, seed=11):
+    m = M.cube_box(cell, n, p)
+    return M.jitter(m, 0.1 / (n * p), seed, interior_only=True)
+
+
+def surface_mesh(cell, p, gdim, n=4, seed=12):
+    m = M.square_box(cell, n, p, gdim=gdim)
+    if gdim == 3:
+        x = m.nodes
+        x[:, 2] = 0.15 * np.sin(np.pi * x[:, 0]) * np.sin(np.pi * x[:, 1])
+    m = M.jitter(m, 0.05 / (n * p), seed, interior_only=True)
+    b = m.blocks[0]
+    b.flags = (np.arange(b.num_cells) % 3 == 0).astype(np.uint32)
+    return m
+
+
+CASES = (
+    [("vol", TET, p) for p in (1, 2, 3, 4, 5)]
+    + [("vol", HEX, p) for p in (1, 2, 3, 4)]
+    + [("vol", PRISM, p) for p in (1, 2, 3)]
+    + [("vol", PYR, p) for p in (1, 2, 3)]
+    + [("surf2", TRI, p) for p in (1, 2, 3, 5)]
+    + [("surf2", QUAD, p) for p in (1, 2, 3, 4)]
+    + [("surf3", TRI, p) for p in (1, 2, 3, 5)]
+    + [("surf3", QUAD, p) for p in (1, 2, 3)]
+    + [("line1", INTERVAL, p) for p in (1, 2, 3, 5)]
+    + [("line2", INTERVAL, p) for p in (1, 3)]
+    + [("line3", INTERVAL, p) for p in (2, 3)]
+)
+
+
+def make_case(kind, cell, p):
+    if kind == "vol":
+        return volume_mesh(cell, p)
+    if kind == "surf2":
+        return surface_mesh(cell, p, 2)
+    if kind == "surf3":
+        return surface_mesh(cell, p, 3)
+    return line_mesh(7, p, int(kind[-1]))
+
+
+def factor_names(td, gd):
+    f = ["points", "jacobians", "detj", "pinv"]
+    if td + 1 == gd:
+        f.append("normals")
+    return f
+
+
+DIRECT_K = {(3, 3): {4, 10, 20, 8, 27, 64, 6, 18, 5, 14}, (2, 2): {3, 6, 10, 4, 9, 16},
+            (2, 3): {3, 6, 10, 4, 9, 16}, (1, 1): {2, 3, 4}, (1, 2): {2, 3, 4}, (1, 3): {2, 3, 4}}
+
+
+@pytest.mark.parametrize("kind,cell,p", CASES)
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_factors_match_oracle(cuda, kind, cell, p, dtype):
+    import torch
+
+    mesh = make_case(kind, cell, p)
+    td = M.TDIM[cell]
+    gd = mesh.gdim
+    tdt = torch.float64 if dtype == "f64" else torch.float32
+    g = api.geometry_from_host(mesh, dtype=tdt)
+    ps = api.PointSet.gauss(cell, 2 * p)
+    names = factor_names(td, gd)
+    got = gpu_factors(g, 0, ps, names)
+    want = oracle_factors(g, mesh, 0, ps, names)
+    err = rel_errs(got, want)
+    k = mesh.blocks[0].nodes_per_cell
+    direct = k in DIRECT_K[(td, gd)] and (dtype == "f32" or k <= 27)
+    print(f"{kind} cell={cell} p={p} {dtype} direct={direct} rel_err={err}")
+    if dtype == "f64":
+        for name, e in err.items():
+            assert e <= 1e-12, (name, e)
+    else:
+        base = 1e-6 if direct else 1e-5
+        for name in ("points", "jacobians", "detj"):
+            assert err[name] <= base, (name, err[name])
+        for name in ("pinv", "normals"):
+            if name in err:
+                assert err[name] <= 1e-5, (name, err[name])
+
+
+@pytest.mark.parametrize("cell,p", [(TET, 1), (TET, 2), (TET, 3), (HEX, 2), (PRISM, 2),
+                                    (PYR, 1), (HEX, 3)])
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_generic_kernel_matches_direct(cuda, cell, p, dtype):
+    """The runtime-K kernel and the register-resident kernel agree (the reference checks every
+    strategy against the same tolerance, tests/common/mod.rs:634-647)."""
+    import torch
+
+    mesh = volume_mesh(cell, p)
+    tdt = torch.float64 if dtype == "f64" else torch.float32
+    g = api.geometry_from_host(mesh, dtype=tdt)
+    ps = api.PointSet.gauss(cell, 2 * p)
+    names = ["points", "jacobians", "detj"]
+    a = gpu_factors(g, 0, ps, names)
+    b = gpu_factors(g, 0, ps, names, options=api.OPT_FORCE_GENERIC)
+    for n in names:
+        tol = 1e-14 if dtype == "f64" else 1e-6
+        assert np.abs(a[n] - b[n]).max() <= tol * max(1.0, np.abs(a[n]).max()), n
+
+
+@pytest.mark.parametrize("tile", [1, 2, 4, 8, 16])
+def test_tiles_and_padding(cuda, tile):
+    """Every tile width gives the same values; padding lanes hold the last cell's values
+    (field.rs:126-134; evector.rs:197-236 pads with copies of the last cell)."""
+    import torch
+
+    mesh = volume_mesh(TET, 2, n=2)  # 48 cells: padded for tile 16 to 48, so cut to 45 below
+    mesh.blocks[0].cells = mesh.blocks[0].cells[:45]
+    g = api.geometry_from_host(mesh, dtype=torch.float32)
+    ps = api.PointSet.tet_14()
+    ref = gpu_factors(g, 0, ps, ["points", "jacobians", "detj"], tile=1)
+    fields = api.geometric_factors_block(g, 0, ps, ["points", "jacobians", "detj"], tile=tile)
+    torch.cuda.synchronize()
+    for name, f in fields.items():
+        host = f.to_host_cell_major()
+        assert np.array_equal(host, ref[name].astype(host.dtype)), name
+        t = f.tile
+        padded = (45 + t - 1) // t * t
+        a = f.data.cpu().numpy()[: padded * f.num_points * f.len]
+        a = a.reshape(padded // t, f.num_points, f.len, t).transpose(0, 3, 1, 2)
+        a = a.reshape(padded, f.num_points, f.len)
+        for c in range(45, padded):
+            assert np.array_equal(a[c], a[44]), (name, c)
+
+
+def test_dofmap_indices_clamp_to_last_node(cuda):
+    """Indices past the last node read the last node (evector.rs:187-191, tests/bounds.rs)."""
+    import torch
+
+    mesh = volume_mesh(TET, 1, n=2)
+    cells = mesh.blocks[0].cells.copy()
+    cells[3, 2] = 10_000_000
+    cells[7, 0] = 0xFFFFFFFF
+    mesh.blocks[0].cells = cells
+    g = api.geometry_from_host(mesh, dtype=torch.float64)
+    ps = api.PointSet.tet_keast4()
+    got = gpu_factors(g, 0, ps, ["points", "jacobians", "detj"])
+    want = oracle_factors(g, mesh, 0, ps, ["points", "jacobians", "detj"])
+    for n in got:
+        assert np.abs(got[n] - want[n]).max() <= 1e-13, n
+
+
+def test_normals_flags_and_unit_length(cuda):
+    """Normals of a surface are unit vectors, flipped where bit 0 of the flags is set
+    (layout.rs:335-367), and agree with the oracle."""
+    import torch
+
+    mesh = surface_mesh(TRI, 3, 3)
+    g = api.geometry_from_host(mesh, dtype=torch.float32)
+    ps = api.PointSet.gauss(TRI, 4)
+    n = gpu_factors(g, 0, ps, ["normals"])["normals"]
+    assert np.allclose(np.linalg.norm(n, axis=2), 1.0, atol=1e-6)
+    flags = mesh.blocks[0].flags
+    # The unflipped surface normal points up (+z) on this graph surface.
+    assert np.all(n[flags == 0][..., 2] > 0) and np.all(n[flags == 1][..., 2] < 0)
+    want = oracle_factors(g, mesh, 0, ps, ["normals"])["normals"]
+    assert np.abs(n - want).max() <= 2e-6
+
+
+@pytest.mark.parametrize("cell,p", [(QUAD, 1), (TRI, 2), (TET, 1), (TET, 2), (HEX, 2),
+                                    (PRISM, 1)])
+def test_facet_normals(cuda, cell, p):
+    """Outward unit facet normals K^T n / |K^T n| (facets.rs:1-61) against the oracle; on the
+    unit square quadrilateral they are the four axis directions (facets.rs:27-42)."""
+    import torch
+
+    mesh = volume_mesh(cell, p, n=2) if M.TDIM[cell] == 3 else surface_mesh(cell, p, 2, n=2)
+    g = api.geometry_from_host(mesh, dtype=torch.float64)
+    fps = api.PointSet.facet_gauss(cell, 2)
+    got = gpu_factors(g, 0, fps, ["facet_normals"])["facet_normals"]
+    want = oracle_factors(g, mesh, 0, fps, ["facet_normals"])["facet_normals"]
+    assert np.abs(got - want).max() <= 1e-12
+    if cell == QUAD and p == 1:
+        sq = M.square_box(QUAD, 1, 1)
+        g2 = api.geometry_from_host(sq, dtype=torch.float32)
+        n = gpu_factors(g2, 0, api.PointSet.facet_gauss(QUAD, 1), ["facet_normals"])
+        axes = sorted(tuple(np.rint(v).astype(int)) for v in n["facet_normals"][0])
+        assert axes == [(-1, 0), (0, -1), (0, 1), (1, 0)]
+
+
+def test_doc_examples(cuda):
+    """Known answers of the reference's doc tests: 2x2 unit-square quadrilaterals have
+    det J = 1/4, J = I/2, K = 2I at the centre point (lib.rs:24-39, points_jac.rs:22-86,
+    detj_pinv.rs:68-86); the unit square in two triangles has |det J| = 1 (detj_pinv.rs:22-37)."""
+    import torch
+
+    sq = M.square_box(QUAD, 2, 1)
+    g = api.geometry_from_host(sq, dtype=torch.float32)
+    ps = api.PointSet.gauss(QUAD, 1)
+    f = gpu_factors(g, 0, ps, ["points", "jacobians", "detj", "pinv"])
+    assert np.allclose(f["detj"], 0.25, atol=1e-6)
+    assert np.allclose(f["jacobians"], [0.5, 0.0, 0.0, 0.5], atol=1e-6)
+    assert np.allclose(f["pinv"], [2.0, 0.0, 0.0, 2.0], atol=1e-5)
+    centres = sorted(tuple(np.round(x, 6)) for x in f["points"][:, 0, :])
+    assert centres == [(0.25, 0.25), (0.25, 0.75), (0.75, 0.25), (0.75, 0.75)]
+    tri = M.square_box(TRI, 1, 1)
+    g = api.geometry_from_host(tri, dtype=torch.float64)
+    d = gpu_factors(g, 0, api.PointSet.gauss(TRI, 1), ["detj"])["detj"]
+    assert np.allclose(np.abs(d), 1.0, atol=1e-14)
+
+
+def test_affine_tets_closed_form(cuda):
+    """On affine P1 tetrahedra J = [v1 - v0, v2 - v0, v3 - v0] and det J = 6 vol at every
+    point (tetraloma-geometry/tests/factors.rs:149-185)."""
+    import torch
+
+    mesh = volume_mesh(TET, 1, n=3)
+    g = api.geometry_from_host(mesh, dtype=torch.float64)
+    ps = api.PointSet.tet_14()
+    f = gpu_factors(g, 0, ps, ["jacobians", "detj"])
+    x = api.stored_nodes_f64(g)[mesh.blocks[0].cells.astype(np.int64)]
+    cols = np.stack([x[:, 1] - x[:, 0], x[:, 2] - x[:, 0], x[:, 3] - x[:, 0]], axis=1)
+    assert np.abs(f["jacobians"] - cols.reshape(-1, 1, 9)).max() <= 1e-15
+    vol6 = np.linalg.det(cols.transpose(0, 2, 1))
+    assert np.abs(f["detj"][:, :, 0] - vol6[:, None]).max() <= 1e-15
+
+
+@pytest.mark.parametrize("cell,p,deg", [(TET, 1, 0), (TET, 2, 3), (TET, 3, 6), (HEX, 1, 3),
+                                        (HEX, 2, 5), (PRISM, 2, 5)])
+def test_unit_volume(cuda, cell, p, deg):
+    """sum_c sum_q w_q det J = 1 on boxes whose boundary nodes stay put, with a rule exact for
+    det J (tetraloma/tests/unit_volume.rs)."""
+    import torch
+
+    mesh = volume_mesh(cell, p, n=3)
+    g = api.geometry_from_host(mesh, dtype=torch.float64)
+    ps = api.PointSet.gauss(cell, deg)
+    d = gpu_factors(g, 0, ps, ["detj"])["detj"][:, :, 0]
+    assert abs((d @ ps.weights).sum() - 1.0) <= 1e-12
+
+
+def test_evector(cuda):
+    """The E-vector [k][g][padded cells] equals the gather on the host (evector.rs:197-236)."""
+    import torch
+
+    mesh = volume_mesh(TET, 2, n=2)
+    mesh.blocks[0].cells = mesh.blocks[0].cells[:37]
+    g = api.geometry_from_host(mesh, dtype=torch.float32)
+    ev = g.evector(0).cpu().numpy()
+    cells = mesh.blocks[0].cells.astype(np.int64)
+    padded = 48
+    idx = np.concatenate([cells, np.repeat(cells[-1:], padded - 37, axis=0)], axis=0)
+    x = g.nodes.cpu().numpy()
+    want = x[idx].transpose(1, 2, 0).reshape(-1)  # [k][g][cell]
+    assert np.array_equal(ev, want)
+
+
+def test_empty_block_and_points(cuda):
+    """An empty block or point set launches nothing and returns empty fields
+    (layout.rs:509-511)."""
+    import torch
+
+    mesh = volume_mesh(TET, 1, n=1)
+    mesh.blocks[0].cells = mesh.blocks[0].cells[:0]
+    g = api.geometry_from_host(mesh, dtype=torch.float32)
+    f = api.geometric_factors_block(g, 0, api.PointSet.tet_keast4(), ["detj"])
+    assert f["detj"].num_cells == 0
+    mesh = volume_mesh(TET, 1, n=1)
+    g = api.geometry_from_host(mesh, dtype=torch.float32)
+    empty = api.PointSet.custom(TET, np.zeros((0, 3)), np.zeros(0))
+    f = api.geometric_factors_block(g, 0, empty, ["detj"])
+    assert f["detj"].num_points == 0
+
+
+def test_fp64_fixup_on_near_degenerate_cells(cuda):
+    """Near-degenerate points (|det J| < tau |J|_F^3) get det J in FP64: pointwise det J then
+    matches the oracle to 1e-6 where the FP32 value alone does not."""
+    import torch
+
+    mesh = volume_mesh(TET, 2, n=2)
+    x = mesh.nodes.copy()
+    # Flatten the box in z by 1e-4 so that every cell is a sliver; J stays well scaled in x, y.
+    x[:, 2] = 0.3 + 1e-4 * x[:, 2]
+    mesh.nodes = x
+    g = api.geometry_from_host(mesh, dtype=torch.float32)
+    ps = api.PointSet.tet_14()
+    want = oracle_factors(g, mesh, 0, ps, ["detj"])["detj"]
+    g.set_fixup_threshold(1e-2)
+    fixed = gpu_factors(g, 0, ps, ["detj"])["detj"]
+    plain = gpu_factors(g, 0, ps, ["detj"], options=api.OPT_NO_FP64_FIXUP)["detj"]
+    e_fixed, e_plain = pointwise_rel(fixed, want), pointwise_rel(plain, want)
+    print("near-degenerate det J pointwise rel err: fixed", e_fixed, "fp32 only", e_plain)
+    assert e_fixed <= 1e-6
+    assert e_plain > e_fixed
+
+
+def test_runs_are_bitwise_identical(cuda):
+    """Two evaluations give bitwise equal fields (tetraloma/tests/determinism.rs)."""
+    import torch
+
+    mesh = volume_mesh(HEX, 2)
+    g = api.geometry_from_host(mesh, dtype=torch.float32)
+    ps = api.PointSet.gauss(HEX, 4)
+    a = gpu_factors(g, 0, ps, ["points", "jacobians", "detj", "pinv"])
+    b = gpu_factors(g, 0, ps, ["points", "jacobians", "detj", "pinv"])
+    for n in a:
+        assert np.array_equal(a[n], b[n])
+
+
+def test_mesh_level_loops(cuda):
+    """Mesh.detj etc. loop over every block with the rule's point set for its cell type
+    (mesh.rs:249-257); a rule without the cell raises CellNotInRule (rule.rs:163-172)."""
+    import torch
+
+    h = M.config4_mesh(seed=4, scale=0.002)
+    g = api.geometry_from_host(h, dtype=torch.float32)
+    mesh = api.Mesh(g)
+    rule = api.QuadRule.for_grid(g, 4)
+    dets = mesh.detj(rule)
+    assert len(dets) == 5
+    for b, f in enumerate(dets):
+        want = oracle_factors(g, h, b, rule.get(h.blocks[b].cell), ["detj"])["detj"]
+        assert np.abs(f.to_host_cell_major() - want).max() <= 1e-6 * np.abs(want).max()
+    with pytest.raises(api.CellNotInRule):
+        mesh.detj(api.QuadRule.gauss([TET], 2))
