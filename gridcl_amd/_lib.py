@@ -54,6 +54,12 @@ SIGNATURES = {
     ),
     "tg_evector_build": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p]),
     "tg_evector_elems": (c_uint64, [c_void_p, c_int32]),
+    "tg_functional_create": (c_int32, [c_void_p, c_int32, c_void_p, POINTER(c_void_p)]),
+    "tg_functional_destroy": (c_int32, [c_void_p]),
+    "tg_functional_forward": (c_int32, [c_void_p, c_void_p, c_double, c_void_p, c_void_p]),
+    "tg_functional_vjp": (
+        c_int32, [c_void_p, c_void_p, c_double, c_void_p, c_void_p, c_void_p]
+    ),
 }
 
 

@@ -483,6 +483,70 @@ class Mesh:
         return self._per_block(rule, "facet_normals", tile)
 
 
+class MeshFunctional:
+    """F(X) = sum_c sum_q w_q det J + beta sum_c sum_q w_q |J|_F^2 / det(J)^(2/3) over block b
+    at `pts`, with its vector-Jacobian product (no reference counterpart).
+
+    ``forward(nodes)`` returns F as a 0-d FP64 device tensor; ``vjp(seed, nodes)`` returns
+    seed * dF/dX as an FP64 device tensor [num_nodes][3]; ``apply(nodes)`` is the same pair as a
+    torch.autograd.Function, so ``F = fn.apply(x); F.backward()`` fills ``x.grad``."""
+
+    def __init__(self, g: Geometry, b: int, pts: PointSet, beta: float = 0.1):
+        self.g, self.b, self.pts, self.beta = g, b, pts, float(beta)
+        h = ctypes.c_void_p()
+        _check(_lib.load().tg_functional_create(g.handle, b, pts._h, ctypes.byref(h)))
+        self._h = h
+
+    def _nodes(self, nodes):
+        if nodes is None:
+            return None
+        if nodes.dtype != self.g.torch_dtype or nodes.shape != self.g.nodes.shape:
+            raise LengthMismatch("nodes must match the geometry's shape and type")
+        return _dptr(nodes.contiguous())
+
+    def forward(self, nodes: torch.Tensor | None = None) -> torch.Tensor:
+        out = torch.empty((), dtype=torch.float64, device=self.g.nodes.device)
+        _check(_lib.load().tg_functional_forward(self._h, self._nodes(nodes), self.beta,
+                                                 _dptr(out), _stream()))
+        return out
+
+    def vjp(self, seed: torch.Tensor | None = None,
+            nodes: torch.Tensor | None = None) -> torch.Tensor:
+        grad = torch.empty(self.g.nodes.shape, dtype=torch.float64, device=self.g.nodes.device)
+        s = None
+        if seed is not None:
+            seed = seed.to(dtype=torch.float64).reshape(()).contiguous()
+            s = _dptr(seed)
+        _check(_lib.load().tg_functional_vjp(self._h, self._nodes(nodes), self.beta, s,
+                                             _dptr(grad), _stream()))
+        return grad
+
+    def apply(self, nodes: torch.Tensor) -> torch.Tensor:
+        return _FunctionalFn.apply(nodes, self)
+
+    def __del__(self):
+        try:
+            if self._h:
+                _lib.load().tg_functional_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
+class _FunctionalFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, nodes, fn):
+        ctx.save_for_backward(nodes)
+        ctx.fn = fn
+        return fn.forward(nodes.detach())
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        (nodes,) = ctx.saved_tensors
+        g = ctx.fn.vjp(grad_out, nodes.detach())
+        return g.to(nodes.dtype), None
+
+
 def geometry_from_host(mesh, dtype=torch.float32, device="cuda", origin=None):
     """Uploads a HostMesh: nodes relative to the centre of their bounding box (the reference's
     origin, geometry.rs:14-19), narrowed to `dtype`; dofmaps and flags as u32 bits."""

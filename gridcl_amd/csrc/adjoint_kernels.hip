@@ -1,0 +1,304 @@
+// The mesh functional F = sum_c sum_q w_q det J + beta sum_c sum_q w_q |J|_F^2 / det(J)^(2/3)
+// over a block of 3D cells, and its gradient with respect to the node coordinates (a vector-
+// Jacobian product). The reference has no adjoint; the forward reuses the factor kernels'
+// Jacobian arithmetic (tetraloma-geometry/src/layout.rs:138-213) and the reduction is
+// deterministic like measure.rs:17-102 (fixed-order per-block trees, then one ordered pass).
+//
+// Forward: one lane per cell; J in T (FP32: coordinates shifted to the cell's first node) at
+// each point, the integrand in T, accumulated per cell in FP64; a fixed tree per workgroup
+// writes one FP64 partial; a single-workgroup pass sums the partials in order.
+// Backward: J is recomputed (not stored), dF/dJ is formed in FP64 from it, and the lane writes
+// its cell's contributions dF/dx(k, g) = sum_q sum_d dF/dJ(g, d) dphi_k/dxi_d in FP64; a node
+// pass then sums each node's contributions in CSR order (ascending cell, then local node), so
+// the gradient is bitwise reproducible without atomics.
+#include <hip/hip_runtime.h>
+
+#include "adjoint_internal.hpp"
+
+namespace tg {
+namespace {
+
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ float tfma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double tfma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float tcbrt(float x) { return cbrtf(x); }
+__device__ __forceinline__ double tcbrt(double x) { return cbrt(x); }
+
+template <typename T>
+__device__ __forceinline__ T det3(const T* j) {
+  return j[0] * (j[4] * j[8] - j[7] * j[5]) - j[3] * (j[1] * j[8] - j[7] * j[2]) +
+         j[6] * (j[1] * j[5] - j[4] * j[2]);
+}
+
+// Gathers the K nodes of cell c (indices clamped to the last node); SHIFT subtracts node 0.
+template <typename T, int K, bool SHIFT>
+__device__ __forceinline__ void gather(const T* __restrict__ nodes, const uint32_t* __restrict__ row,
+                                       uint32_t last_node, T (&x)[K][3]) {
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    uint32_t n = row[k];
+    n = n < last_node ? n : last_node;
+#pragma unroll
+    for (int g = 0; g < 3; ++g) x[k][g] = nodes[static_cast<uint64_t>(n) * 3 + g];
+  }
+  if constexpr (SHIFT) {
+#pragma unroll
+    for (int k = 1; k < K; ++k)
+#pragma unroll
+      for (int g = 0; g < 3; ++g) x[k][g] -= x[0][g];
+  }
+}
+
+// J (column-major, component d * 3 + g) of the gathered cell at point q.
+template <typename T, int K, bool SHIFT>
+__device__ __forceinline__ void jacobian(const T (&x)[K][3], const T* __restrict__ t, T* j) {
+  constexpr int K0 = SHIFT ? 1 : 0;
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+#pragma unroll
+    for (int g = 0; g < 3; ++g) {
+      T acc = x[K0][g] * t[4 * K0 + 1 + d];
+#pragma unroll
+      for (int k = K0 + 1; k < K; ++k) acc = tfma(x[k][g], t[4 * k + 1 + d], acc);
+      j[d * 3 + g] = acc;
+    }
+}
+
+// Fixed-order tree over the workgroup; thread 0 returns the sum.
+__device__ __forceinline__ double block_sum(double v, double* lds) {
+  lds[threadIdx.x] = v;
+  __syncthreads();
+#pragma unroll
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < static_cast<unsigned>(s)) lds[threadIdx.x] += lds[threadIdx.x + s];
+    __syncthreads();
+  }
+  return lds[0];
+}
+
+template <typename T, int K, bool SHIFT>
+__global__ void __launch_bounds__(kBlock)
+    functional_forward(const T* __restrict__ nodes, const uint32_t* __restrict__ dofmap,
+                       const T* __restrict__ tab, const double* __restrict__ wts, uint32_t nq,
+                       uint64_t num_cells, uint32_t last_node, double beta,
+                       double* __restrict__ partials) {
+  __shared__ double lds[kBlock];
+  const uint64_t c = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  double cell_sum = 0.0;
+  if (c < num_cells) {
+    T x[K][3];
+    gather<T, K, SHIFT>(nodes, dofmap + c * K, last_node, x);
+    const T b = static_cast<T>(beta);
+    for (uint32_t q = 0; q < nq; ++q) {
+      T j[9];
+      jacobian<T, K, SHIFT>(x, tab + static_cast<uint64_t>(q) * K * 4, j);
+      const T det = det3(j);
+      T n2 = j[0] * j[0];
+#pragma unroll
+      for (int i = 1; i < 9; ++i) n2 = tfma(j[i], j[i], n2);
+      const T r = tcbrt(det);
+      const T f = det + b * n2 / (r * r);
+      cell_sum = tfma(wts[q], static_cast<double>(f), cell_sum);
+    }
+  }
+  const double s = block_sum(cell_sum, lds);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// Sums n partials in a fixed order into *out (one workgroup).
+__global__ void __launch_bounds__(kBlock)
+    sum_partials(const double* __restrict__ partials, uint32_t n, double* __restrict__ out) {
+  __shared__ double lds[kBlock];
+  double acc = 0.0;
+  for (uint32_t i = threadIdx.x; i < n; i += kBlock) acc += partials[i];
+  const double s = block_sum(acc, lds);
+  if (threadIdx.x == 0) *out = s;
+}
+
+template <typename T, int K, bool SHIFT>
+__global__ void __launch_bounds__(kBlock)
+    functional_vjp_cells(const T* __restrict__ nodes, const uint32_t* __restrict__ dofmap,
+                         const T* __restrict__ tab, const double* __restrict__ tab64,
+                         const double* __restrict__ wts, uint32_t nq, uint64_t num_cells,
+                         uint32_t last_node, double beta, const double* __restrict__ seed,
+                         double* __restrict__ contrib) {
+  const uint64_t c = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (c >= num_cells) return;
+  T x[K][3];
+  gather<T, K, SHIFT>(nodes, dofmap + c * K, last_node, x);
+  double ct[K][3];
+#pragma unroll
+  for (int k = 0; k < K; ++k) ct[k][0] = ct[k][1] = ct[k][2] = 0.0;
+  for (uint32_t q = 0; q < nq; ++q) {
+    T jt[9];
+    jacobian<T, K, SHIFT>(x, tab + static_cast<uint64_t>(q) * K * 4, jt);
+    double j[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) j[i] = static_cast<double>(jt[i]);
+    // Cofactors cof(g, d) = d det / d J(g, d), stored like J at d * 3 + g.
+    double cof[9];
+    cof[0] = j[4] * j[8] - j[7] * j[5];
+    cof[3] = -(j[1] * j[8] - j[7] * j[2]);
+    cof[6] = j[1] * j[5] - j[4] * j[2];
+    cof[1] = -(j[3] * j[8] - j[6] * j[5]);
+    cof[4] = j[0] * j[8] - j[6] * j[2];
+    cof[7] = -(j[0] * j[5] - j[3] * j[2]);
+    cof[2] = j[3] * j[7] - j[6] * j[4];
+    cof[5] = -(j[0] * j[7] - j[6] * j[1]);
+    cof[8] = j[0] * j[4] - j[3] * j[1];
+    const double det = j[0] * cof[0] + j[3] * cof[3] + j[6] * cof[6];
+    double n2 = 0.0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) n2 = __builtin_fma(j[i], j[i], n2);
+    const double r = cbrt(det);
+    const double inv23 = 1.0 / (r * r);
+    const double w = wts[q];
+    const double a = w * (1.0 - beta * (2.0 / 3.0) * n2 * inv23 / det);
+    const double bb = w * beta * 2.0 * inv23;
+    double G[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) G[i] = a * cof[i] + bb * j[i];
+    const double* t = tab64 + static_cast<uint64_t>(q) * K * 4;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const double d0 = t[4 * k + 1], d1 = t[4 * k + 2], d2 = t[4 * k + 3];
+#pragma unroll
+      for (int g = 0; g < 3; ++g)
+        ct[k][g] = __builtin_fma(G[g], d0, __builtin_fma(G[3 + g], d1, __builtin_fma(G[6 + g], d2, ct[k][g])));
+    }
+  }
+  const double s = seed ? *seed : 1.0;
+  double* out = contrib + c * K * 3;
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+#pragma unroll
+    for (int g = 0; g < 3; ++g) out[k * 3 + g] = s * ct[k][g];
+}
+
+// grad[n] = sum over the CSR entries of node n, in order, of contrib[entry] (FP64).
+__global__ void __launch_bounds__(kBlock)
+    csr_node_sum(const uint64_t* __restrict__ offsets, const uint32_t* __restrict__ entries,
+                 const double* __restrict__ contrib, uint64_t num_nodes, double* __restrict__ grad) {
+  const uint64_t n = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (n >= num_nodes) return;
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  const uint64_t e1 = offsets[n + 1];
+  for (uint64_t e = offsets[n]; e < e1; ++e) {
+    const double* v = contrib + static_cast<uint64_t>(entries[e]) * 3;
+    a0 += v[0];
+    a1 += v[1];
+    a2 += v[2];
+  }
+  grad[n * 3] = a0;
+  grad[n * 3 + 1] = a1;
+  grad[n * 3 + 2] = a2;
+}
+
+__global__ void __launch_bounds__(kBlock) fill_zero(double* __restrict__ p, uint64_t n) {
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i < n) p[i] = 0.0;
+}
+
+template <typename T>
+using FwdFn =void (*)(const T*, const uint32_t*, const T*, const double*, uint32_t, uint64_t,
+                       uint32_t, double, double*);
+template <typename T>
+using VjpFn = void (*)(const T*, const uint32_t*, const T*, const double*, const double*, uint32_t,
+                       uint64_t, uint32_t, double, const double*, double*);
+
+// Register-resident 3D elements: P1-P3 tetrahedra, Q1-Q2 hexahedra, P1-P2 prisms, P1 pyramids.
+#define TG_ADJ_SHAPES(X) X(4) X(10) X(20) X(8) X(27) X(6) X(18) X(5)
+
+template <typename T>
+FwdFn<T> fwd_kernel(uint32_t k) {
+  constexpr bool kShift = sizeof(T) == 4;
+#define TG_CASE(K_) \
+  if (k == K_) return functional_forward<T, K_, kShift>;
+  TG_ADJ_SHAPES(TG_CASE)
+#undef TG_CASE
+  return nullptr;
+}
+
+template <typename T>
+VjpFn<T> vjp_kernel(uint32_t k) {
+  constexpr bool kShift = sizeof(T) == 4;
+#define TG_CASE(K_) \
+  if (k == K_) return functional_vjp_cells<T, K_, kShift>;
+  TG_ADJ_SHAPES(TG_CASE)
+#undef TG_CASE
+  return nullptr;
+}
+
+}  // namespace
+
+uint32_t functional_partials(uint64_t num_cells) {
+  return static_cast<uint32_t>((num_cells + kBlock - 1) / kBlock);
+}
+
+template <typename T>
+hipError_t launch_functional_forward(const T* nodes, const uint32_t* dofmap, uint32_t k,
+                                     const T* tab, const double* wts, uint32_t nq,
+                                     uint64_t num_cells, uint32_t last_node, double beta,
+                                     double* partials, double* out, hipStream_t stream) {
+  FwdFn<T> fn = fwd_kernel<T>(k);
+  if (!fn) return hipErrorNotSupported;
+  const uint32_t blocks = functional_partials(num_cells);
+  hipLaunchKernelGGL(fn, dim3(blocks), dim3(kBlock), 0, stream, nodes, dofmap, tab, wts, nq,
+                     num_cells, last_node, beta, partials);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(sum_partials, dim3(1), dim3(kBlock), 0, stream, partials, blocks, out);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_functional_vjp(const T* nodes, const uint32_t* dofmap, uint32_t k,
+                                 const T* tab, const double* tab64, const double* wts,
+                                 uint32_t nq, uint64_t num_cells, uint32_t last_node, double beta,
+                                 const double* seed, double* contrib, const uint64_t* offsets,
+                                 const uint32_t* entries, uint64_t num_nodes, double* grad,
+                                 hipStream_t stream) {
+  VjpFn<T> fn = vjp_kernel<T>(k);
+  if (!fn) return hipErrorNotSupported;
+  const uint32_t blocks = functional_partials(num_cells);
+  hipLaunchKernelGGL(fn, dim3(blocks), dim3(kBlock), 0, stream, nodes, dofmap, tab, tab64, wts, nq,
+                     num_cells, last_node, beta, seed, contrib);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const uint64_t nb = (num_nodes + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(csr_node_sum, dim3(static_cast<uint32_t>(nb)), dim3(kBlock), 0, stream,
+                     offsets, entries, contrib, num_nodes, grad);
+  return hipGetLastError();
+}
+
+bool functional_supported(uint32_t k) { return fwd_kernel<float>(k) != nullptr; }
+
+// Zeroes n doubles with a kernel (no memset node, so the call may sit in a captured graph).
+hipError_t launch_fill_zero(double* p, uint64_t n, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  const uint64_t nb = (n + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(fill_zero, dim3(static_cast<uint32_t>(nb)), dim3(kBlock), 0, stream, p, n);
+  return hipGetLastError();
+}
+
+template hipError_t launch_functional_forward<float>(const float*, const uint32_t*, uint32_t,
+                                                     const float*, const double*, uint32_t,
+                                                     uint64_t, uint32_t, double, double*,
+                                                     double*, hipStream_t);
+template hipError_t launch_functional_forward<double>(const double*, const uint32_t*, uint32_t,
+                                                      const double*, const double*, uint32_t,
+                                                      uint64_t, uint32_t, double, double*,
+                                                      double*, hipStream_t);
+template hipError_t launch_functional_vjp<float>(const float*, const uint32_t*, uint32_t,
+                                                 const float*, const double*, const double*,
+                                                 uint32_t, uint64_t, uint32_t, double,
+                                                 const double*, double*, const uint64_t*,
+                                                 const uint32_t*, uint64_t, double*, hipStream_t);
+template hipError_t launch_functional_vjp<double>(const double*, const uint32_t*, uint32_t,
+                                                  const double*, const double*, const double*,
+                                                  uint32_t, uint64_t, uint32_t, double,
+                                                  const double*, double*, const uint64_t*,
+                                                  const uint32_t*, uint64_t, double*, hipStream_t);
+
+}  // namespace tg

@@ -12,11 +12,14 @@
 
 #include "../../include/tetraloma_geom.h"
 #include "geom_internal.hpp"
+#include "handles.hpp"
 #include "refcell.hpp"
 
 namespace {
-
 thread_local std::string g_last_error;
+}  // namespace
+
+namespace tg {
 
 int fail(int code, const std::string& msg) {
   g_last_error = msg;
@@ -26,58 +29,30 @@ int fail(int code, const std::string& msg) {
 const char* cell_name(int c) {
   static const char* names[] = {"Point",       "Interval",   "Triangle", "Quadrilateral",
                                 "Tetrahedron", "Hexahedron", "Prism",    "Pyramid"};
-  return tg::valid_cell(c) ? names[c] : "?";
+  return valid_cell(c) ? names[c] : "?";
 }
 
 int hip_fail(hipError_t e, const char* what) {
   return fail(TG_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-struct Block {
-  tg::Cell cell;
-  unsigned degree;
-  tg::Family family;
-  const uint32_t* dofmap;
-  uint64_t num_cells;
-  const uint32_t* flags;
-  uint32_t k;
-  int tdim;
-};
+DeviceTables::~DeviceTables() {
+  if (tab) (void)hipFree(tab);
+  if (tab64) (void)hipFree(tab64);
+  if (ref_normals) (void)hipFree(ref_normals);
+}
 
-// Device copies of one element's tables at one point set: interleaved [q][k][4] (value, then
-// the tdim derivatives, zero-padded) in the geometry's type, the same in f64 for the FP64
-// fix-up, and the reference facet normals [q][tdim] in the geometry's type.
-struct DeviceTables {
-  void* tab = nullptr;
-  double* tab64 = nullptr;
-  void* ref_normals = nullptr;
-  ~DeviceTables() {
-    if (tab) (void)hipFree(tab);
-    if (tab64) (void)hipFree(tab64);
-    if (ref_normals) (void)hipFree(ref_normals);
-  }
-};
+}  // namespace tg
 
-}  // namespace
+tg_pointset::~tg_pointset() {
+  if (weights_device) (void)hipFree(weights_device);
+}
 
-struct tg_geometry {
-  int dtype;
-  int gdim;
-  const void* nodes;
-  uint64_t num_nodes;
-  double origin[3];
-  std::vector<Block> blocks;
-  double tau = 1e-2;
-};
-
-struct tg_pointset {
-  tg::Cell cell;
-  int tdim;
-  uint64_t nq;
-  std::vector<double> points, weights, normals;
-  bool has_normals = false;
-  std::map<std::tuple<unsigned, int, int>, std::unique_ptr<DeviceTables>> cache;
-};
+using tg::Block;
+using tg::cell_name;
+using tg::DeviceTables;
+using tg::fail;
+using tg::hip_fail;
 
 namespace {
 
@@ -120,6 +95,10 @@ int upload(const std::vector<double>& src, void** dst) {
   return TG_OK;
 }
 
+}  // namespace
+
+namespace tg {
+
 // The cached device tables of (degree, family, dtype), computed and uploaded on first use
 // (point_set.rs:355-365).
 int device_tables(tg_pointset* p, unsigned degree, int family, int dtype, DeviceTables** out) {
@@ -147,6 +126,23 @@ int device_tables(tg_pointset* p, unsigned degree, int family, int dtype, Device
   p->cache.emplace(key, std::move(t));
   return TG_OK;
 }
+
+int device_weights(tg_pointset* p, const double** out) {
+  if (!p->weights_device) {
+    void* d = nullptr;
+    int rc = upload<double>(p->weights, &d);
+    if (rc) return rc;
+    p->weights_device = static_cast<double*>(d);
+  }
+  *out = p->weights_device;
+  return TG_OK;
+}
+
+}  // namespace tg
+
+using tg::device_tables;
+
+namespace {
 
 int make_pointset(tg::Cell cell, std::vector<double> pts, std::vector<double> wts,
                   std::vector<double> normals, bool has_normals, tg_pointset** out) {

@@ -1,0 +1,70 @@
+// The opaque handles of the C ABI and the host helpers shared by its translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <memory>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "refcell.hpp"
+
+namespace tg {
+
+struct Block {
+  Cell cell;
+  unsigned degree;
+  Family family;
+  const uint32_t* dofmap;
+  uint64_t num_cells;
+  const uint32_t* flags;
+  uint32_t k;
+  int tdim;
+};
+
+// Device copies of one element's tables at one point set: interleaved [q][k][4] (value, then
+// the tdim derivatives, zero-padded) in the geometry's type, the same in f64, and the reference
+// facet normals [q][tdim] in the geometry's type.
+struct DeviceTables {
+  void* tab = nullptr;
+  double* tab64 = nullptr;
+  void* ref_normals = nullptr;
+  ~DeviceTables();
+};
+
+// Records the message of the last error on this thread and returns `code`.
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+const char* cell_name(int c);
+
+}  // namespace tg
+
+struct tg_geometry {
+  int dtype;
+  int gdim;
+  const void* nodes;
+  uint64_t num_nodes;
+  double origin[3];
+  std::vector<tg::Block> blocks;
+  double tau = 1e-2;
+};
+
+struct tg_pointset {
+  tg::Cell cell;
+  int tdim;
+  uint64_t nq;
+  std::vector<double> points, weights, normals;
+  bool has_normals = false;
+  std::map<std::tuple<unsigned, int, int>, std::unique_ptr<tg::DeviceTables>> cache;
+  double* weights_device = nullptr;  // uploaded on first use
+  ~tg_pointset();
+};
+
+namespace tg {
+// The cached device tables of (degree, family, dtype), computed and uploaded on first use.
+int device_tables(tg_pointset* p, unsigned degree, int family, int dtype, DeviceTables** out);
+// The point weights on the device (f64), uploaded on first use.
+int device_weights(tg_pointset* p, const double** out);
+}  // namespace tg

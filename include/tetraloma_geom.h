@@ -183,6 +183,32 @@ int tg_eval_block(const tg_geometry* g, int32_t block, tg_pointset* pts, uint32_
 int tg_evector_build(const tg_geometry* g, int32_t block, void* out, void* stream);
 uint64_t tg_evector_elems(const tg_geometry* g, int32_t block);
 
+/* ---- The mesh functional and its adjoint (no reference counterpart) --------------------------
+ *
+ * F(X) = sum_c sum_q w_q det J_cq(X) + beta sum_c sum_q w_q |J_cq(X)|_F^2 / det(J_cq(X))^(2/3)
+ * over one block of 3D cells in 3D at the points and weights of a point set. The forward runs in
+ * the geometry's type (F32: J of coordinates shifted to each cell's first node) and reduces in
+ * FP64 in a fixed order; the vector-Jacobian product recomputes J, forms dF/dJ in FP64 and sums
+ * each node's per-(cell, local node) contributions in FP64 in CSR order, so both are bitwise
+ * reproducible. The reduction follows Mesh::measure (tetraloma/src/mesh.rs:579;
+ * tetraloma-geometry/src/measure.rs:17-102). */
+typedef struct tg_functional tg_functional;
+
+/* Binds the functional of block `block` at `pts`: builds the node incidence in CSR form (the
+ * block's dofmap is copied to the host once) and allocates the FP64 work arrays (one partial
+ * per 256 cells, [cells][k][3] contributions). Register-resident kernels cover P1-P3
+ * tetrahedra, Q1-Q2 hexahedra, P1-P2 prisms and P1 pyramids (UnsupportedDegree otherwise). */
+int tg_functional_create(const tg_geometry* g, int32_t block, tg_pointset* pts,
+                         tg_functional** out);
+int tg_functional_destroy(tg_functional* f);
+/* F into *value (a device double). `nodes` NULL: the geometry's nodes; otherwise a device array
+ * of the same shape and type (the functional is then evaluated there). */
+int tg_functional_forward(tg_functional* f, const void* nodes, double beta, double* value,
+                          void* stream);
+/* grad ([num_nodes][3] FP64, device) = seed * dF/dX, with `seed` a device double (NULL: 1). */
+int tg_functional_vjp(tg_functional* f, const void* nodes, double beta, const double* seed,
+                      double* grad, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,115 @@
+"""The mesh functional and its vector-Jacobian product on the GPU against the FP64 CPU adjoint
+of the oracle and central finite differences (needs the GPU).
+
+Config 5: the config-1 mesh as P2 tetrahedra (32^3 x 6 = 196,608 cells, curved by a further
+jitter of the interior nodes), the 14-point degree-5 rule, beta = 0.1.
+Tolerances (written per test): FP64 kernels against the FP64 CPU adjoint 1e-12 (value) and
+1e-11 (gradient, max-norm relative); FP32 forward 1e-6 relative; FP32-J gradient 1e-5
+max-norm relative; central differences with h = 1e-6 on the FP64 forward 1e-6 relative.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from gridcl_amd import api
+from gridcl_amd import meshes as M
+
+pytestmark = pytest.mark.gpu
+BETA = 0.1
+
+
+def oracle_f(g, mesh, ps, grad=True):
+    blk = mesh.blocks[0]
+    return O.functional(blk.cell, blk.degree, 0, api.stored_nodes_f64(g), blk.cells.reshape(-1),
+                        ps.points, ps.weights, beta=BETA, grad=grad)
+
+
+@pytest.fixture(scope="module")
+def config5():
+    return M.config5_mesh()
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_functional_and_vjp_config5(cuda, config5, dtype):
+    import torch
+
+    mesh = config5
+    assert mesh.blocks[0].num_cells == 196_608 and mesh.num_nodes == 274_625
+    tdt = torch.float64 if dtype == "f64" else torch.float32
+    g = api.geometry_from_host(mesh, dtype=tdt)
+    ps = api.PointSet.tet_14()
+    fn = api.MeshFunctional(g, 0, ps, beta=BETA)
+    F = fn.forward().item()
+    grad = fn.vjp().cpu().numpy()
+    F_ref, grad_ref = oracle_f(g, mesh, ps)
+    ef = abs(F - F_ref) / abs(F_ref)
+    eg = np.abs(grad - grad_ref).max() / np.abs(grad_ref).max()
+    print(f"config 5 {dtype}: F {F:.15e} ref {F_ref:.15e} rel {ef:.2e}; grad rel {eg:.2e}")
+    if dtype == "f64":
+        assert ef <= 1e-12 and eg <= 1e-11
+    else:
+        assert ef <= 1e-6 and eg <= 1e-5
+    # Deterministic: a second evaluation is bitwise equal.
+    assert fn.forward().item() == F
+    assert np.array_equal(fn.vjp().cpu().numpy(), grad)
+
+
+def test_central_differences(cuda):
+    """<dF/dX, d> against (F(X + h d) - F(X - h d)) / 2h for 64 seeded directions, h = 1e-6,
+    on the FP64 forward (a smaller curved P2 box, so each direction touches every node)."""
+    import torch
+
+    mesh = M.config5_mesh(n=6)
+    g = api.geometry_from_host(mesh, dtype=torch.float64)
+    ps = api.PointSet.tet_14()
+    fn = api.MeshFunctional(g, 0, ps, beta=BETA)
+    grad = fn.vjp().cpu().numpy()
+    x0 = g.nodes.clone()
+    h = 1e-6
+    worst = 0.0
+    for i in range(64):
+        d = M.splitmix64_uniform(1000 + i, x0.numel()).reshape(x0.shape) * 2.0 - 1.0
+        dt = torch.from_numpy(d).to(x0.device)
+        fp = fn.forward(x0 + h * dt).item()
+        fm = fn.forward(x0 - h * dt).item()
+        fd = (fp - fm) / (2 * h)
+        an = float((grad * d).sum())
+        worst = max(worst, abs(fd - an) / max(abs(an), 1e-300))
+    print(f"central differences: worst relative mismatch {worst:.2e}")
+    assert worst <= 1e-6
+
+
+def test_autograd_semantics(cuda):
+    """apply() behaves as a torch.autograd.Function: backward fills x.grad with dF/dX, scaled by
+    the upstream gradient."""
+    import torch
+
+    mesh = M.config5_mesh(n=4)
+    g = api.geometry_from_host(mesh, dtype=torch.float64)
+    ps = api.PointSet.tet_14()
+    fn = api.MeshFunctional(g, 0, ps, beta=BETA)
+    x = g.nodes.clone().requires_grad_(True)
+    F = fn.apply(x)
+    (3.0 * F).backward()
+    want = 3.0 * fn.vjp().cpu().numpy()
+    assert np.allclose(x.grad.cpu().numpy(), want, rtol=0, atol=1e-14 * np.abs(want).max())
+
+
+def test_affine_tet_closed_form(cuda):
+    """With beta = 0 on affine P1 tetrahedra F is the volume and dF/dx_k of a cell is
+    cof(J) dphi_k / 6: the gradient of the box's volume moves only boundary nodes."""
+    import torch
+
+    mesh = M.config1_mesh(n=3)
+    g = api.geometry_from_host(mesh, dtype=torch.float64)
+    ps = api.PointSet.tet_keast4()
+    fn = api.MeshFunctional(g, 0, ps, beta=0.0)
+    assert abs(fn.forward().item() - 1.0) <= 1e-14
+    grad = fn.vjp().cpu().numpy()
+    x = mesh.nodes
+    inside = np.all((x > 1e-12) & (x < 1 - 1e-12), axis=1)
+    assert np.abs(grad[inside]).max() <= 1e-15
+    # A node on the face x = 1 (not on an edge) moves the volume by its dual area along +x.
+    face = (np.abs(x[:, 0] - 1) < 1e-12) & np.all((x[:, 1:] > 1e-12) & (x[:, 1:] < 1 - 1e-12),
+                                                   axis=1)
+    assert np.all(grad[face, 0] > 0) and np.abs(grad[face, 1:]).max() <= 1e-15

@@ -1,0 +1,102 @@
+"""Pins the FP64 oracle's factors and adjoint without any reference output (CPU only): affine
+closed forms, exact volumes, K J = I (tetraloma-geometry/tests/factors.rs:51-185,
+tetraloma/tests/unit_volume.rs), the tiled CPU path against the cell-major one, and the
+functional's gradient against central differences and a hand-derived single-cell gradient."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from gridcl_amd import meshes as M
+from gridcl_amd.api import PointSet
+
+TET, HEX, PRISM = 4, 5, 6
+
+
+def test_affine_tets_closed_form():
+    mesh = M.config1_mesh(n=2)
+    blk = mesh.blocks[0]
+    ps = PointSet.tet_14()
+    f = O.eval_block(TET, 1, 0, 3, mesh.nodes, blk.cells.reshape(-1), ps.points,
+                     factors=("jacobians", "detj", "pinv", "points"))
+    x = mesh.nodes[blk.cells.astype(np.int64)]
+    cols = np.stack([x[:, 1] - x[:, 0], x[:, 2] - x[:, 0], x[:, 3] - x[:, 0]], axis=1)
+    assert np.abs(f["jacobians"] - cols.reshape(-1, 1, 9)).max() <= 1e-15
+    det = np.linalg.det(cols.transpose(0, 2, 1))
+    assert np.abs(f["detj"][:, :, 0] - det[:, None]).max() <= 1e-16
+    # X is the affine image of the reference point.
+    want = x[:, :1, :] + np.einsum("qd,cdg->cqg", ps.points, cols)
+    assert np.abs(f["points"] - want).max() <= 1e-15
+    # K J = I (factors.rs:51-147): K column-major tdim x gdim at g*3+t, J at d*3+g.
+    K = f["pinv"].reshape(-1, 3, 3)   # [.., g, t]
+    J = f["jacobians"].reshape(-1, 3, 3)  # [.., d, g]
+    KJ = np.einsum("ngt,ndg->ntd", K, J)
+    assert np.abs(KJ - np.eye(3)).max() <= 1e-12
+
+
+@pytest.mark.parametrize("cell,p,deg", [(TET, 2, 3), (TET, 3, 6), (HEX, 2, 5), (PRISM, 2, 5),
+                                        (HEX, 3, 8)])
+def test_unit_volume(cell, p, deg):
+    mesh = M.jitter(M.cube_box(cell, 3, p), 0.1 / (3 * p), 7, interior_only=True)
+    blk = mesh.blocks[0]
+    ps = PointSet.gauss(cell, deg)
+    d = O.eval_block(cell, p, 0, 3, mesh.nodes, blk.cells.reshape(-1), ps.points,
+                     factors=("detj",))["detj"][:, :, 0]
+    assert abs((d @ ps.weights).sum() - 1.0) <= 1e-13
+
+
+def test_tiled_cpu_path_matches_cell_major():
+    mesh = M.config2_mesh(n=3)
+    blk = mesh.blocks[0]
+    ps = PointSet.tet_14()
+    origin = np.array([0.5, 0.5, 0.5])
+    v, d = O.tabulate(TET, 2, 0, ps.points)
+    x, j, dj = O.cpu_factors(3, 3, 10, mesh.nodes - origin, origin, blk.cells, v, d, tile=4)
+    want = O.eval_block(TET, 2, 0, 3, mesh.nodes, blk.cells.reshape(-1), ps.points)
+    nc, nq = blk.num_cells, len(ps)
+    pad = (nc + 3) // 4 * 4
+
+    def untile(a, length):
+        a = a.reshape(pad // 4, nq, length, 4).transpose(0, 3, 1, 2)
+        return a.reshape(pad, nq, length)[:nc]
+
+    # The tiled path works on coordinates relative to the origin, the cell-major one on absolute
+    # coordinates: the two differ by rounding only.
+    assert O.rel_err(untile(x, 3), want["points"]) <= 1e-15
+    assert O.rel_err(untile(j, 9), want["jacobians"]) <= 1e-13
+    assert O.rel_err(untile(dj, 1), want["detj"]) <= 1e-13
+
+
+def test_functional_gradient_central_differences():
+    mesh = M.config5_mesh(n=2)
+    blk = mesh.blocks[0]
+    ps = PointSet.tet_14()
+    F, g = O.functional(TET, 2, 0, mesh.nodes, blk.cells.reshape(-1), ps.points, ps.weights,
+                        beta=0.1)
+    h = 1e-6
+    for i in range(16):
+        d = M.splitmix64_uniform(50 + i, mesh.nodes.size).reshape(mesh.nodes.shape) * 2 - 1
+        fp, _ = O.functional(TET, 2, 0, mesh.nodes + h * d, blk.cells.reshape(-1), ps.points,
+                             ps.weights, beta=0.1, grad=False)
+        fm, _ = O.functional(TET, 2, 0, mesh.nodes - h * d, blk.cells.reshape(-1), ps.points,
+                             ps.weights, beta=0.1, grad=False)
+        fd = (fp - fm) / (2 * h)
+        an = float((g * d).sum())
+        assert abs(fd - an) <= 1e-7 * abs(an), i
+
+
+def test_functional_single_affine_tet():
+    """One affine tetrahedron: F = vol + beta vol |J|^2 / det^(2/3) exactly, and with beta = 0
+    dF/dx_k = cof(J) dphi_k / 6 (dphi_0 = -(1,1,1), dphi_i = e_i)."""
+    x = np.array([[0.1, 0.0, 0.0], [1.2, 0.1, 0.0], [0.2, 0.9, 0.1], [0.0, 0.2, 1.1]])
+    cells = np.array([0, 1, 2, 3], dtype=np.uint32)
+    ps = PointSet.tet_keast4()
+    J = np.stack([x[1] - x[0], x[2] - x[0], x[3] - x[0]], axis=1)  # rows g, columns d
+    det = np.linalg.det(J)
+    F, g = O.functional(TET, 1, 0, x, cells, ps.points, ps.weights, beta=0.1)
+    want = det / 6 + 0.1 * (J ** 2).sum() / det ** (2 / 3) / 6
+    assert abs(F - want) <= 1e-15 * abs(want)
+    F0, g0 = O.functional(TET, 1, 0, x, cells, ps.points, ps.weights, beta=0.0)
+    cof = det * np.linalg.inv(J).T  # cof[g, d]
+    dphi = np.array([[-1, -1, -1], [1, 0, 0], [0, 1, 0], [0, 0, 1]], dtype=float)
+    want_g = (dphi @ cof.T) / 6  # [k][g] = sum_d cof[g, d] dphi[k, d] / 6
+    assert np.abs(g0 - want_g).max() <= 1e-15
