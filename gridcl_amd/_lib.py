@@ -54,6 +54,15 @@ SIGNATURES = {
     ),
     "tg_evector_build": (c_int32, [c_void_p, c_int32, c_void_p, c_void_p]),
     "tg_evector_elems": (c_uint64, [c_void_p, c_int32]),
+    "tg_plan_create": (
+        c_int32, [c_void_p, c_void_p, c_uint32, c_uint32, c_uint32, c_void_p, POINTER(c_void_p)]
+    ),
+    "tg_plan_destroy": (c_int32, [c_void_p]),
+    "tg_plan_info": (
+        c_int32, [c_void_p, POINTER(c_uint32), POINTER(c_uint32), POINTER(c_int32)]
+    ),
+    "tg_plan_launch": (c_int32, [c_void_p, c_void_p]),
+    "tg_plan_graph_launch": (c_int32, [c_void_p, c_void_p]),
     "tg_functional_create": (c_int32, [c_void_p, c_int32, c_void_p, POINTER(c_void_p)]),
     "tg_functional_destroy": (c_int32, [c_void_p]),
     "tg_functional_forward": (c_int32, [c_void_p, c_void_p, c_double, c_void_p, c_void_p]),

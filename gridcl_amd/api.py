@@ -453,11 +453,77 @@ def normals_block(g: Geometry, b: int, pts: PointSet, tile: int = 16) -> QuadFie
     return geometric_factors_block(g, b, pts, ["normals"], tile)["normals"]
 
 
+class MeshPlan:
+    """Every block of a geometry evaluated in one persistent launch (or one launch per block
+    when an element has no register-resident kernel), optionally replayed from a hipGraph.
+    ``fields[b][name]`` are the outputs of block b, filled by ``launch()``/``graph_launch()``."""
+
+    def __init__(self, g: Geometry, rule: QuadRule, factors: Sequence[str], tile: int = 16,
+                 options: int = 0):
+        lib = _lib.load()
+        self.g = g
+        self.sets = [rule.get(b.element.cell) for b in g.blocks]
+        nb = len(g.blocks)
+        outs = (_lib.Outputs * max(1, nb))()
+        handles = (ctypes.c_void_p * max(1, nb))()
+        mask = 0
+        for name in factors:
+            mask |= FACTORS[name]
+        self.fields: list[dict[str, QuadField]] = []
+        for b, blk in enumerate(g.blocks):
+            td = TDIM[CellType(blk.element.cell)]
+            ps = self.sets[b]
+            handles[b] = ps._h
+            allowed = {"points", "jacobians", "detj", "pinv"}
+            if td == g.gdim and ps.is_facet:
+                allowed.add("facet_normals")
+            if td + 1 == g.gdim:
+                allowed.add("normals")
+            fb = {}
+            ncells = g.block_cells(b)
+            for name in factors:
+                if name not in allowed:
+                    continue
+                length = factor_len(name, td, g.gdim)
+                n = int(lib.tg_field_elems(ncells, len(ps), length, tile))
+                data = torch.empty(max(n, 1), dtype=g.torch_dtype, device=g.nodes.device)
+                setattr(outs[b], name, _dptr(data))
+                fb[name] = QuadField(data, ncells, len(ps), length, tile)
+            self.fields.append(fb)
+        h = ctypes.c_void_p()
+        _check(lib.tg_plan_create(g.handle, handles, mask, tile, options, outs, ctypes.byref(h)))
+        self._h = h
+        items, grid, pers = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_int32()
+        _check(lib.tg_plan_info(h, ctypes.byref(items), ctypes.byref(grid), ctypes.byref(pers)))
+        self.num_items, self.grid, self.persistent = items.value, grid.value, bool(pers.value)
+
+    def launch(self) -> None:
+        _check(_lib.load().tg_plan_launch(self._h, _stream()))
+
+    def graph_launch(self) -> None:
+        _check(_lib.load().tg_plan_graph_launch(self._h, _stream()))
+
+    def __del__(self):
+        try:
+            if self._h:
+                _lib.load().tg_plan_destroy(self._h)
+                self._h = None
+        except Exception:
+            pass
+
+
 class Mesh:
     """The block loop of tetraloma::Mesh's geometric methods (mesh.rs:249-257, 434-964)."""
 
     def __init__(self, geometry: Geometry):
         self.geometry = geometry
+
+    def geometric_factors(self, rule: QuadRule, factors: Sequence[str],
+                          tile: int = 16) -> list[dict[str, QuadField]]:
+        """Every block's factors in one launch (a MeshPlan launched once)."""
+        plan = MeshPlan(self.geometry, rule, factors, tile)
+        plan.launch()
+        return plan.fields
 
     def _per_block(self, rule: QuadRule, name: str, tile: int) -> list[QuadField]:
         g = self.geometry

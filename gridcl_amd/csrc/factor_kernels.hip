@@ -16,6 +16,7 @@
 // the nodes shared by neighbouring cells in that XCD's L2.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 
 #include "geom_internal.hpp"
@@ -197,16 +198,17 @@ __device__ __noinline__ double detj_fp64(const T* __restrict__ nodes,
   return det_of<double, TD, GD>(j);
 }
 
-// One lane per cell, K nodes held in registers.
+// All requested factors of the cell of lane cl (cl < padded cells; lanes past the last cell
+// compute the last cell), K nodes held in registers.
 template <typename T, int TD, int GD, int K, bool SHIFT>
-__global__ void __launch_bounds__(kBlock)
-    factor_direct(const T* __restrict__ nodes, const uint32_t* __restrict__ dofmap,
-                  const uint32_t* __restrict__ flags, const T* __restrict__ tab,
-                  const double* __restrict__ tab64, const T* __restrict__ ref_normals,
-                  FactorParams<T> p, FactorOutputs o) {
-  const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
-  const uint64_t cl = static_cast<uint64_t>(b) * kBlock + threadIdx.x;
-  if (cl >= p.padded_cells) return;
+__device__ __forceinline__ void direct_cell(const T* __restrict__ nodes,
+                                            const uint32_t* __restrict__ dofmap,
+                                            const uint32_t* __restrict__ flags,
+                                            const T* __restrict__ tab,
+                                            const double* __restrict__ tab64,
+                                            const T* __restrict__ ref_normals,
+                                            const FactorParams<T>& p, const FactorOutputs& o,
+                                            uint64_t cl) {
   const uint64_t c = cl < p.num_cells ? cl : p.num_cells - 1;
   const uint32_t* __restrict__ row = dofmap + c * K;
 
@@ -270,6 +272,70 @@ __global__ void __launch_bounds__(kBlock)
         }
       }
       store_from_j<T, TD, GD>(j, detj, ref_normals, flags, c, q, mask, row_q, tl, lane, o);
+    }
+  }
+}
+
+// One lane per cell over one block.
+template <typename T, int TD, int GD, int K, bool SHIFT>
+__global__ void __launch_bounds__(kBlock)
+    factor_direct(const T* __restrict__ nodes, const uint32_t* __restrict__ dofmap,
+                  const uint32_t* __restrict__ flags, const T* __restrict__ tab,
+                  const double* __restrict__ tab64, const T* __restrict__ ref_normals,
+                  FactorParams<T> p, FactorOutputs o) {
+  const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
+  const uint64_t cl = static_cast<uint64_t>(b) * kBlock + threadIdx.x;
+  if (cl >= p.padded_cells) return;
+  direct_cell<T, TD, GD, K, SHIFT>(nodes, dofmap, flags, tab, tab64, ref_normals, p, o, cl);
+}
+
+// The shapes of the persistent mixed-mesh kernel, in the order of their shape index: every
+// (tdim, gdim, k) with a register-resident kernel and k <= 27.
+#define TG_HYBRID_SHAPES(X)                                                                    \
+  X(0, 3, 3, 4) X(1, 3, 3, 10) X(2, 3, 3, 20) X(3, 3, 3, 8) X(4, 3, 3, 27) X(5, 3, 3, 6)     \
+  X(6, 3, 3, 18) X(7, 3, 3, 5) X(8, 3, 3, 14) X(9, 2, 3, 3) X(10, 2, 3, 6) X(11, 2, 3, 10)    \
+  X(12, 2, 3, 4) X(13, 2, 3, 9) X(14, 2, 2, 3) X(15, 2, 2, 6) X(16, 2, 2, 10) X(17, 2, 2, 4) \
+  X(18, 2, 2, 9)
+
+// One persistent launch over the work items of every block of a mesh. Items are split into
+// eight contiguous ranges, one per XCD (workgroup b runs on XCD b % 8); the workgroups of an
+// XCD take items of their range in order from one counter. The last workgroup to finish resets
+// the counters, so the next launch (or graph replay) starts from zero.
+template <typename T, bool SHIFT>
+__global__ void __launch_bounds__(kBlock)
+    factor_hybrid(const HybridBlock<T>* __restrict__ blocks, const WorkItem* __restrict__ items,
+                  const uint32_t* __restrict__ xcd_begin, uint32_t* counters) {
+  __shared__ uint32_t s_item;
+  const uint32_t x = blockIdx.x & 7u;
+  const uint32_t begin = xcd_begin[x], end = xcd_begin[x + 1];
+  for (;;) {
+    if (threadIdx.x == 0) s_item = begin + atomicAdd(&counters[x], 1u);
+    __syncthreads();
+    const uint32_t it = __builtin_amdgcn_readfirstlane(s_item);
+    __syncthreads();
+    if (it >= end) break;
+    const WorkItem w = items[it];
+    const HybridBlock<T>* __restrict__ hb = blocks + w.block;
+    const uint64_t cl = static_cast<uint64_t>(w.first_cell) + threadIdx.x;
+    if (cl < hb->p.padded_cells) {
+      switch (hb->shape) {
+#define TG_HCASE(S_, TD_, GD_, K_)                                                            \
+  case S_:                                                                                     \
+    direct_cell<T, TD_, GD_, K_, SHIFT>(hb->nodes, hb->dofmap, hb->flags, hb->tab, hb->tab64, \
+                                        hb->ref_normals, hb->p, hb->out, cl);                  \
+    break;
+        TG_HYBRID_SHAPES(TG_HCASE)
+#undef TG_HCASE
+        default:
+          break;
+      }
+    }
+  }
+  if (threadIdx.x == 0) {
+    __threadfence();
+    if (atomicAdd(&counters[8], 1u) == gridDim.x - 1) {
+      for (int i = 0; i < 9; ++i) atomicExch(&counters[i], 0u);
+      __threadfence();
     }
   }
 }
@@ -428,6 +494,48 @@ hipError_t launch_factor(int tdim, int gdim, bool shift, bool generic, const T* 
                      dofmap, flags, tab, tab64, ref_normals, p, out);
   return hipGetLastError();
 }
+
+int hybrid_shape(int tdim, int gdim, uint32_t k) {
+#define TG_HSHAPE(S_, TD_, GD_, K_) \
+  if (tdim == TD_ && gdim == GD_ && k == K_) return S_;
+  TG_HYBRID_SHAPES(TG_HSHAPE)
+#undef TG_HSHAPE
+  return -1;
+}
+
+template <typename T>
+static auto hybrid_fn(bool shift) {
+  return shift ? factor_hybrid<T, true> : factor_hybrid<T, false>;
+}
+
+template <typename T>
+hipError_t hybrid_grid(bool shift, uint32_t* grid) {
+  int dev = 0, cus = 0, per_cu = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, hybrid_fn<T>(shift), kBlock, 0);
+  if (e != hipSuccess) return e;
+  const uint32_t n = static_cast<uint32_t>(std::max(1, cus) * std::max(1, per_cu));
+  *grid = std::max(8u, (n + 7u) / 8u * 8u);
+  return hipSuccess;
+}
+
+template <typename T>
+hipError_t launch_hybrid(bool shift, const HybridBlock<T>* blocks, const WorkItem* items,
+                         const uint32_t* xcd_begin, uint32_t* counters, uint32_t grid,
+                         hipStream_t stream) {
+  hipLaunchKernelGGL(hybrid_fn<T>(shift), dim3(grid), dim3(kBlock), 0, stream, blocks, items,
+                     xcd_begin, counters);
+  return hipGetLastError();
+}
+
+template hipError_t hybrid_grid<float>(bool, uint32_t*);
+template hipError_t hybrid_grid<double>(bool, uint32_t*);
+template hipError_t launch_hybrid<float>(bool, const HybridBlock<float>*, const WorkItem*,
+                                         const uint32_t*, uint32_t*, uint32_t, hipStream_t);
+template hipError_t launch_hybrid<double>(bool, const HybridBlock<double>*, const WorkItem*,
+                                          const uint32_t*, uint32_t*, uint32_t, hipStream_t);
 
 template <typename T>
 hipError_t launch_evector(const T* nodes, const uint32_t* dofmap, uint64_t num_cells,

@@ -42,6 +42,37 @@ struct FactorOutputs {
   void* normals;
 };
 
+// One block of a mixed-mesh launch: its arrays, scalars, outputs and shape index.
+template <typename T>
+struct HybridBlock {
+  const T* nodes;
+  const uint32_t* dofmap;
+  const uint32_t* flags;
+  const T* tab;
+  const double* tab64;
+  const T* ref_normals;
+  FactorOutputs out;
+  FactorParams<T> p;
+  uint32_t shape;
+  uint32_t pad_;
+};
+
+// One work item: 256 consecutive lanes (cells, padding included) of one block.
+struct WorkItem {
+  uint32_t block;
+  uint32_t first_cell;
+};
+
+// Shape index of (tdim, gdim, k) in the persistent mixed-mesh kernel, or -1.
+int hybrid_shape(int tdim, int gdim, uint32_t k);
+// Workgroups that fit on the device at once for the mixed-mesh kernel (a multiple of 8).
+template <typename T>
+hipError_t hybrid_grid(bool shift, uint32_t* grid);
+template <typename T>
+hipError_t launch_hybrid(bool shift, const HybridBlock<T>* blocks, const WorkItem* items,
+                         const uint32_t* xcd_begin, uint32_t* counters, uint32_t grid,
+                         hipStream_t stream);
+
 // Launches the factor kernel for (T, tdim, gdim, k); `shift` selects the per-cell origin shift,
 // `generic` forces the runtime-K kernel. Returns hipSuccess or the launch error, and
 // hipErrorNotSupported if no kernel exists for the shape.

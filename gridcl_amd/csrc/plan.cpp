@@ -1,0 +1,267 @@
+// C-ABI entry points of the mixed-mesh evaluation plan: every block of a geometry in one
+// persistent launch over a work list, optionally replayed from a captured hipGraph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "../../include/tetraloma_geom.h"
+#include "geom_internal.hpp"
+#include "handles.hpp"
+
+namespace {
+
+constexpr uint32_t kItemCells = 256;  // lanes per work item = the kernel's workgroup size
+
+struct PlanBlock {
+  int index;
+  tg_pointset* pts;
+  tg::DeviceTables* tabs;
+  uint32_t mask;
+  tg_outputs out;
+};
+
+}  // namespace
+
+struct tg_plan {
+  const tg_geometry* g = nullptr;
+  uint32_t tile = 16, options = 0;
+  bool persistent = false;
+  bool shift = false;
+  uint32_t grid = 0;
+  uint32_t num_items = 0;
+  std::vector<PlanBlock> blocks;
+  void* d_blocks = nullptr;
+  tg::WorkItem* d_items = nullptr;
+  uint32_t* d_xcd_begin = nullptr;
+  uint32_t* d_counters = nullptr;
+  hipStream_t capture = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  ~tg_plan() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (capture) (void)hipStreamDestroy(capture);
+    for (void* p : {d_blocks, static_cast<void*>(d_items), static_cast<void*>(d_xcd_begin),
+                    static_cast<void*>(d_counters)})
+      if (p) (void)hipFree(p);
+  }
+};
+
+namespace {
+
+uint32_t log2u(uint32_t v) {
+  uint32_t r = 0;
+  while ((1u << r) < v) ++r;
+  return r;
+}
+
+template <typename T>
+tg::FactorParams<T> params(const tg_geometry* g, const tg::Block& b, const tg_pointset* pts,
+                           uint32_t mask, uint32_t tile, uint32_t options) {
+  tg::FactorParams<T> p{};
+  p.num_cells = b.num_cells;
+  p.padded_cells = (b.num_cells + tile - 1) / tile * tile;
+  p.last_node = static_cast<uint32_t>(g->num_nodes - 1);
+  p.nq = static_cast<uint32_t>(pts->nq);
+  p.k = b.k;
+  p.tile_log2 = log2u(tile);
+  p.mask = mask;
+  const bool f32 = sizeof(T) == 4;
+  p.fixup = (f32 && !(options & TG_OPT_NO_FP64_FIXUP) && !(options & TG_OPT_NO_CELL_SHIFT)) ? 1u
+                                                                                            : 0u;
+  p.tau = g->tau;
+  for (int i = 0; i < 3; ++i) p.origin[i] = static_cast<T>(i < g->gdim ? g->origin[i] : 0.0);
+  return p;
+}
+
+tg::FactorOutputs outputs(const tg_outputs& o) {
+  return tg::FactorOutputs{o.points, o.jacobians, o.detj, o.pinv, o.facet_normals, o.normals};
+}
+
+template <typename T>
+int upload_persistent(tg_plan* plan) {
+  const tg_geometry* g = plan->g;
+  std::vector<tg::HybridBlock<T>> hb;
+  std::vector<double> cost;  // per plan block: work per lane
+  for (const PlanBlock& pb : plan->blocks) {
+    const tg::Block& b = g->blocks[pb.index];
+    tg::HybridBlock<T> h{};
+    h.nodes = static_cast<const T*>(g->nodes);
+    h.dofmap = b.dofmap;
+    h.flags = b.flags;
+    h.tab = static_cast<const T*>(pb.tabs->tab);
+    h.tab64 = pb.tabs->tab64;
+    h.ref_normals = static_cast<const T*>(pb.tabs->ref_normals);
+    h.out = outputs(pb.out);
+    h.p = params<T>(g, b, pb.pts, pb.mask, plan->tile, plan->options);
+    h.shape = static_cast<uint32_t>(tg::hybrid_shape(b.tdim, g->gdim, b.k));
+    hb.push_back(h);
+    cost.push_back(static_cast<double>(pb.pts->nq) * (b.k + 4.0) * (b.tdim + 1.0));
+  }
+  // Work items block by block; eight contiguous ranges of about equal cost, one per XCD.
+  std::vector<tg::WorkItem> items;
+  std::vector<double> item_cost;
+  for (size_t i = 0; i < hb.size(); ++i)
+    for (uint64_t c = 0; c < hb[i].p.padded_cells; c += kItemCells) {
+      items.push_back(tg::WorkItem{static_cast<uint32_t>(i), static_cast<uint32_t>(c)});
+      item_cost.push_back(cost[i]);
+    }
+  double total = 0.0;
+  for (double c : item_cost) total += c;
+  std::vector<uint32_t> begin(9, static_cast<uint32_t>(items.size()));
+  begin[0] = 0;
+  double acc = 0.0;
+  int x = 1;
+  for (size_t i = 0; i < items.size() && x < 8; ++i) {
+    acc += item_cost[i];
+    while (x < 8 && acc >= total * x / 8.0) begin[x++] = static_cast<uint32_t>(i + 1);
+  }
+  plan->num_items = static_cast<uint32_t>(items.size());
+  hipError_t e = hipMalloc(&plan->d_blocks, std::max<size_t>(1, hb.size()) * sizeof(hb[0]));
+  if (e == hipSuccess)
+    e = hipMalloc(reinterpret_cast<void**>(&plan->d_items),
+                  std::max<size_t>(1, items.size()) * sizeof(tg::WorkItem));
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&plan->d_xcd_begin), 9 * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&plan->d_counters), 9 * sizeof(uint32_t));
+  if (e == hipSuccess && !hb.empty())
+    e = hipMemcpy(plan->d_blocks, hb.data(), hb.size() * sizeof(hb[0]), hipMemcpyHostToDevice);
+  if (e == hipSuccess && !items.empty())
+    e = hipMemcpy(plan->d_items, items.data(), items.size() * sizeof(tg::WorkItem),
+                  hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(plan->d_xcd_begin, begin.data(), 9 * sizeof(uint32_t), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(plan->d_counters, 0, 9 * sizeof(uint32_t));
+  if (e == hipSuccess) e = tg::hybrid_grid<T>(plan->shift, &plan->grid);
+  if (e != hipSuccess) return tg::hip_fail(e, "plan upload");
+  plan->grid = std::min(plan->grid, (plan->num_items + 15u) / 8u * 8u);
+  plan->grid = std::max(plan->grid, 8u);
+  return TG_OK;
+}
+
+template <typename T>
+hipError_t launch_blocks(tg_plan* plan, hipStream_t s) {
+  const tg_geometry* g = plan->g;
+  for (const PlanBlock& pb : plan->blocks) {
+    const tg::Block& b = g->blocks[pb.index];
+    tg::FactorParams<T> p = params<T>(g, b, pb.pts, pb.mask, plan->tile, plan->options);
+    hipError_t e = tg::launch_factor<T>(
+        b.tdim, g->gdim, plan->shift, (plan->options & TG_OPT_FORCE_GENERIC) != 0,
+        static_cast<const T*>(g->nodes), b.dofmap, b.flags, static_cast<const T*>(pb.tabs->tab),
+        pb.tabs->tab64, static_cast<const T*>(pb.tabs->ref_normals), p, outputs(pb.out), s);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+hipError_t plan_enqueue(tg_plan* plan, hipStream_t s) {
+  if (plan->num_items == 0 && plan->persistent) return hipSuccess;
+  if (plan->g->dtype == TG_F32) {
+    if (plan->persistent)
+      return tg::launch_hybrid<float>(plan->shift,
+                                      static_cast<const tg::HybridBlock<float>*>(plan->d_blocks),
+                                      plan->d_items, plan->d_xcd_begin, plan->d_counters,
+                                      plan->grid, s);
+    return launch_blocks<float>(plan, s);
+  }
+  if (plan->persistent)
+    return tg::launch_hybrid<double>(plan->shift,
+                                     static_cast<const tg::HybridBlock<double>*>(plan->d_blocks),
+                                     plan->d_items, plan->d_xcd_begin, plan->d_counters,
+                                     plan->grid, s);
+  return launch_blocks<double>(plan, s);
+}
+
+}  // namespace
+
+extern "C" {
+
+int tg_plan_create(const tg_geometry* g, tg_pointset* const* pts, uint32_t factors,
+                   uint32_t tile, uint32_t options, const tg_outputs* outs, tg_plan** out) {
+  if (!g || !pts || !outs || !out) return tg::fail(TG_ERR_INVALID_ARGUMENT, "null argument");
+  *out = nullptr;
+  if (tile != 1 && tile != 2 && tile != 4 && tile != 8 && tile != 16)
+    return tg::fail(TG_ERR_INVALID_ARGUMENT, "tile must be 1, 2, 4, 8 or 16");
+  const uint32_t all = TG_POINTS | TG_JACOBIAN | TG_DETJ | TG_PINV | TG_FACET_NORMALS | TG_NORMALS;
+  if (factors == 0 || (factors & ~all)) return tg::fail(TG_ERR_INVALID_ARGUMENT, "bad factor mask");
+  auto plan = std::make_unique<tg_plan>();
+  plan->g = g;
+  plan->tile = tile;
+  plan->options = options;
+  plan->shift = g->dtype == TG_F32 && !(options & TG_OPT_NO_CELL_SHIFT);
+  bool all_hybrid = !(options & TG_OPT_FORCE_GENERIC) && (g->dtype == TG_F64 || plan->shift);
+  for (size_t bi = 0; bi < g->blocks.size(); ++bi) {
+    const tg::Block& b = g->blocks[bi];
+    tg_pointset* p = pts[bi];
+    if (!p) return tg::fail(TG_ERR_CELL_NOT_IN_RULE, std::string("CellNotInRule(\"") +
+                                                          tg::cell_name(b.cell) + "\")");
+    if (p->cell != b.cell)
+      return tg::fail(TG_ERR_CELL_NOT_IN_RULE,
+                      std::string("CellNotInRule(\"") + tg::cell_name(b.cell) + "\")");
+    uint32_t allowed = TG_POINTS | TG_JACOBIAN | TG_DETJ | TG_PINV;
+    if (b.tdim == g->gdim && p->has_normals) allowed |= TG_FACET_NORMALS;
+    if (b.tdim + 1 == g->gdim) allowed |= TG_NORMALS;
+    const uint32_t mask = factors & allowed;
+    if (mask == 0 || b.num_cells == 0 || p->nq == 0) continue;
+    const void* ptrs[6] = {outs[bi].points, outs[bi].jacobians, outs[bi].detj,
+                           outs[bi].pinv, outs[bi].facet_normals, outs[bi].normals};
+    for (int i = 0; i < 6; ++i)
+      if ((mask & (1u << i)) && !ptrs[i])
+        return tg::fail(TG_ERR_INVALID_ARGUMENT,
+                        "block " + std::to_string(bi) + ": missing output for a requested factor");
+    PlanBlock pb{static_cast<int>(bi), p, nullptr, mask, outs[bi]};
+    int rc = tg::device_tables(p, b.degree, b.family, g->dtype, &pb.tabs);
+    if (rc) return rc;
+    if (tg::hybrid_shape(b.tdim, g->gdim, b.k) < 0) all_hybrid = false;
+    plan->blocks.push_back(pb);
+  }
+  plan->persistent = all_hybrid;
+  if (plan->persistent) {
+    int rc = g->dtype == TG_F32 ? upload_persistent<float>(plan.get())
+                                : upload_persistent<double>(plan.get());
+    if (rc) return rc;
+  }
+  *out = plan.release();
+  return TG_OK;
+}
+
+int tg_plan_destroy(tg_plan* plan) {
+  delete plan;
+  return TG_OK;
+}
+
+int tg_plan_info(const tg_plan* plan, uint32_t* num_items, uint32_t* grid, int32_t* persistent) {
+  if (!plan) return tg::fail(TG_ERR_INVALID_ARGUMENT, "null plan");
+  if (num_items) *num_items = plan->num_items;
+  if (grid) *grid = plan->grid;
+  if (persistent) *persistent = plan->persistent ? 1 : 0;
+  return TG_OK;
+}
+
+int tg_plan_launch(tg_plan* plan, void* stream) {
+  if (!plan) return tg::fail(TG_ERR_INVALID_ARGUMENT, "null plan");
+  hipError_t e = plan_enqueue(plan, static_cast<hipStream_t>(stream));
+  return e == hipSuccess ? TG_OK : tg::hip_fail(e, "plan launch");
+}
+
+int tg_plan_graph_launch(tg_plan* plan, void* stream) {
+  if (!plan) return tg::fail(TG_ERR_INVALID_ARGUMENT, "null plan");
+  hipError_t e = hipSuccess;
+  if (!plan->exec) {
+    // Capture on a private stream of default priority, once; replays launch on `stream`.
+    e = hipStreamCreateWithFlags(&plan->capture, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamBeginCapture(plan->capture, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+      hipError_t le = plan_enqueue(plan, plan->capture);
+      e = hipStreamEndCapture(plan->capture, &plan->graph);
+      if (le != hipSuccess) e = le;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&plan->exec, plan->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) return tg::hip_fail(e, "plan graph capture");
+  }
+  e = hipGraphLaunch(plan->exec, static_cast<hipStream_t>(stream));
+  return e == hipSuccess ? TG_OK : tg::hip_fail(e, "plan graph launch");
+}
+
+}  // extern "C"

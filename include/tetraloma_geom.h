@@ -183,6 +183,29 @@ int tg_eval_block(const tg_geometry* g, int32_t block, tg_pointset* pts, uint32_
 int tg_evector_build(const tg_geometry* g, int32_t block, void* out, void* stream);
 uint64_t tg_evector_elems(const tg_geometry* g, int32_t block);
 
+/* ---- Mixed meshes: every block in one launch ---------------------------------------------------
+ *
+ * A plan binds, for every block b of a geometry, the point set pts[b] (of the block's cell type)
+ * and the outputs outs[b] of the factors in `factors` that apply to it (normals on blocks of
+ * codimension 1, facet normals on blocks of codimension 0 with a facet point set; the others on
+ * every block). Replaces the reference's per-block loop over launch_factor
+ * (tetraloma/src/mesh.rs:249-257 per_block; tetraloma-geometry/src/layout.rs:488) with one
+ * persistent kernel over a work list of 256-cell items of all blocks, split in eight ranges of
+ * equal work, one per XCD. Blocks whose element has no register-resident kernel (more than 27
+ * nodes per cell, or forced generic) make the plan fall back to one launch per block.
+ * Outputs are laid out exactly as tg_eval_block's. */
+typedef struct tg_plan tg_plan;
+int tg_plan_create(const tg_geometry* g, tg_pointset* const* pts, uint32_t factors,
+                   uint32_t tile, uint32_t options, const tg_outputs* outs, tg_plan** out);
+int tg_plan_destroy(tg_plan* plan);
+/* Work items, workgroups of the persistent launch, and whether the plan is one launch. */
+int tg_plan_info(const tg_plan* plan, uint32_t* num_items, uint32_t* grid, int32_t* persistent);
+/* Enqueues the plan on `stream`. */
+int tg_plan_launch(tg_plan* plan, void* stream);
+/* Captures the plan into a hipGraph on first use (on a private stream) and replays the graph on
+ * `stream`. The plan's pointers are fixed at capture: later replays write the same outputs. */
+int tg_plan_graph_launch(tg_plan* plan, void* stream);
+
 /* ---- The mesh functional and its adjoint (no reference counterpart) --------------------------
  *
  * F(X) = sum_c sum_q w_q det J_cq(X) + beta sum_c sum_q w_q |J_cq(X)|_F^2 / det(J_cq(X))^(2/3)
