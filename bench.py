@@ -98,6 +98,92 @@ def cpu_baseline(mesh, ps, sample_cells: int):
                       f"{calls} calls"}, out
 
 
+def bench_hybrid(api, M, steps, warmup, scale=1.0):
+    """The mixed mesh (P2 tets, Q2 hexes, P2 prisms, P1 pyramids, P3 surface triangles with
+    normals; Gauss degree 4): one persistent launch, its graph replay, and per-block launches."""
+    import torch
+
+    h = M.config4_mesh(scale=scale)
+    g = api.geometry_from_host(h, dtype=torch.float32)
+    rule = api.QuadRule.for_grid(g, 4)
+    factors = ["points", "jacobians", "detj", "normals"]
+    plan = api.MeshPlan(g, rule, factors)
+    evals = sum(b.num_cells * len(rule.get(b.cell)) for b in h.blocks)
+    nbytes = 0
+    for b, blk in enumerate(h.blocks):
+        nq = len(rule.get(blk.cell))
+        td = M.TDIM[blk.cell]
+        length = 3 + 3 * td + 1 + (3 if td == 2 else 0)
+        nbytes += 4 * blk.nodes_per_cell * blk.num_cells + length * 4 * nq * blk.num_cells
+    nbytes += 3 * 4 * h.num_nodes
+    ms_plan = time_steps(plan.launch, steps, warmup)
+    ms_graph = time_steps(plan.graph_launch, steps, warmup)
+    outs = [{n: f.data for n, f in fb.items()} for fb in plan.fields]
+    sets = [rule.get(b.cell) for b in h.blocks]
+
+    def per_block():
+        for b in range(len(h.blocks)):
+            api.geometric_factors_block(g, b, sets[b], list(outs[b]), out=outs[b])
+
+    ms_blocks = time_steps(per_block, steps, warmup)
+    return {
+        "blocks": [(int(b.cell), int(b.degree), b.num_cells) for b in h.blocks],
+        "evals": evals, "persistent": plan.persistent, "work_items": plan.num_items,
+        "grid": plan.grid,
+        "ms_persistent": ms_plan, "ms_graph": ms_graph, "ms_per_block_launches": ms_blocks,
+        "evals_per_s_graph": evals / (ms_graph * 1e-3),
+        "achieved_gbs_graph": nbytes / (ms_graph * 1e-3) / 1e9,
+    }
+
+
+def bench_adjoint(api, M, steps, warmup):
+    """Forward functional and its vjp on the config-5 mesh (P2 tets 32^3 x 6, 14 points)."""
+    import torch
+
+    h = M.config5_mesh()
+    ps = api.PointSet.tet_14()
+    out = {}
+    for dtype, tdt in (("f32", torch.float32), ("f64", torch.float64)):
+        g = api.geometry_from_host(h, dtype=tdt)
+        fn = api.MeshFunctional(g, 0, ps, beta=0.1)
+        val = torch.empty((), dtype=torch.float64, device="cuda")
+        grad = torch.empty(g.nodes.shape, dtype=torch.float64, device="cuda")
+        lib = api._lib.load()
+
+        def fwd():
+            lib.tg_functional_forward(fn._h, None, 0.1, val.data_ptr(), api._stream())
+
+        def vjp():
+            lib.tg_functional_vjp(fn._h, None, 0.1, None, grad.data_ptr(), api._stream())
+
+        ms_f = time_steps(fwd, steps, warmup)
+        ms_b = time_steps(vjp, steps, warmup)
+        out[dtype] = {"ms_forward": ms_f, "ms_vjp": ms_b, "vjp_over_forward": ms_b / ms_f}
+        del fn, g
+    out["cells"] = h.blocks[0].num_cells
+    out["nodes"] = h.num_nodes
+    return out
+
+
+def bench_q3(api, M, steps, warmup, n=96):
+    """Q3 hexahedra (96^3 cells, 64 nodes and 64 Gauss points per cell), X + J + det J, FP32."""
+    import torch
+
+    h = M.config3_mesh(n=n)
+    g = api.geometry_from_host(h, dtype=torch.float32)
+    ps = api.PointSet.gauss(5, 7)
+    names = ["points", "jacobians", "detj"]
+    C, N = h.blocks[0].num_cells, h.num_nodes
+    out = {nm: torch.empty(int(api._lib.load().tg_field_elems(C, 64, api.factor_len(nm, 3, 3),
+                                                               16)),
+                           dtype=torch.float32, device="cuda") for nm in names}
+    ms = time_steps(lambda: api.geometric_factors_block(g, 0, ps, names, out=out), steps, warmup)
+    nbytes = algorithmic_bytes(64, C, N, 64, 4)
+    return {"cells": C, "nodes": N, "ms": ms, "evals_per_s": C * 64 / (ms * 1e-3),
+            "achieved_gbs": nbytes / (ms * 1e-3) / 1e9, "kernel": "direct FP32 (64 nodes in "
+            "registers)"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,6 +192,8 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=64, help="hexahedra per direction (64: full size)")
     ap.add_argument("--cpu-sample", type=int, default=262_144)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the Q3, mixed-mesh and adjoint legs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding "
@@ -177,6 +265,19 @@ def main() -> int:
         del a, b
     log(f"FP32 vs FP64 kernel (whole mesh, incl. FP32 storage of nodes): {vs_fp64_kernel}")
 
+    extras = None
+    if not args.no_extras:
+        del fields
+        torch.cuda.empty_cache()
+        extras = {}
+        extras["adjoint_p2tet_32"] = bench_adjoint(api, M, args.steps, args.warmup)
+        log(f"adjoint: {extras['adjoint_p2tet_32']}")
+        extras["hybrid"] = bench_hybrid(api, M, args.steps, args.warmup)
+        log(f"hybrid: {extras['hybrid']}")
+        torch.cuda.empty_cache()
+        extras["q3_hex_96"] = bench_q3(api, M, max(5, args.steps // 5), 2)
+        log(f"q3: {extras['q3_hex_96']}")
+
     r32, r64 = results["f32"], results["f64"]
     line = {
         "metric": METRIC,
@@ -208,6 +309,7 @@ def main() -> int:
         "frac_of_copy_ceiling": r32["achieved_gbs"] / ceiling,
         "fp32_rel_err_vs_fp64_sample": accuracy,
         "fp32_rel_err_vs_fp64_kernel": vs_fp64_kernel,
+        "extras": extras,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -198,8 +198,48 @@ __device__ __noinline__ double detj_fp64(const T* __restrict__ nodes,
   return det_of<double, TD, GD>(j);
 }
 
-// All requested factors of the cell of lane cl (cl < padded cells; lanes past the last cell
-// compute the last cell), K nodes held in registers.
+// Orders the LDS accesses of a wavefront before and after this point.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+  typedef float type __attribute__((ext_vector_type(4)));
+};
+template <>
+struct Vec16<double> {
+  typedef double type __attribute__((ext_vector_type(2)));
+};
+
+// Copies one staged field of a wavefront at point q: for each of its four tiles the LEN * 16
+// values of (tile, q) are contiguous both in the staging buffer and in the QuadField, so they
+// leave as 16-byte stores. Tiles at or past `ntiles` are padding of the launch and skipped.
+template <typename T, int LEN>
+__device__ __forceinline__ void copy_staged(const T* __restrict__ stage, void* out, uint64_t tile0,
+                                            uint64_t ntiles, uint32_t nq, uint32_t q, uint32_t w) {
+  using V = typename Vec16<T>::type;
+  constexpr uint32_t E = 16 / sizeof(T);
+  constexpr uint32_t per_tile = LEN * 16 / E;
+  for (uint32_t s = w; s < 4 * per_tile; s += 64) {
+    const uint32_t t = s / per_tile, jj = s - t * per_tile;
+    const uint64_t ta = tile0 + t;
+    if (ta < ntiles) {
+      const V v = *reinterpret_cast<const V*>(stage + t * LEN * 16 + jj * E);
+      T* dst = static_cast<T*>(out) + ((ta * nq + q) * LEN) * 16 + jj * E;
+      __builtin_nontemporal_store(v, reinterpret_cast<V*>(dst));
+    }
+  }
+}
+
+// All requested factors of the cell of lane cl, K nodes held in registers. Lanes past the last
+// cell compute the last cell (the padding of the field). With a staging buffer (tiles of 16)
+// the whole wavefront must call: lanes at or past the padded cell count then compute the last
+// padded lane's cell and write nothing of their own.
 template <typename T, int TD, int GD, int K, bool SHIFT>
 __device__ __forceinline__ void direct_cell(const T* __restrict__ nodes,
                                             const uint32_t* __restrict__ dofmap,
@@ -208,7 +248,9 @@ __device__ __forceinline__ void direct_cell(const T* __restrict__ nodes,
                                             const double* __restrict__ tab64,
                                             const T* __restrict__ ref_normals,
                                             const FactorParams<T>& p, const FactorOutputs& o,
-                                            uint64_t cl) {
+                                            uint64_t cl_in, T* __restrict__ stage) {
+  const bool live = cl_in < p.padded_cells;
+  const uint64_t cl = live ? cl_in : p.padded_cells - 1;
   const uint64_t c = cl < p.num_cells ? cl : p.num_cells - 1;
   const uint32_t* __restrict__ row = dofmap + c * K;
 
@@ -227,11 +269,23 @@ __device__ __forceinline__ void direct_cell(const T* __restrict__ nodes,
       for (int g = 0; g < GD; ++g) x[k][g] -= x[0][g];
   }
   constexpr int K0 = SHIFT ? 1 : 0;  // first node whose offset is not identically zero
+  constexpr int JL = TD * GD;
   const uint32_t tl = p.tile_log2;
   const uint64_t tile = cl >> tl;
   const uint32_t lane = static_cast<uint32_t>(cl) & ((1u << tl) - 1u);
   const uint32_t mask = p.mask;
   const bool need_j = (mask & ~kFPoints) != 0;
+  // Staged stores (tiles of 16 only): X, J and det J of the wavefront's 64 cells, four whole
+  // tiles, go to LDS in their global order, then leave as 16-byte non-temporal stores.
+  const bool staged = stage != nullptr;
+  const uint32_t w = threadIdx.x & 63u;
+  const uint32_t wt = w >> 4, wl = w & 15u;
+  const uint64_t tile0 = (cl_in - w) >> 4;
+  const uint64_t ntiles = p.padded_cells >> 4;
+  const uint32_t ex = 0;
+  const uint32_t ej = ex + ((mask & kFPoints) ? 64u * GD : 0u);
+  const uint32_t ed = ej + ((mask & kFJac) ? 64u * JL : 0u);
+  const uint32_t own = !live ? 0u : staged ? (mask & ~(kFPoints | kFJac | kFDetJ)) : mask;
 
   for (uint32_t q = 0; q < p.nq; ++q) {
     const T* __restrict__ t = tab + static_cast<uint64_t>(q) * K * 4;
@@ -243,11 +297,14 @@ __device__ __forceinline__ void direct_cell(const T* __restrict__ nodes,
 #pragma unroll
         for (int k = K0 + 1; k < K; ++k) acc = tfma(x[k][g], t[4 * k], acc);
         const T v = SHIFT ? (x[0][g] + acc) + p.origin[g] : acc + p.origin[g];
-        st<T>(o.x, (((row_q * GD) + g) << tl) | lane, v);
+        if (staged)
+          stage[ex + (wt * GD + g) * 16 + wl] = v;
+        else if (live)
+          st<T>(o.x, (((row_q * GD) + g) << tl) | lane, v);
       }
     }
     if (need_j) {
-      T j[TD * GD];
+      T j[JL];
 #pragma unroll
       for (int d = 0; d < TD; ++d)
 #pragma unroll
@@ -264,17 +321,34 @@ __device__ __forceinline__ void direct_cell(const T* __restrict__ nodes,
           if (p.fixup) {
             T n2 = T(0);
 #pragma unroll
-            for (int i = 0; i < TD * GD; ++i) n2 = tfma(j[i], j[i], n2);
+            for (int i = 0; i < JL; ++i) n2 = tfma(j[i], j[i], n2);
             if (__builtin_fabsf(detj) < static_cast<T>(p.tau) * n2 * tsqrt(n2))
               detj = static_cast<T>(
                   detj_fp64<T, TD, GD>(nodes, row, K, p.last_node, tab64, q));
           }
         }
       }
-      store_from_j<T, TD, GD>(j, detj, ref_normals, flags, c, q, mask, row_q, tl, lane, o);
+      if (staged) {
+        if (mask & kFJac) {
+#pragma unroll
+          for (int i = 0; i < JL; ++i) stage[ej + (wt * JL + i) * 16 + wl] = j[i];
+        }
+        if (mask & kFDetJ) stage[ed + wt * 16 + wl] = detj;
+      }
+      if (own) store_from_j<T, TD, GD>(j, detj, ref_normals, flags, c, q, own, row_q, tl, lane, o);
+    }
+    if (staged) {
+      wave_sync();
+      if (mask & kFPoints) copy_staged<T, GD>(stage + ex, o.x, tile0, ntiles, p.nq, q, w);
+      if (mask & kFJac) copy_staged<T, JL>(stage + ej, o.j, tile0, ntiles, p.nq, q, w);
+      if (mask & kFDetJ) copy_staged<T, 1>(stage + ed, o.detj, tile0, ntiles, p.nq, q, w);
+      wave_sync();
     }
   }
 }
+
+// Elements of the staging buffer per wavefront (X, J, det J of 64 cells in 3D).
+constexpr int kStageElems = 64 * 13;
 
 // One lane per cell over one block.
 template <typename T, int TD, int GD, int K, bool SHIFT>
@@ -283,10 +357,20 @@ __global__ void __launch_bounds__(kBlock)
                   const uint32_t* __restrict__ flags, const T* __restrict__ tab,
                   const double* __restrict__ tab64, const T* __restrict__ ref_normals,
                   FactorParams<T> p, FactorOutputs o) {
+  __shared__ __align__(16) T lds[kBlock / 64][kStageElems];
   const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
   const uint64_t cl = static_cast<uint64_t>(b) * kBlock + threadIdx.x;
-  if (cl >= p.padded_cells) return;
-  direct_cell<T, TD, GD, K, SHIFT>(nodes, dofmap, flags, tab, tab64, ref_normals, p, o, cl);
+  const bool staged = p.tile_log2 == 4;
+  if (staged) {
+    // Whole wavefronts stay (a wavefront entirely past the padding leaves).
+    if (cl - (threadIdx.x & 63u) >= p.padded_cells) return;
+    direct_cell<T, TD, GD, K, SHIFT>(nodes, dofmap, flags, tab, tab64, ref_normals, p, o, cl,
+                                     lds[threadIdx.x >> 6]);
+  } else {
+    if (cl >= p.padded_cells) return;
+    direct_cell<T, TD, GD, K, SHIFT>(nodes, dofmap, flags, tab, tab64, ref_normals, p, o, cl,
+                                     nullptr);
+  }
 }
 
 // The shapes of the persistent mixed-mesh kernel, in the order of their shape index: every
@@ -306,6 +390,7 @@ __global__ void __launch_bounds__(kBlock)
     factor_hybrid(const HybridBlock<T>* __restrict__ blocks, const WorkItem* __restrict__ items,
                   const uint32_t* __restrict__ xcd_begin, uint32_t* counters) {
   __shared__ uint32_t s_item;
+  __shared__ __align__(16) T lds[kBlock / 64][kStageElems];
   const uint32_t x = blockIdx.x & 7u;
   const uint32_t begin = xcd_begin[x], end = xcd_begin[x + 1];
   for (;;) {
@@ -317,12 +402,15 @@ __global__ void __launch_bounds__(kBlock)
     const WorkItem w = items[it];
     const HybridBlock<T>* __restrict__ hb = blocks + w.block;
     const uint64_t cl = static_cast<uint64_t>(w.first_cell) + threadIdx.x;
-    if (cl < hb->p.padded_cells) {
+    const bool staged = hb->p.tile_log2 == 4;
+    // Staged: whole wavefronts call (one entirely past the padding skips); otherwise lanes.
+    if ((staged ? cl - (threadIdx.x & 63u) : cl) < hb->p.padded_cells) {
+      T* stage = staged ? lds[threadIdx.x >> 6] : nullptr;
       switch (hb->shape) {
 #define TG_HCASE(S_, TD_, GD_, K_)                                                            \
   case S_:                                                                                     \
     direct_cell<T, TD_, GD_, K_, SHIFT>(hb->nodes, hb->dofmap, hb->flags, hb->tab, hb->tab64, \
-                                        hb->ref_normals, hb->p, hb->out, cl);                  \
+                                        hb->ref_normals, hb->p, hb->out, cl, stage);           \
     break;
         TG_HYBRID_SHAPES(TG_HCASE)
 #undef TG_HCASE

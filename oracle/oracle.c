@@ -520,8 +520,30 @@ static double det_of(const double* j, int td, int gd) {
   return td == 1 ? sqrt(a) : sqrt(a * c - b * b);
 }
 
-/* K = (J^T J)^-1 J^T via G = J^T J and its inverse (reference.rs:197-256). */
+/* K = J^-1 by the adjugate when tdim = gdim (layout.rs:219-246), else (J^T J)^-1 J^T via
+ * G = J^T J and its inverse (reference.rs:197-256). */
 static void pinv_of(const double* j, int td, int gd, double* k) {
+  if (td == gd && td == 3) {
+    double inv = 1.0 / det_of(j, 3, 3);
+    k[0] = (j[4] * j[8] - j[7] * j[5]) * inv;
+    k[3] = (j[6] * j[5] - j[3] * j[8]) * inv;
+    k[6] = (j[3] * j[7] - j[6] * j[4]) * inv;
+    k[1] = (j[7] * j[2] - j[1] * j[8]) * inv;
+    k[4] = (j[0] * j[8] - j[6] * j[2]) * inv;
+    k[7] = (j[6] * j[1] - j[0] * j[7]) * inv;
+    k[2] = (j[1] * j[5] - j[4] * j[2]) * inv;
+    k[5] = (j[3] * j[2] - j[0] * j[5]) * inv;
+    k[8] = (j[0] * j[4] - j[3] * j[1]) * inv;
+    return;
+  }
+  if (td == gd && td == 2) {
+    double inv = 1.0 / (j[0] * j[3] - j[2] * j[1]);
+    k[0] = j[3] * inv;
+    k[2] = -j[2] * inv;
+    k[1] = -j[1] * inv;
+    k[3] = j[0] * inv;
+    return;
+  }
   double G[9] = {0}, Gi[9] = {0};
   for (int a = 0; a < td; ++a)
     for (int b = 0; b < td; ++b) {

@@ -40,8 +40,13 @@ def line_mesh(n, p, gdim, seed=5):
 
 
 def volume_mesh(cell, p, n=3, seed=11):
+    """A box curved by a smooth field that vanishes on its boundary (like the reference's
+    HostMesh::perturbed, meshes/mod.rs:150-200) plus a small jitter of the interior nodes."""
     m = M.cube_box(cell, n, p)
-    return M.jitter(m, 0.1 / (n * p), seed, interior_only=True)
+    x = m.nodes
+    bump = np.prod(np.sin(np.pi * x), axis=1)[:, None] * np.array([1.0, 0.7, 0.4])
+    m = M.HostMesh(3, x + 0.15 / n * bump, m.blocks)
+    return M.jitter(m, 0.02 / (n * p), seed, interior_only=True)
 
 
 def surface_mesh(cell, p, gdim, n=4, seed=12):
