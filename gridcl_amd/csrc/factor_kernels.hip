@@ -216,49 +216,16 @@ struct Vec16<double> {
   typedef double type __attribute__((ext_vector_type(2)));
 };
 
-// Copies one staged field of a wavefront at point q: for each of its four tiles the LEN * 16
-// values of (tile, q) are contiguous both in the staging buffer and in the QuadField, so they
-// leave as 16-byte stores. Tiles at or past `ntiles` are padding of the launch and skipped.
-template <typename T, int LEN>
-__device__ __forceinline__ void copy_staged(const T* __restrict__ stage, void* out, uint64_t tile0,
-                                            uint64_t ntiles, uint32_t nq, uint32_t q, uint32_t w) {
-  using V = typename Vec16<T>::type;
-  constexpr uint32_t E = 16 / sizeof(T);
-  constexpr uint32_t per_tile = LEN * 16 / E;
-  for (uint32_t s = w; s < 4 * per_tile; s += 64) {
-    const uint32_t t = s / per_tile, jj = s - t * per_tile;
-    const uint64_t ta = tile0 + t;
-    if (ta < ntiles) {
-      const V v = *reinterpret_cast<const V*>(stage + t * LEN * 16 + jj * E);
-      T* dst = static_cast<T*>(out) + ((ta * nq + q) * LEN) * 16 + jj * E;
-      __builtin_nontemporal_store(v, reinterpret_cast<V*>(dst));
-    }
-  }
-}
-
-// All requested factors of the cell of lane cl, K nodes held in registers. Lanes past the last
-// cell compute the last cell (the padding of the field). With a staging buffer (tiles of 16)
-// the whole wavefront must call: lanes at or past the padded cell count then compute the last
-// padded lane's cell and write nothing of their own.
-template <typename T, int TD, int GD, int K, bool SHIFT>
-__device__ __forceinline__ void direct_cell(const T* __restrict__ nodes,
-                                            const uint32_t* __restrict__ dofmap,
-                                            const uint32_t* __restrict__ flags,
-                                            const T* __restrict__ tab,
-                                            const double* __restrict__ tab64,
-                                            const T* __restrict__ ref_normals,
-                                            const FactorParams<T>& p, const FactorOutputs& o,
-                                            uint64_t cl_in, T* __restrict__ stage) {
-  const bool live = cl_in < p.padded_cells;
-  const uint64_t cl = live ? cl_in : p.padded_cells - 1;
-  const uint64_t c = cl < p.num_cells ? cl : p.num_cells - 1;
-  const uint32_t* __restrict__ row = dofmap + c * K;
-
-  T x[K][GD];
+// Gathers the K nodes of cell c (indices clamped to the last node); with SHIFT, nodes 1..K-1
+// become offsets from node 0.
+template <typename T, int GD, int K, bool SHIFT>
+__device__ __forceinline__ void gather_cell(const T* __restrict__ nodes,
+                                            const uint32_t* __restrict__ row, uint32_t last_node,
+                                            T (&x)[K][GD]) {
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     uint32_t n = row[k];
-    n = n < p.last_node ? n : p.last_node;
+    n = n < last_node ? n : last_node;
 #pragma unroll
     for (int g = 0; g < GD; ++g) x[k][g] = nodes[static_cast<uint64_t>(n) * GD + g];
   }
@@ -268,89 +235,168 @@ __device__ __forceinline__ void direct_cell(const T* __restrict__ nodes,
 #pragma unroll
       for (int g = 0; g < GD; ++g) x[k][g] -= x[0][g];
   }
+}
+
+// X (without the origin) and J of the gathered cell at the point whose table row is t; FP32
+// adds node 0 back to X after the contraction of the offsets.
+template <typename T, int TD, int GD, int K, bool SHIFT>
+__device__ __forceinline__ void contract(const T (&x)[K][GD], const T* __restrict__ t,
+                                         bool want_x, bool want_j, T* xv, T* j) {
   constexpr int K0 = SHIFT ? 1 : 0;  // first node whose offset is not identically zero
+  if (want_x) {
+#pragma unroll
+    for (int g = 0; g < GD; ++g) {
+      T acc = x[K0][g] * t[4 * K0];
+#pragma unroll
+      for (int k = K0 + 1; k < K; ++k) acc = tfma(x[k][g], t[4 * k], acc);
+      xv[g] = SHIFT ? x[0][g] + acc : acc;
+    }
+  }
+  if (want_j) {
+#pragma unroll
+    for (int d = 0; d < TD; ++d)
+#pragma unroll
+      for (int g = 0; g < GD; ++g) {
+        T acc = x[K0][g] * t[4 * K0 + 1 + d];
+#pragma unroll
+        for (int k = K0 + 1; k < K; ++k) acc = tfma(x[k][g], t[4 * k + 1 + d], acc);
+        j[d * GD + g] = acc;
+      }
+  }
+}
+
+// det J, recomputed in FP64 from the stored coordinates where |det J| < tau |J|_F^3 (FP32).
+template <typename T, int TD, int GD, int K>
+__device__ __forceinline__ T detj_checked(const T* j, const FactorParams<T>& p,
+                                          const T* __restrict__ nodes,
+                                          const uint32_t* __restrict__ row,
+                                          const double* __restrict__ tab64, uint32_t q) {
+  T detj = det_of<T, TD, GD>(j);
+  if constexpr (TD == GD && sizeof(T) == 4) {
+    if (p.fixup) {
+      T n2 = T(0);
+#pragma unroll
+      for (int i = 0; i < TD * GD; ++i) n2 = tfma(j[i], j[i], n2);
+      if (__builtin_fabsf(detj) < static_cast<T>(p.tau) * n2 * tsqrt(n2))
+        detj = static_cast<T>(detj_fp64<T, TD, GD>(nodes, row, K, p.last_node, tab64, q));
+    }
+  }
+  return detj;
+}
+
+// All requested factors of the cell of lane cl (cl < padded cells; lanes past the last cell
+// compute the last cell, the padding of the field), stored straight from registers. Any tile.
+template <typename T, int TD, int GD, int K, bool SHIFT>
+__device__ __forceinline__ void direct_cell(const T* __restrict__ nodes,
+                                            const uint32_t* __restrict__ dofmap,
+                                            const uint32_t* __restrict__ flags,
+                                            const T* __restrict__ tab,
+                                            const double* __restrict__ tab64,
+                                            const T* __restrict__ ref_normals,
+                                            const FactorParams<T>& p, const FactorOutputs& o,
+                                            uint64_t cl) {
   constexpr int JL = TD * GD;
+  const uint64_t c = cl < p.num_cells ? cl : p.num_cells - 1;
+  const uint32_t* __restrict__ row = dofmap + c * K;
+  T x[K][GD];
+  gather_cell<T, GD, K, SHIFT>(nodes, row, p.last_node, x);
   const uint32_t tl = p.tile_log2;
   const uint64_t tile = cl >> tl;
   const uint32_t lane = static_cast<uint32_t>(cl) & ((1u << tl) - 1u);
   const uint32_t mask = p.mask;
   const bool need_j = (mask & ~kFPoints) != 0;
-  // Staged stores (tiles of 16 only): X, J and det J of the wavefront's 64 cells, four whole
-  // tiles, go to LDS in their global order, then leave as 16-byte non-temporal stores.
-  const bool staged = stage != nullptr;
-  const uint32_t w = threadIdx.x & 63u;
-  const uint32_t wt = w >> 4, wl = w & 15u;
-  const uint64_t tile0 = (cl_in - w) >> 4;
-  const uint64_t ntiles = p.padded_cells >> 4;
-  const uint32_t ex = 0;
-  const uint32_t ej = ex + ((mask & kFPoints) ? 64u * GD : 0u);
-  const uint32_t ed = ej + ((mask & kFJac) ? 64u * JL : 0u);
-  const uint32_t own = !live ? 0u : staged ? (mask & ~(kFPoints | kFJac | kFDetJ)) : mask;
-
   for (uint32_t q = 0; q < p.nq; ++q) {
-    const T* __restrict__ t = tab + static_cast<uint64_t>(q) * K * 4;
     const uint64_t row_q = tile * p.nq + q;  // (tile * nq + q): times LEN and the component next
+    T xv[GD], j[JL];
+    contract<T, TD, GD, K, SHIFT>(x, tab + static_cast<uint64_t>(q) * K * 4,
+                                  (mask & kFPoints) != 0, need_j, xv, j);
     if (mask & kFPoints) {
 #pragma unroll
-      for (int g = 0; g < GD; ++g) {
-        T acc = x[K0][g] * t[4 * K0];
-#pragma unroll
-        for (int k = K0 + 1; k < K; ++k) acc = tfma(x[k][g], t[4 * k], acc);
-        const T v = SHIFT ? (x[0][g] + acc) + p.origin[g] : acc + p.origin[g];
-        if (staged)
-          stage[ex + (wt * GD + g) * 16 + wl] = v;
-        else if (live)
-          st<T>(o.x, (((row_q * GD) + g) << tl) | lane, v);
-      }
+      for (int g = 0; g < GD; ++g) st<T>(o.x, (((row_q * GD) + g) << tl) | lane, xv[g] + p.origin[g]);
     }
     if (need_j) {
-      T j[JL];
-#pragma unroll
-      for (int d = 0; d < TD; ++d)
-#pragma unroll
-        for (int g = 0; g < GD; ++g) {
-          T acc = x[K0][g] * t[4 * K0 + 1 + d];
-#pragma unroll
-          for (int k = K0 + 1; k < K; ++k) acc = tfma(x[k][g], t[4 * k + 1 + d], acc);
-          j[d * GD + g] = acc;
-        }
-      T detj = T(0);
-      if (mask & kFDetJ) {
-        detj = det_of<T, TD, GD>(j);
-        if constexpr (TD == GD && sizeof(T) == 4) {
-          if (p.fixup) {
-            T n2 = T(0);
-#pragma unroll
-            for (int i = 0; i < JL; ++i) n2 = tfma(j[i], j[i], n2);
-            if (__builtin_fabsf(detj) < static_cast<T>(p.tau) * n2 * tsqrt(n2))
-              detj = static_cast<T>(
-                  detj_fp64<T, TD, GD>(nodes, row, K, p.last_node, tab64, q));
-          }
-        }
-      }
-      if (staged) {
-        if (mask & kFJac) {
-#pragma unroll
-          for (int i = 0; i < JL; ++i) stage[ej + (wt * JL + i) * 16 + wl] = j[i];
-        }
-        if (mask & kFDetJ) stage[ed + wt * 16 + wl] = detj;
-      }
-      if (own) store_from_j<T, TD, GD>(j, detj, ref_normals, flags, c, q, own, row_q, tl, lane, o);
-    }
-    if (staged) {
-      wave_sync();
-      if (mask & kFPoints) copy_staged<T, GD>(stage + ex, o.x, tile0, ntiles, p.nq, q, w);
-      if (mask & kFJac) copy_staged<T, JL>(stage + ej, o.j, tile0, ntiles, p.nq, q, w);
-      if (mask & kFDetJ) copy_staged<T, 1>(stage + ed, o.detj, tile0, ntiles, p.nq, q, w);
-      wave_sync();
+      const T detj = (mask & kFDetJ) ? detj_checked<T, TD, GD, K>(j, p, nodes, row, tab64, q) : T(0);
+      store_from_j<T, TD, GD>(j, detj, ref_normals, flags, c, q, mask, row_q, tl, lane, o);
     }
   }
 }
 
-// Elements of the staging buffer per wavefront (X, J, det J of 64 cells in 3D).
+// Copies rows q0..q0+nv-1 of one staged field of a tile: they are contiguous (nv * LEN * 16
+// values) in the staging buffer and in the QuadField, and leave as 16-byte stores.
+template <typename T, int LEN>
+__device__ __forceinline__ void copy_rows(const T* __restrict__ stage, void* out, uint64_t tile,
+                                          uint32_t nq, uint32_t q0, uint32_t nv, uint32_t w) {
+  using V = typename Vec16<T>::type;
+  constexpr uint32_t E = 16 / sizeof(T);
+  const uint32_t n = nv * (LEN * 16 / E);
+  T* dst = static_cast<T*>(out) + ((tile * nq + q0) * LEN) * 16;
+  for (uint32_t s = w; s < n; s += 64) {
+    const V v = *reinterpret_cast<const V*>(stage + s * E);
+    __builtin_nontemporal_store(v, reinterpret_cast<V*>(dst + s * E));
+  }
+}
+
+// All requested factors of the 16 cells of one tile (tiles of 16 cells), by one wavefront: the
+// lanes are 16 cells x 4 point groups, and each step covers points q0..q0+3, whose rows of the
+// tile are contiguous in every field. X, J and det J go through LDS and leave as sequential
+// 16-byte non-temporal stores, so each wavefront streams through its tile's output in order;
+// other factors are stored from registers.
+template <typename T, int TD, int GD, int K, bool SHIFT>
+__device__ __forceinline__ void direct_tile(const T* __restrict__ nodes,
+                                            const uint32_t* __restrict__ dofmap,
+                                            const uint32_t* __restrict__ flags,
+                                            const T* __restrict__ tab,
+                                            const double* __restrict__ tab64,
+                                            const T* __restrict__ ref_normals,
+                                            const FactorParams<T>& p, const FactorOutputs& o,
+                                            uint64_t tile, T* __restrict__ stage) {
+  constexpr int JL = TD * GD;
+  const uint32_t w = threadIdx.x & 63u, cl16 = w & 15u, qg = w >> 4;
+  const uint64_t cl = tile * 16 + cl16;
+  const uint64_t c = cl < p.num_cells ? cl : p.num_cells - 1;
+  const uint32_t* __restrict__ row = dofmap + c * K;
+  T x[K][GD];
+  gather_cell<T, GD, K, SHIFT>(nodes, row, p.last_node, x);
+  const uint32_t mask = p.mask;
+  const bool need_j = (mask & ~kFPoints) != 0;
+  const uint32_t ex = 0;
+  const uint32_t ej = ex + ((mask & kFPoints) ? 64u * GD : 0u);
+  const uint32_t ed = ej + ((mask & kFJac) ? 64u * JL : 0u);
+  const uint32_t own = mask & ~(kFPoints | kFJac | kFDetJ);
+  for (uint32_t q0 = 0; q0 < p.nq; q0 += 4) {
+    const uint32_t nv = p.nq - q0 < 4u ? p.nq - q0 : 4u;
+    const bool qlive = qg < nv;
+    const uint32_t q = qlive ? q0 + qg : p.nq - 1;
+    T xv[GD], j[JL];
+    contract<T, TD, GD, K, SHIFT>(x, tab + static_cast<uint64_t>(q) * K * 4,
+                                  (mask & kFPoints) != 0, need_j, xv, j);
+    if (mask & kFPoints) {
+#pragma unroll
+      for (int g = 0; g < GD; ++g) stage[ex + (qg * GD + g) * 16 + cl16] = xv[g] + p.origin[g];
+    }
+    if (need_j) {
+      const T detj = (mask & kFDetJ) ? detj_checked<T, TD, GD, K>(j, p, nodes, row, tab64, q) : T(0);
+      if (mask & kFJac) {
+#pragma unroll
+        for (int i = 0; i < JL; ++i) stage[ej + (qg * JL + i) * 16 + cl16] = j[i];
+      }
+      if (mask & kFDetJ) stage[ed + qg * 16 + cl16] = detj;
+      if (own && qlive)
+        store_from_j<T, TD, GD>(j, detj, ref_normals, flags, c, q, own, tile * p.nq + q, 4u, cl16, o);
+    }
+    wave_sync();
+    if (mask & kFPoints) copy_rows<T, GD>(stage + ex, o.x, tile, p.nq, q0, nv, w);
+    if (mask & kFJac) copy_rows<T, JL>(stage + ej, o.j, tile, p.nq, q0, nv, w);
+    if (mask & kFDetJ) copy_rows<T, 1>(stage + ed, o.detj, tile, p.nq, q0, nv, w);
+    wave_sync();
+  }
+}
+
+// Elements of the staging buffer per wavefront (X, J, det J of 64 (cell, point) pairs in 3D).
 constexpr int kStageElems = 64 * 13;
 
-// One lane per cell over one block.
+// One block: tiles of 16 cells one wavefront per tile (kBlock / 64 tiles per workgroup);
+// other tiles one lane per cell.
 template <typename T, int TD, int GD, int K, bool SHIFT>
 __global__ void __launch_bounds__(kBlock)
     factor_direct(const T* __restrict__ nodes, const uint32_t* __restrict__ dofmap,
@@ -359,17 +405,15 @@ __global__ void __launch_bounds__(kBlock)
                   FactorParams<T> p, FactorOutputs o) {
   __shared__ __align__(16) T lds[kBlock / 64][kStageElems];
   const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
-  const uint64_t cl = static_cast<uint64_t>(b) * kBlock + threadIdx.x;
-  const bool staged = p.tile_log2 == 4;
-  if (staged) {
-    // Whole wavefronts stay (a wavefront entirely past the padding leaves).
-    if (cl - (threadIdx.x & 63u) >= p.padded_cells) return;
-    direct_cell<T, TD, GD, K, SHIFT>(nodes, dofmap, flags, tab, tab64, ref_normals, p, o, cl,
+  if (p.tile_log2 == 4) {
+    const uint64_t tile = static_cast<uint64_t>(b) * (kBlock / 64) + (threadIdx.x >> 6);
+    if (tile >= (p.padded_cells >> 4)) return;
+    direct_tile<T, TD, GD, K, SHIFT>(nodes, dofmap, flags, tab, tab64, ref_normals, p, o, tile,
                                      lds[threadIdx.x >> 6]);
   } else {
+    const uint64_t cl = static_cast<uint64_t>(b) * kBlock + threadIdx.x;
     if (cl >= p.padded_cells) return;
-    direct_cell<T, TD, GD, K, SHIFT>(nodes, dofmap, flags, tab, tab64, ref_normals, p, o, cl,
-                                     nullptr);
+    direct_cell<T, TD, GD, K, SHIFT>(nodes, dofmap, flags, tab, tab64, ref_normals, p, o, cl);
   }
 }
 
@@ -383,8 +427,9 @@ __global__ void __launch_bounds__(kBlock)
 
 // One persistent launch over the work items of every block of a mesh. Items are split into
 // eight contiguous ranges, one per XCD (workgroup b runs on XCD b % 8); the workgroups of an
-// XCD take items of their range in order from one counter. The last workgroup to finish resets
-// the counters, so the next launch (or graph replay) starts from zero.
+// XCD take items of their range in order from one counter. An item is kBlock / 64 tiles of 16
+// cells (tiles of 16) or kBlock cells (other tiles). The last workgroup to finish resets the
+// counters, so the next launch (or graph replay) starts from zero.
 template <typename T, bool SHIFT>
 __global__ void __launch_bounds__(kBlock)
     factor_hybrid(const HybridBlock<T>* __restrict__ blocks, const WorkItem* __restrict__ items,
@@ -401,21 +446,36 @@ __global__ void __launch_bounds__(kBlock)
     if (it >= end) break;
     const WorkItem w = items[it];
     const HybridBlock<T>* __restrict__ hb = blocks + w.block;
-    const uint64_t cl = static_cast<uint64_t>(w.first_cell) + threadIdx.x;
-    const bool staged = hb->p.tile_log2 == 4;
-    // Staged: whole wavefronts call (one entirely past the padding skips); otherwise lanes.
-    if ((staged ? cl - (threadIdx.x & 63u) : cl) < hb->p.padded_cells) {
-      T* stage = staged ? lds[threadIdx.x >> 6] : nullptr;
-      switch (hb->shape) {
-#define TG_HCASE(S_, TD_, GD_, K_)                                                            \
-  case S_:                                                                                     \
-    direct_cell<T, TD_, GD_, K_, SHIFT>(hb->nodes, hb->dofmap, hb->flags, hb->tab, hb->tab64, \
-                                        hb->ref_normals, hb->p, hb->out, cl, stage);           \
+    if (hb->p.tile_log2 == 4) {
+      const uint64_t tile = (static_cast<uint64_t>(w.first_cell) >> 4) + (threadIdx.x >> 6);
+      if (tile < (hb->p.padded_cells >> 4)) {
+        T* stage = lds[threadIdx.x >> 6];
+        switch (hb->shape) {
+#define TG_HCASE(S_, TD_, GD_, K_)                                                           \
+  case S_:                                                                                    \
+    direct_tile<T, TD_, GD_, K_, SHIFT>(hb->nodes, hb->dofmap, hb->flags, hb->tab, hb->tab64, \
+                                        hb->ref_normals, hb->p, hb->out, tile, stage);        \
     break;
-        TG_HYBRID_SHAPES(TG_HCASE)
+          TG_HYBRID_SHAPES(TG_HCASE)
 #undef TG_HCASE
-        default:
-          break;
+          default:
+            break;
+        }
+      }
+    } else {
+      const uint64_t cl = static_cast<uint64_t>(w.first_cell) + threadIdx.x;
+      if (cl < hb->p.padded_cells) {
+        switch (hb->shape) {
+#define TG_HCASE(S_, TD_, GD_, K_)                                                           \
+  case S_:                                                                                    \
+    direct_cell<T, TD_, GD_, K_, SHIFT>(hb->nodes, hb->dofmap, hb->flags, hb->tab, hb->tab64, \
+                                        hb->ref_normals, hb->p, hb->out, cl);                 \
+    break;
+          TG_HYBRID_SHAPES(TG_HCASE)
+#undef TG_HCASE
+          default:
+            break;
+        }
       }
     }
   }
@@ -574,9 +634,12 @@ hipError_t launch_factor(int tdim, int gdim, bool shift, bool generic, const T* 
                          const double* tab64, const T* ref_normals, const FactorParams<T>& p,
                          const FactorOutputs& out, hipStream_t stream) {
   FactorFn<T> fn = generic ? nullptr : direct_kernel<T>(tdim, gdim, p.k, shift);
+  const bool direct = fn != nullptr;
   if (fn == nullptr) fn = generic_kernel<T>(tdim, gdim, shift);
   if (fn == nullptr) return hipErrorNotSupported;
-  const uint64_t blocks = (p.padded_cells + kBlock - 1) / kBlock;
+  // Register-resident kernels on tiles of 16: one wavefront per tile; else one lane per cell.
+  const uint64_t per_block = (direct && p.tile_log2 == 4) ? kBlock / 4 : kBlock;
+  const uint64_t blocks = (p.padded_cells + per_block - 1) / per_block;
   if (blocks > 0x7fffffffu) return hipErrorInvalidConfiguration;
   hipLaunchKernelGGL(fn, dim3(static_cast<uint32_t>(blocks)), dim3(kBlock), 0, stream, nodes,
                      dofmap, flags, tab, tab64, ref_normals, p, out);

@@ -12,7 +12,9 @@
 
 namespace {
 
-constexpr uint32_t kItemCells = 256;  // lanes per work item = the kernel's workgroup size
+// Cells per work item: a workgroup of 256 lanes covers 4 tiles of 16 cells (one wavefront per
+// tile) with tiles of 16, else 256 cells (one lane per cell).
+uint32_t item_cells(uint32_t tile) { return tile == 16 ? 64u : 256u; }
 
 struct PlanBlock {
   int index;
@@ -104,7 +106,7 @@ int upload_persistent(tg_plan* plan) {
   std::vector<tg::WorkItem> items;
   std::vector<double> item_cost;
   for (size_t i = 0; i < hb.size(); ++i)
-    for (uint64_t c = 0; c < hb[i].p.padded_cells; c += kItemCells) {
+    for (uint64_t c = 0; c < hb[i].p.padded_cells; c += item_cells(plan->tile)) {
       items.push_back(tg::WorkItem{static_cast<uint32_t>(i), static_cast<uint32_t>(c)});
       item_cost.push_back(cost[i]);
     }
