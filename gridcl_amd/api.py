@@ -51,6 +51,9 @@ FACTORS = {"points": 1, "jacobians": 2, "detj": 4, "pinv": 8, "facet_normals": 1
 OPT_NO_CELL_SHIFT = 1
 OPT_NO_FP64_FIXUP = 2
 OPT_FORCE_GENERIC = 4
+OPT_MAP_CELL = 8
+OPT_MAP_CELL_STAGED = 16
+OPT_MAP_TILE = 32
 
 
 # ---- Errors: one class per tetraloma_device::GridError variant of the path -------------------

@@ -16,7 +16,7 @@ OBJ_DIR = os.path.join(HERE, "_build")
 ARCH = "gfx950"
 
 HIP_SOURCES = ["factor_kernels.hip", "adjoint_kernels.hip"]
-HOST_HIP_SOURCES = ["capi.cpp", "adjoint.cpp", "plan.cpp"]  # host code that calls the HIP runtime
+HOST_HIP_SOURCES = ["capi.cpp", "adjoint.cpp", "plan.cpp", "tuning.cpp"]  # host code that calls the HIP runtime
 HOST_SOURCES = ["refcell.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

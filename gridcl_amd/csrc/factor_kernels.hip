@@ -239,29 +239,62 @@ __device__ __forceinline__ void gather_cell(const T* __restrict__ nodes,
 
 // X (without the origin) and J of the gathered cell at the point whose table row is t; FP32
 // adds node 0 back to X after the contraction of the offsets.
+// The (value, d/dxi_0, d/dxi_1, d/dxi_2) row of node k at one point: one 16-byte load in FP32,
+// two in FP64.
+template <typename T>
+struct TabRow {
+  T v[4];
+};
+template <typename T>
+__device__ __forceinline__ TabRow<T> load_row(const T* __restrict__ t) {
+  using V = typename Vec16<T>::type;
+  TabRow<T> r;
+  if constexpr (sizeof(T) == 4) {
+    const V a = *reinterpret_cast<const V*>(t);
+    r.v[0] = a.x;
+    r.v[1] = a.y;
+    r.v[2] = a.z;
+    r.v[3] = a.w;
+  } else {
+    const V a = *reinterpret_cast<const V*>(t);
+    const V b = *reinterpret_cast<const V*>(t + 2);
+    r.v[0] = a.x;
+    r.v[1] = a.y;
+    r.v[2] = b.x;
+    r.v[3] = b.y;
+  }
+  return r;
+}
+
+// X (without the origin) and J of the gathered cell at the point whose table rows start at t
+// (16-byte aligned); FP32 adds node 0 back to X after the contraction of the offsets. Each sum
+// starts from its first product and runs in ascending node order.
 template <typename T, int TD, int GD, int K, bool SHIFT>
 __device__ __forceinline__ void contract(const T (&x)[K][GD], const T* __restrict__ t,
                                          bool want_x, bool want_j, T* xv, T* j) {
   constexpr int K0 = SHIFT ? 1 : 0;  // first node whose offset is not identically zero
-  if (want_x) {
+  T ax[GD], aj[TD * GD];
+#pragma unroll
+  for (int k = K0; k < K; ++k) {
+    const TabRow<T> r = load_row<T>(t + 4 * k);
 #pragma unroll
     for (int g = 0; g < GD; ++g) {
-      T acc = x[K0][g] * t[4 * K0];
+      if (want_x) ax[g] = k == K0 ? x[k][g] * r.v[0] : tfma(x[k][g], r.v[0], ax[g]);
+      if (want_j) {
 #pragma unroll
-      for (int k = K0 + 1; k < K; ++k) acc = tfma(x[k][g], t[4 * k], acc);
-      xv[g] = SHIFT ? x[0][g] + acc : acc;
+        for (int d = 0; d < TD; ++d)
+          aj[d * GD + g] =
+              k == K0 ? x[k][g] * r.v[1 + d] : tfma(x[k][g], r.v[1 + d], aj[d * GD + g]);
+      }
     }
+  }
+  if (want_x) {
+#pragma unroll
+    for (int g = 0; g < GD; ++g) xv[g] = SHIFT ? x[0][g] + ax[g] : ax[g];
   }
   if (want_j) {
 #pragma unroll
-    for (int d = 0; d < TD; ++d)
-#pragma unroll
-      for (int g = 0; g < GD; ++g) {
-        T acc = x[K0][g] * t[4 * K0 + 1 + d];
-#pragma unroll
-        for (int k = K0 + 1; k < K; ++k) acc = tfma(x[k][g], t[4 * k + 1 + d], acc);
-        j[d * GD + g] = acc;
-      }
+    for (int i = 0; i < TD * GD; ++i) j[i] = aj[i];
   }
 }
 
@@ -318,6 +351,80 @@ __device__ __forceinline__ void direct_cell(const T* __restrict__ nodes,
       const T detj = (mask & kFDetJ) ? detj_checked<T, TD, GD, K>(j, p, nodes, row, tab64, q) : T(0);
       store_from_j<T, TD, GD>(j, detj, ref_normals, flags, c, q, mask, row_q, tl, lane, o);
     }
+  }
+}
+
+// Copies one staged field of a wavefront's four tiles at point q: for each tile the LEN * 16
+// values of (tile, q) are contiguous in the staging buffer and in the QuadField. Tiles at or past
+// `ntiles` are padding of the launch and skipped.
+template <typename T, int LEN>
+__device__ __forceinline__ void copy_point(const T* __restrict__ stage, void* out, uint64_t tile0,
+                                           uint64_t ntiles, uint32_t nq, uint32_t q, uint32_t w) {
+  using V = typename Vec16<T>::type;
+  constexpr uint32_t E = 16 / sizeof(T);
+  constexpr uint32_t per_tile = LEN * 16 / E;
+  for (uint32_t s = w; s < 4 * per_tile; s += 64) {
+    const uint32_t t = s / per_tile, jj = s - t * per_tile;
+    const uint64_t ta = tile0 + t;
+    if (ta < ntiles) {
+      const V v = *reinterpret_cast<const V*>(stage + t * LEN * 16 + jj * E);
+      T* dst = static_cast<T*>(out) + ((ta * nq + q) * LEN) * 16 + jj * E;
+      __builtin_nontemporal_store(v, reinterpret_cast<V*>(dst));
+    }
+  }
+}
+
+// Lane per cell over tiles of 16, whole wavefronts (four tiles): X, J and det J of each point
+// go through LDS and leave as 16-byte stores, one point at a time. Lanes at or past the padded
+// cell count compute the last padded lane's cell and write nothing of their own.
+template <typename T, int TD, int GD, int K, bool SHIFT>
+__device__ __forceinline__ void direct_cell_staged(const T* __restrict__ nodes,
+                                                   const uint32_t* __restrict__ dofmap,
+                                                   const uint32_t* __restrict__ flags,
+                                                   const T* __restrict__ tab,
+                                                   const double* __restrict__ tab64,
+                                                   const T* __restrict__ ref_normals,
+                                                   const FactorParams<T>& p,
+                                                   const FactorOutputs& o, uint64_t cl_in,
+                                                   T* __restrict__ stage) {
+  constexpr int JL = TD * GD;
+  const bool live = cl_in < p.padded_cells;
+  const uint64_t cl = live ? cl_in : p.padded_cells - 1;
+  const uint64_t c = cl < p.num_cells ? cl : p.num_cells - 1;
+  const uint32_t* __restrict__ row = dofmap + c * K;
+  T x[K][GD];
+  gather_cell<T, GD, K, SHIFT>(nodes, row, p.last_node, x);
+  const uint32_t w = threadIdx.x & 63u, wt = w >> 4, wl = w & 15u;
+  const uint64_t tile0 = (cl_in - w) >> 4, tile = cl >> 4;
+  const uint64_t ntiles = p.padded_cells >> 4;
+  const uint32_t mask = p.mask;
+  const bool need_j = (mask & ~kFPoints) != 0;
+  const uint32_t ex = 0;
+  const uint32_t ej = ex + ((mask & kFPoints) ? 64u * GD : 0u);
+  const uint32_t ed = ej + ((mask & kFJac) ? 64u * JL : 0u);
+  const uint32_t own = live ? mask & ~(kFPoints | kFJac | kFDetJ) : 0u;
+  for (uint32_t q = 0; q < p.nq; ++q) {
+    T xv[GD], j[JL];
+    contract<T, TD, GD, K, SHIFT>(x, tab + static_cast<uint64_t>(q) * K * 4,
+                                  (mask & kFPoints) != 0, need_j, xv, j);
+    if (mask & kFPoints) {
+#pragma unroll
+      for (int g = 0; g < GD; ++g) stage[ex + (wt * GD + g) * 16 + wl] = xv[g] + p.origin[g];
+    }
+    if (need_j) {
+      const T detj = (mask & kFDetJ) ? detj_checked<T, TD, GD, K>(j, p, nodes, row, tab64, q) : T(0);
+      if (mask & kFJac) {
+#pragma unroll
+        for (int i = 0; i < JL; ++i) stage[ej + (wt * JL + i) * 16 + wl] = j[i];
+      }
+      if (mask & kFDetJ) stage[ed + wt * 16 + wl] = detj;
+      if (own) store_from_j<T, TD, GD>(j, detj, ref_normals, flags, c, q, own, tile * p.nq + q, 4u, wl, o);
+    }
+    wave_sync();
+    if (mask & kFPoints) copy_point<T, GD>(stage + ex, o.x, tile0, ntiles, p.nq, q, w);
+    if (mask & kFJac) copy_point<T, JL>(stage + ej, o.j, tile0, ntiles, p.nq, q, w);
+    if (mask & kFDetJ) copy_point<T, 1>(stage + ed, o.detj, tile0, ntiles, p.nq, q, w);
+    wave_sync();
   }
 }
 
@@ -405,11 +512,16 @@ __global__ void __launch_bounds__(kBlock)
                   FactorParams<T> p, FactorOutputs o) {
   __shared__ __align__(16) T lds[kBlock / 64][kStageElems];
   const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
-  if (p.tile_log2 == 4) {
+  if (p.mapping == kMapTile) {
     const uint64_t tile = static_cast<uint64_t>(b) * (kBlock / 64) + (threadIdx.x >> 6);
     if (tile >= (p.padded_cells >> 4)) return;
     direct_tile<T, TD, GD, K, SHIFT>(nodes, dofmap, flags, tab, tab64, ref_normals, p, o, tile,
                                      lds[threadIdx.x >> 6]);
+  } else if (p.mapping == kMapCellStaged) {
+    const uint64_t cl = static_cast<uint64_t>(b) * kBlock + threadIdx.x;
+    if (cl - (threadIdx.x & 63u) >= p.padded_cells) return;  // whole wavefronts stay
+    direct_cell_staged<T, TD, GD, K, SHIFT>(nodes, dofmap, flags, tab, tab64, ref_normals, p, o,
+                                            cl, lds[threadIdx.x >> 6]);
   } else {
     const uint64_t cl = static_cast<uint64_t>(b) * kBlock + threadIdx.x;
     if (cl >= p.padded_cells) return;
@@ -446,7 +558,24 @@ __global__ void __launch_bounds__(kBlock)
     if (it >= end) break;
     const WorkItem w = items[it];
     const HybridBlock<T>* __restrict__ hb = blocks + w.block;
-    if (hb->p.tile_log2 == 4) {
+    if (hb->p.mapping == kMapCellStaged) {
+      const uint64_t cl = static_cast<uint64_t>(w.first_cell) + threadIdx.x;
+      if (cl - (threadIdx.x & 63u) < hb->p.padded_cells) {
+        T* stage = lds[threadIdx.x >> 6];
+        switch (hb->shape) {
+#define TG_HCASE(S_, TD_, GD_, K_)                                                        \
+  case S_:                                                                                 \
+    direct_cell_staged<T, TD_, GD_, K_, SHIFT>(hb->nodes, hb->dofmap, hb->flags, hb->tab,  \
+                                               hb->tab64, hb->ref_normals, hb->p, hb->out, \
+                                               cl, stage);                                 \
+    break;
+          TG_HYBRID_SHAPES(TG_HCASE)
+#undef TG_HCASE
+          default:
+            break;
+        }
+      }
+    } else if (hb->p.mapping == kMapTile) {
       const uint64_t tile = (static_cast<uint64_t>(w.first_cell) >> 4) + (threadIdx.x >> 6);
       if (tile < (hb->p.padded_cells >> 4)) {
         T* stage = lds[threadIdx.x >> 6];
@@ -631,14 +760,16 @@ FactorFn<T> generic_kernel(int td, int gd, bool shift) {
 template <typename T>
 hipError_t launch_factor(int tdim, int gdim, bool shift, bool generic, const T* nodes,
                          const uint32_t* dofmap, const uint32_t* flags, const T* tab,
-                         const double* tab64, const T* ref_normals, const FactorParams<T>& p,
+                         const double* tab64, const T* ref_normals, const FactorParams<T>& pin,
                          const FactorOutputs& out, hipStream_t stream) {
-  FactorFn<T> fn = generic ? nullptr : direct_kernel<T>(tdim, gdim, p.k, shift);
+  FactorFn<T> fn = generic ? nullptr : direct_kernel<T>(tdim, gdim, pin.k, shift);
   const bool direct = fn != nullptr;
   if (fn == nullptr) fn = generic_kernel<T>(tdim, gdim, shift);
   if (fn == nullptr) return hipErrorNotSupported;
-  // Register-resident kernels on tiles of 16: one wavefront per tile; else one lane per cell.
-  const uint64_t per_block = (direct && p.tile_log2 == 4) ? kBlock / 4 : kBlock;
+  FactorParams<T> p = pin;
+  // The staged mappings need tiles of 16 and a register-resident kernel.
+  if (!direct || p.tile_log2 != 4) p.mapping = kMapCell;
+  const uint64_t per_block = cells_per_workgroup(p.mapping);
   const uint64_t blocks = (p.padded_cells + per_block - 1) / per_block;
   if (blocks > 0x7fffffffu) return hipErrorInvalidConfiguration;
   hipLaunchKernelGGL(fn, dim3(static_cast<uint32_t>(blocks)), dim3(kBlock), 0, stream, nodes,

@@ -17,6 +17,22 @@ enum : uint32_t {
   kFNormals = 1u << 5,
 };
 
+// How a register-resident factor kernel maps work to lanes and writes its outputs:
+//   kMapCell        one lane per cell, every value stored from registers (any tile);
+//   kMapCellStaged  one lane per cell; X, J, det J of each point staged in LDS and written as
+//                   16-byte stores for the wavefront's four tiles (tiles of 16);
+//   kMapTile        one wavefront per tile of 16 cells, lanes = 16 cells x 4 points; X, J,
+//                   det J staged and written as the tile's consecutive rows (tiles of 16).
+// The runtime-K kernel always runs kMapCell.
+enum : uint32_t { kMapCell = 0, kMapCellStaged = 1, kMapTile = 2 };
+
+// Cells one 256-lane workgroup covers under a mapping.
+inline uint32_t cells_per_workgroup(uint32_t mapping) { return mapping == kMapTile ? 64u : 256u; }
+
+// The mapping for (dtype, tdim, gdim, k, nq, tile) and the call's options (tuning.cpp).
+uint32_t choose_mapping(int dtype, int tdim, int gdim, uint32_t k, uint32_t nq, uint32_t tile,
+                        uint32_t options);
+
 // Scalars of one factor launch. Device arrays are passed as separate restrict-qualified kernel
 // parameters so that the table loads can stay in the scalar cache.
 template <typename T>
@@ -29,6 +45,7 @@ struct FactorParams {
   uint32_t tile_log2;     // log2 of the QuadField tile
   uint32_t mask;          // factor bits
   uint32_t fixup;         // 1: recompute det J in FP64 where |det J| < tau |J|_F^3
+  uint32_t mapping;       // kMapCell, kMapCellStaged or kMapTile (the last two: tiles of 16)
   double tau;
   T origin[3];            // origin narrowed to T, added back to X
 };

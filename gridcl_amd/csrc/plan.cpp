@@ -12,10 +12,6 @@
 
 namespace {
 
-// Cells per work item: a workgroup of 256 lanes covers 4 tiles of 16 cells (one wavefront per
-// tile) with tiles of 16, else 256 cells (one lane per cell).
-uint32_t item_cells(uint32_t tile) { return tile == 16 ? 64u : 256u; }
-
 struct PlanBlock {
   int index;
   tg_pointset* pts;
@@ -74,6 +70,7 @@ tg::FactorParams<T> params(const tg_geometry* g, const tg::Block& b, const tg_po
   p.fixup = (f32 && !(options & TG_OPT_NO_FP64_FIXUP) && !(options & TG_OPT_NO_CELL_SHIFT)) ? 1u
                                                                                             : 0u;
   p.tau = g->tau;
+  p.mapping = tg::choose_mapping(g->dtype, b.tdim, g->gdim, b.k, p.nq, tile, options);
   for (int i = 0; i < 3; ++i) p.origin[i] = static_cast<T>(i < g->gdim ? g->origin[i] : 0.0);
   return p;
 }
@@ -106,7 +103,7 @@ int upload_persistent(tg_plan* plan) {
   std::vector<tg::WorkItem> items;
   std::vector<double> item_cost;
   for (size_t i = 0; i < hb.size(); ++i)
-    for (uint64_t c = 0; c < hb[i].p.padded_cells; c += item_cells(plan->tile)) {
+    for (uint64_t c = 0; c < hb[i].p.padded_cells; c += tg::cells_per_workgroup(hb[i].p.mapping)) {
       items.push_back(tg::WorkItem{static_cast<uint32_t>(i), static_cast<uint32_t>(c)});
       item_cost.push_back(cost[i]);
     }

@@ -1,0 +1,52 @@
+// The launch mapping of the register-resident factor kernels per (precision, tdim, gdim, nodes,
+// points): the table written by the autotuner (gridcl_amd/tune.py), compiled in as the
+// reference compiles in its measured degree policy (tetraloma-geometry/degree-policy.toml,
+// strategy.rs:373-407), with the same two overrides: an option bit of the call, then the
+// environment variable TETRALOMA_GEOM_MAPPING (cell | cell_staged | tile).
+#include <cstdlib>
+#include <cstring>
+
+#include "../../include/tetraloma_geom.h"
+#include "geom_internal.hpp"
+
+namespace tg {
+namespace {
+
+struct TuneEntry {
+  int dtype, tdim, gdim;
+  uint32_t k, nq_lo, nq_hi;  // points per cell in [nq_lo, nq_hi]
+  uint32_t mapping;
+};
+
+#include "tuning_table.inc"
+
+uint32_t env_mapping() {
+  static const uint32_t v = [] {
+    const char* e = std::getenv("TETRALOMA_GEOM_MAPPING");
+    if (!e) return 0xffffffffu;
+    if (!std::strcmp(e, "cell")) return static_cast<uint32_t>(kMapCell);
+    if (!std::strcmp(e, "cell_staged")) return static_cast<uint32_t>(kMapCellStaged);
+    if (!std::strcmp(e, "tile")) return static_cast<uint32_t>(kMapTile);
+    return 0xffffffffu;
+  }();
+  return v;
+}
+
+}  // namespace
+
+uint32_t choose_mapping(int dtype, int tdim, int gdim, uint32_t k, uint32_t nq, uint32_t tile,
+                        uint32_t options) {
+  if (tile != 16) return kMapCell;
+  if (options & TG_OPT_MAP_CELL) return kMapCell;
+  if (options & TG_OPT_MAP_CELL_STAGED) return kMapCellStaged;
+  if (options & TG_OPT_MAP_TILE) return kMapTile;
+  const uint32_t env = env_mapping();
+  if (env != 0xffffffffu) return env;
+  for (const TuneEntry& e : kTuned)
+    if (e.dtype == dtype && e.tdim == tdim && e.gdim == gdim && e.k == k && nq >= e.nq_lo &&
+        nq <= e.nq_hi)
+      return e.mapping;
+  return kDefaultMapping;
+}
+
+}  // namespace tg

@@ -81,7 +81,15 @@ enum {
   /* F32 geometries: skip the FP64 recomputation of det J at near-degenerate points. */
   TG_OPT_NO_FP64_FIXUP = 1u << 1,
   /* Force the generic runtime-K kernel (the reference's `loops` strategy) for tests. */
-  TG_OPT_FORCE_GENERIC = 1u << 2
+  TG_OPT_FORCE_GENERIC = 1u << 2,
+  /* Force a lane mapping of the register-resident kernels, overriding the tuned table and the
+   * TETRALOMA_GEOM_MAPPING environment variable (the reference's forced strategies,
+   * tetraloma-geometry/src/strategy.rs:311-353): one lane per cell storing from registers; one
+   * lane per cell with 16-byte stores staged in LDS; one wavefront per 16-cell tile streaming the
+   * tile's rows. The last two need tiles of 16 (other tiles always run the first). */
+  TG_OPT_MAP_CELL = 1u << 3,
+  TG_OPT_MAP_CELL_STAGED = 1u << 4,
+  TG_OPT_MAP_TILE = 1u << 5
 };
 
 typedef struct tg_geometry tg_geometry;

@@ -146,6 +146,32 @@ def test_generic_kernel_matches_direct(cuda, cell, p, dtype):
         assert np.abs(a[n] - b[n]).max() <= tol * max(1.0, np.abs(a[n]).max()), n
 
 
+@pytest.mark.parametrize("kind,cell,p", [("vol", TET, 2), ("vol", HEX, 2), ("vol", PRISM, 1),
+                                         ("surf3", TRI, 3), ("vol", PYR, 1), ("vol", TET, 3)])
+@pytest.mark.parametrize("dtype", ["f32", "f64"])
+def test_mappings_agree(cuda, kind, cell, p, dtype):
+    """The three lane mappings run the same arithmetic: bitwise equal fields, padding included
+    (45 cells: the last tile is partial)."""
+    import torch
+
+    mesh = make_case(kind, cell, p)
+    mesh.blocks[0].cells = mesh.blocks[0].cells[:45]
+    if mesh.blocks[0].flags is not None:
+        mesh.blocks[0].flags = mesh.blocks[0].flags[:45]
+    tdt = torch.float64 if dtype == "f64" else torch.float32
+    g = api.geometry_from_host(mesh, dtype=tdt)
+    ps = api.PointSet.gauss(cell, 2 * p + 1)
+    names = factor_names(M.TDIM[cell], mesh.gdim)
+    res = []
+    for opt in (api.OPT_MAP_CELL, api.OPT_MAP_CELL_STAGED, api.OPT_MAP_TILE):
+        f = api.geometric_factors_block(g, 0, ps, names, options=opt)
+        torch.cuda.synchronize()
+        res.append({n: v.data.cpu().numpy() for n, v in f.items()})
+    for n in names:
+        assert np.array_equal(res[0][n], res[1][n]), n
+        assert np.array_equal(res[0][n], res[2][n]), n
+
+
 @pytest.mark.parametrize("tile", [1, 2, 4, 8, 16])
 def test_tiles_and_padding(cuda, tile):
     """Every tile width gives the same values; padding lanes hold the last cell's values
