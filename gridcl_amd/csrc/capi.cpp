@@ -185,6 +185,7 @@ int eval_typed(const tg_geometry* g, const Block& b, tg_pointset* pts, DeviceTab
                                                                                             : 0u;
   p.tau = g->tau;
   p.mapping = tg::choose_mapping(g->dtype, b.tdim, g->gdim, b.k, p.nq, tile, options);
+  p.xcd_remap = tg::choose_xcd_remap();
   for (int i = 0; i < 3; ++i) p.origin[i] = static_cast<T>(i < g->gdim ? g->origin[i] : 0.0);
   tg::FactorOutputs o{out->points, out->jacobians, out->detj, out->pinv, out->facet_normals,
                       out->normals};

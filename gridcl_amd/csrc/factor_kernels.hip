@@ -502,26 +502,42 @@ __device__ __forceinline__ void direct_tile(const T* __restrict__ nodes,
 // Elements of the staging buffer per wavefront (X, J, det J of 64 (cell, point) pairs in 3D).
 constexpr int kStageElems = 64 * 13;
 
-// One block: tiles of 16 cells one wavefront per tile (kBlock / 64 tiles per workgroup);
-// other tiles one lane per cell.
+// Copies n table values (a multiple of 4) from global memory into LDS with 16-byte accesses;
+// every thread of the workgroup calls.
+template <typename T>
+__device__ __forceinline__ void table_to_lds(const T* __restrict__ src, T* __restrict__ dst,
+                                             uint32_t n) {
+  using V = typename Vec16<T>::type;
+  constexpr uint32_t E = 16 / sizeof(T);
+  for (uint32_t i = threadIdx.x; i < n / E; i += kBlock)
+    reinterpret_cast<V*>(dst)[i] = reinterpret_cast<const V*>(src)[i];
+}
+
+// One block. kMapTile: one wavefront per tile of 16 cells, the basis table staged in LDS;
+// kMapCellStaged / kMapCell: one lane per cell, the table read through the scalar cache.
+// Dynamic LDS: the staging buffers (staged mappings), then the table (kMapTile).
 template <typename T, int TD, int GD, int K, bool SHIFT>
 __global__ void __launch_bounds__(kBlock)
     factor_direct(const T* __restrict__ nodes, const uint32_t* __restrict__ dofmap,
                   const uint32_t* __restrict__ flags, const T* __restrict__ tab,
                   const double* __restrict__ tab64, const T* __restrict__ ref_normals,
                   FactorParams<T> p, FactorOutputs o) {
-  __shared__ __align__(16) T lds[kBlock / 64][kStageElems];
-  const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
+  extern __shared__ __align__(16) unsigned char tg_lds[];
+  T* stage = reinterpret_cast<T*>(tg_lds);
+  const uint32_t b = p.xcd_remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
   if (p.mapping == kMapTile) {
+    T* ltab = stage + (kBlock / 64) * kStageElems;
+    table_to_lds<T>(tab, ltab, p.nq * K * 4);
+    __syncthreads();
     const uint64_t tile = static_cast<uint64_t>(b) * (kBlock / 64) + (threadIdx.x >> 6);
     if (tile >= (p.padded_cells >> 4)) return;
-    direct_tile<T, TD, GD, K, SHIFT>(nodes, dofmap, flags, tab, tab64, ref_normals, p, o, tile,
-                                     lds[threadIdx.x >> 6]);
+    direct_tile<T, TD, GD, K, SHIFT>(nodes, dofmap, flags, ltab, tab64, ref_normals, p, o, tile,
+                                     stage + (threadIdx.x >> 6) * kStageElems);
   } else if (p.mapping == kMapCellStaged) {
     const uint64_t cl = static_cast<uint64_t>(b) * kBlock + threadIdx.x;
     if (cl - (threadIdx.x & 63u) >= p.padded_cells) return;  // whole wavefronts stay
     direct_cell_staged<T, TD, GD, K, SHIFT>(nodes, dofmap, flags, tab, tab64, ref_normals, p, o,
-                                            cl, lds[threadIdx.x >> 6]);
+                                            cl, stage + (threadIdx.x >> 6) * kStageElems);
   } else {
     const uint64_t cl = static_cast<uint64_t>(b) * kBlock + threadIdx.x;
     if (cl >= p.padded_cells) return;
@@ -547,7 +563,10 @@ __global__ void __launch_bounds__(kBlock)
     factor_hybrid(const HybridBlock<T>* __restrict__ blocks, const WorkItem* __restrict__ items,
                   const uint32_t* __restrict__ xcd_begin, uint32_t* counters) {
   __shared__ uint32_t s_item;
-  __shared__ __align__(16) T lds[kBlock / 64][kStageElems];
+  extern __shared__ __align__(16) unsigned char tg_lds[];
+  T* stages = reinterpret_cast<T*>(tg_lds);
+  T* ltab = stages + (kBlock / 64) * kStageElems;  // the table of the kMapTile block in hand
+  uint32_t table_block = 0xffffffffu;
   const uint32_t x = blockIdx.x & 7u;
   const uint32_t begin = xcd_begin[x], end = xcd_begin[x + 1];
   for (;;) {
@@ -558,10 +577,17 @@ __global__ void __launch_bounds__(kBlock)
     if (it >= end) break;
     const WorkItem w = items[it];
     const HybridBlock<T>* __restrict__ hb = blocks + w.block;
+    if (hb->p.mapping == kMapTile && w.block != table_block) {
+      // A new block's table: the workgroup is in step (items are uniform), so it reloads it.
+      __syncthreads();
+      table_to_lds<T>(hb->tab, ltab, hb->p.nq * hb->p.k * 4);
+      __syncthreads();
+      table_block = w.block;
+    }
     if (hb->p.mapping == kMapCellStaged) {
       const uint64_t cl = static_cast<uint64_t>(w.first_cell) + threadIdx.x;
       if (cl - (threadIdx.x & 63u) < hb->p.padded_cells) {
-        T* stage = lds[threadIdx.x >> 6];
+        T* stage = stages + (threadIdx.x >> 6) * kStageElems;
         switch (hb->shape) {
 #define TG_HCASE(S_, TD_, GD_, K_)                                                        \
   case S_:                                                                                 \
@@ -578,12 +604,12 @@ __global__ void __launch_bounds__(kBlock)
     } else if (hb->p.mapping == kMapTile) {
       const uint64_t tile = (static_cast<uint64_t>(w.first_cell) >> 4) + (threadIdx.x >> 6);
       if (tile < (hb->p.padded_cells >> 4)) {
-        T* stage = lds[threadIdx.x >> 6];
+        T* stage = stages + (threadIdx.x >> 6) * kStageElems;
         switch (hb->shape) {
-#define TG_HCASE(S_, TD_, GD_, K_)                                                           \
-  case S_:                                                                                    \
-    direct_tile<T, TD_, GD_, K_, SHIFT>(hb->nodes, hb->dofmap, hb->flags, hb->tab, hb->tab64, \
-                                        hb->ref_normals, hb->p, hb->out, tile, stage);        \
+#define TG_HCASE(S_, TD_, GD_, K_)                                                      \
+  case S_:                                                                               \
+    direct_tile<T, TD_, GD_, K_, SHIFT>(hb->nodes, hb->dofmap, hb->flags, ltab, hb->tab64, \
+                                        hb->ref_normals, hb->p, hb->out, tile, stage);   \
     break;
           TG_HYBRID_SHAPES(TG_HCASE)
 #undef TG_HCASE
@@ -767,14 +793,26 @@ hipError_t launch_factor(int tdim, int gdim, bool shift, bool generic, const T* 
   if (fn == nullptr) fn = generic_kernel<T>(tdim, gdim, shift);
   if (fn == nullptr) return hipErrorNotSupported;
   FactorParams<T> p = pin;
-  // The staged mappings need tiles of 16 and a register-resident kernel.
+  // The staged mappings need tiles of 16 and a register-resident kernel; kMapTile a table that
+  // fits its LDS budget.
   if (!direct || p.tile_log2 != 4) p.mapping = kMapCell;
+  const size_t table_bytes = static_cast<size_t>(p.nq) * p.k * 4 * sizeof(T);
+  if (p.mapping == kMapTile && table_bytes > kMaxLdsTableBytes) p.mapping = kMapCellStaged;
+  const size_t stage_bytes = (kBlock / 64) * kStageElems * sizeof(T);
+  const size_t lds = p.mapping == kMapTile         ? stage_bytes + table_bytes
+                     : p.mapping == kMapCellStaged ? stage_bytes
+                                                   : 0;
   const uint64_t per_block = cells_per_workgroup(p.mapping);
   const uint64_t blocks = (p.padded_cells + per_block - 1) / per_block;
   if (blocks > 0x7fffffffu) return hipErrorInvalidConfiguration;
-  hipLaunchKernelGGL(fn, dim3(static_cast<uint32_t>(blocks)), dim3(kBlock), 0, stream, nodes,
-                     dofmap, flags, tab, tab64, ref_normals, p, out);
+  hipLaunchKernelGGL(fn, dim3(static_cast<uint32_t>(blocks)), dim3(kBlock),
+                     static_cast<uint32_t>(lds), stream, nodes, dofmap, flags, tab, tab64,
+                     ref_normals, p, out);
   return hipGetLastError();
+}
+
+size_t hybrid_lds_bytes(size_t elem, size_t max_table_bytes) {
+  return (kBlock / 64) * kStageElems * elem + max_table_bytes;
 }
 
 int hybrid_shape(int tdim, int gdim, uint32_t k) {
@@ -791,12 +829,12 @@ static auto hybrid_fn(bool shift) {
 }
 
 template <typename T>
-hipError_t hybrid_grid(bool shift, uint32_t* grid) {
+hipError_t hybrid_grid(bool shift, size_t lds, uint32_t* grid) {
   int dev = 0, cus = 0, per_cu = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (e == hipSuccess)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, hybrid_fn<T>(shift), kBlock, 0);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, hybrid_fn<T>(shift), kBlock, lds);
   if (e != hipSuccess) return e;
   const uint32_t n = static_cast<uint32_t>(std::max(1, cus) * std::max(1, per_cu));
   *grid = std::max(8u, (n + 7u) / 8u * 8u);
@@ -806,18 +844,20 @@ hipError_t hybrid_grid(bool shift, uint32_t* grid) {
 template <typename T>
 hipError_t launch_hybrid(bool shift, const HybridBlock<T>* blocks, const WorkItem* items,
                          const uint32_t* xcd_begin, uint32_t* counters, uint32_t grid,
-                         hipStream_t stream) {
-  hipLaunchKernelGGL(hybrid_fn<T>(shift), dim3(grid), dim3(kBlock), 0, stream, blocks, items,
-                     xcd_begin, counters);
+                         size_t lds, hipStream_t stream) {
+  hipLaunchKernelGGL(hybrid_fn<T>(shift), dim3(grid), dim3(kBlock), static_cast<uint32_t>(lds),
+                     stream, blocks, items, xcd_begin, counters);
   return hipGetLastError();
 }
 
-template hipError_t hybrid_grid<float>(bool, uint32_t*);
-template hipError_t hybrid_grid<double>(bool, uint32_t*);
+template hipError_t hybrid_grid<float>(bool, size_t, uint32_t*);
+template hipError_t hybrid_grid<double>(bool, size_t, uint32_t*);
 template hipError_t launch_hybrid<float>(bool, const HybridBlock<float>*, const WorkItem*,
-                                         const uint32_t*, uint32_t*, uint32_t, hipStream_t);
+                                         const uint32_t*, uint32_t*, uint32_t, size_t,
+                                         hipStream_t);
 template hipError_t launch_hybrid<double>(bool, const HybridBlock<double>*, const WorkItem*,
-                                          const uint32_t*, uint32_t*, uint32_t, hipStream_t);
+                                          const uint32_t*, uint32_t*, uint32_t, size_t,
+                                          hipStream_t);
 
 template <typename T>
 hipError_t launch_evector(const T* nodes, const uint32_t* dofmap, uint64_t num_cells,

@@ -32,6 +32,9 @@ inline uint32_t cells_per_workgroup(uint32_t mapping) { return mapping == kMapTi
 // The mapping for (dtype, tdim, gdim, k, nq, tile) and the call's options (tuning.cpp).
 uint32_t choose_mapping(int dtype, int tdim, int gdim, uint32_t k, uint32_t nq, uint32_t tile,
                         uint32_t options);
+// Whether per-block launches remap workgroups to XCD-contiguous ranges (default 1;
+// TETRALOMA_GEOM_XCD_REMAP=0 turns it off).
+uint32_t choose_xcd_remap();
 
 // Scalars of one factor launch. Device arrays are passed as separate restrict-qualified kernel
 // parameters so that the table loads can stay in the scalar cache.
@@ -46,6 +49,7 @@ struct FactorParams {
   uint32_t mask;          // factor bits
   uint32_t fixup;         // 1: recompute det J in FP64 where |det J| < tau |J|_F^3
   uint32_t mapping;       // kMapCell, kMapCellStaged or kMapTile (the last two: tiles of 16)
+  uint32_t xcd_remap;     // 1: workgroups of one XCD take one contiguous range of the block
   double tau;
   T origin[3];            // origin narrowed to T, added back to X
 };
@@ -80,15 +84,20 @@ struct WorkItem {
   uint32_t first_cell;
 };
 
+// Largest basis table (bytes) kMapTile stages in LDS; larger tables run kMapCellStaged.
+constexpr size_t kMaxLdsTableBytes = 32 * 1024;
+
 // Shape index of (tdim, gdim, k) in the persistent mixed-mesh kernel, or -1.
 int hybrid_shape(int tdim, int gdim, uint32_t k);
+// Dynamic LDS of the mixed-mesh kernel: staging buffers plus the largest kMapTile table.
+size_t hybrid_lds_bytes(size_t elem, size_t max_table_bytes);
 // Workgroups that fit on the device at once for the mixed-mesh kernel (a multiple of 8).
 template <typename T>
-hipError_t hybrid_grid(bool shift, uint32_t* grid);
+hipError_t hybrid_grid(bool shift, size_t lds, uint32_t* grid);
 template <typename T>
 hipError_t launch_hybrid(bool shift, const HybridBlock<T>* blocks, const WorkItem* items,
                          const uint32_t* xcd_begin, uint32_t* counters, uint32_t grid,
-                         hipStream_t stream);
+                         size_t lds, hipStream_t stream);
 
 // Launches the factor kernel for (T, tdim, gdim, k); `shift` selects the per-cell origin shift,
 // `generic` forces the runtime-K kernel. Returns hipSuccess or the launch error, and

@@ -29,6 +29,7 @@ struct tg_plan {
   bool shift = false;
   uint32_t grid = 0;
   uint32_t num_items = 0;
+  size_t lds = 0;  // dynamic LDS of the persistent kernel
   std::vector<PlanBlock> blocks;
   void* d_blocks = nullptr;
   tg::WorkItem* d_items = nullptr;
@@ -71,6 +72,7 @@ tg::FactorParams<T> params(const tg_geometry* g, const tg::Block& b, const tg_po
                                                                                             : 0u;
   p.tau = g->tau;
   p.mapping = tg::choose_mapping(g->dtype, b.tdim, g->gdim, b.k, p.nq, tile, options);
+  p.xcd_remap = tg::choose_xcd_remap();
   for (int i = 0; i < 3; ++i) p.origin[i] = static_cast<T>(i < g->gdim ? g->origin[i] : 0.0);
   return p;
 }
@@ -132,7 +134,12 @@ int upload_persistent(tg_plan* plan) {
   if (e == hipSuccess)
     e = hipMemcpy(plan->d_xcd_begin, begin.data(), 9 * sizeof(uint32_t), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemset(plan->d_counters, 0, 9 * sizeof(uint32_t));
-  if (e == hipSuccess) e = tg::hybrid_grid<T>(plan->shift, &plan->grid);
+  size_t max_table = 0;
+  for (const auto& h : hb)
+    if (h.p.mapping == tg::kMapTile)
+      max_table = std::max<size_t>(max_table, static_cast<size_t>(h.p.nq) * h.p.k * 4 * sizeof(T));
+  plan->lds = tg::hybrid_lds_bytes(sizeof(T), max_table);
+  if (e == hipSuccess) e = tg::hybrid_grid<T>(plan->shift, plan->lds, &plan->grid);
   if (e != hipSuccess) return tg::hip_fail(e, "plan upload");
   plan->grid = std::min(plan->grid, (plan->num_items + 15u) / 8u * 8u);
   plan->grid = std::max(plan->grid, 8u);
@@ -161,14 +168,14 @@ hipError_t plan_enqueue(tg_plan* plan, hipStream_t s) {
       return tg::launch_hybrid<float>(plan->shift,
                                       static_cast<const tg::HybridBlock<float>*>(plan->d_blocks),
                                       plan->d_items, plan->d_xcd_begin, plan->d_counters,
-                                      plan->grid, s);
+                                      plan->grid, plan->lds, s);
     return launch_blocks<float>(plan, s);
   }
   if (plan->persistent)
     return tg::launch_hybrid<double>(plan->shift,
                                      static_cast<const tg::HybridBlock<double>*>(plan->d_blocks),
                                      plan->d_items, plan->d_xcd_begin, plan->d_counters,
-                                     plan->grid, s);
+                                     plan->grid, plan->lds, s);
   return launch_blocks<double>(plan, s);
 }
 

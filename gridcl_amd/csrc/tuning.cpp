@@ -34,19 +34,40 @@ uint32_t env_mapping() {
 
 }  // namespace
 
+uint32_t choose_xcd_remap() {
+  static const uint32_t v = [] {
+    const char* e = std::getenv("TETRALOMA_GEOM_XCD_REMAP");
+    return (e && e[0] == '0') ? 0u : 1u;
+  }();
+  return v;
+}
+
 uint32_t choose_mapping(int dtype, int tdim, int gdim, uint32_t k, uint32_t nq, uint32_t tile,
                         uint32_t options) {
   if (tile != 16) return kMapCell;
   if (options & TG_OPT_MAP_CELL) return kMapCell;
   if (options & TG_OPT_MAP_CELL_STAGED) return kMapCellStaged;
-  if (options & TG_OPT_MAP_TILE) return kMapTile;
+  if (options & TG_OPT_MAP_TILE) {
+    const size_t bytes = static_cast<size_t>(nq) * k * 4 * (dtype == TG_F64 ? 8 : 4);
+    return bytes > kMaxLdsTableBytes ? static_cast<uint32_t>(kMapCellStaged)
+                                     : static_cast<uint32_t>(kMapTile);
+  }
   const uint32_t env = env_mapping();
-  if (env != 0xffffffffu) return env;
+  if (env != 0xffffffffu) {
+    const size_t bytes = static_cast<size_t>(nq) * k * 4 * (dtype == TG_F64 ? 8 : 4);
+    return (env == kMapTile && bytes > kMaxLdsTableBytes) ? static_cast<uint32_t>(kMapCellStaged)
+                                                          : env;
+  }
+  uint32_t m = kDefaultMapping;
   for (const TuneEntry& e : kTuned)
     if (e.dtype == dtype && e.tdim == tdim && e.gdim == gdim && e.k == k && nq >= e.nq_lo &&
-        nq <= e.nq_hi)
-      return e.mapping;
-  return kDefaultMapping;
+        nq <= e.nq_hi) {
+      m = e.mapping;
+      break;
+    }
+  const size_t table_bytes = static_cast<size_t>(nq) * k * 4 * (dtype == TG_F64 ? 8 : 4);
+  if (m == kMapTile && table_bytes > kMaxLdsTableBytes) m = kMapCellStaged;
+  return m;
 }
 
 }  // namespace tg
