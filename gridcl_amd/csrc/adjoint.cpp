@@ -15,9 +15,9 @@ struct tg_functional {
   tg::DeviceTables* tabs = nullptr;
   const double* wts = nullptr;
   double* partials = nullptr;   // one FP64 per forward workgroup
-  double* contrib = nullptr;    // [cells][k][3] FP64
+  double* contrib = nullptr;    // [k][cells][3] FP64
   uint64_t* offsets = nullptr;  // [nodes + 1]
-  uint32_t* entries = nullptr;  // [cells * k]: cell * k + local node, grouped by node
+  uint32_t* entries = nullptr;  // [cells * k]: local node * cells + cell, grouped by node
   ~tg_functional() {
     for (void* p : {static_cast<void*>(partials), static_cast<void*>(contrib),
                     static_cast<void*>(offsets), static_cast<void*>(entries)})
@@ -50,9 +50,11 @@ int build_incidence(tg_functional* f) {
   for (uint64_t i = 0; i < nn; ++i) offsets[i + 1] += offsets[i];
   std::vector<uint64_t> fill(offsets.begin(), offsets.end() - 1);
   std::vector<uint32_t> entries(n_entries);
+  // Entries in ascending (cell, local node) order per node; each names the contribution's slot
+  // k * cells + cell of the [k][cell][3] contribution array.
   for (uint64_t i = 0; i < n_entries; ++i) {
     const uint32_t v = dofmap[i] < last ? dofmap[i] : last;
-    entries[fill[v]++] = static_cast<uint32_t>(i);
+    entries[fill[v]++] = static_cast<uint32_t>((i % b.k) * b.num_cells + i / b.k);
   }
   int rc = alloc(&f->offsets, nn + 1, "hipMalloc(offsets)");
   if (!rc) rc = alloc(&f->entries, n_entries, "hipMalloc(entries)");

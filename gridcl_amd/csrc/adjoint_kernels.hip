@@ -8,7 +8,8 @@
 // each point, the integrand in T, accumulated per cell in FP64; a fixed tree per workgroup
 // writes one FP64 partial; a single-workgroup pass sums the partials in order.
 // Backward: J is recomputed (not stored), dF/dJ is formed in FP64 from it, and the lane writes
-// its cell's contributions dF/dx(k, g) = sum_q sum_d dF/dJ(g, d) dphi_k/dxi_d in FP64; a node
+// its cell's contributions dF/dx(k, g) = sum_q sum_d dF/dJ(g, d) dphi_k/dxi_d in FP64 (stored
+// [k][cell][3], so that a wavefront's stores are contiguous); a node
 // pass then sums each node's contributions in CSR order (ascending cell, then local node), so
 // the gradient is bitwise reproducible without atomics.
 #include <hip/hip_runtime.h>
@@ -63,6 +64,13 @@ __device__ __forceinline__ void jacobian(const T (&x)[K][3], const T* __restrict
       for (int k = K0 + 1; k < K; ++k) acc = tfma(x[k][g], t[4 * k + 1 + d], acc);
       j[d * 3 + g] = acc;
     }
+}
+
+// Orders the LDS accesses of a wavefront before and after this point.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // Fixed-order tree over the workgroup; thread 0 returns the sum.
@@ -123,8 +131,13 @@ __global__ void __launch_bounds__(kBlock)
                          const double* __restrict__ wts, uint32_t nq, uint64_t num_cells,
                          uint32_t last_node, double beta, const double* __restrict__ seed,
                          double* __restrict__ contrib) {
-  const uint64_t c = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
-  if (c >= num_cells) return;
+  __shared__ double lds[kBlock / 64][64 * 3];
+  const uint64_t cl = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  const uint32_t w = threadIdx.x & 63u;
+  const uint64_t c0 = cl - w;  // the wavefront's first cell
+  if (c0 >= num_cells) return;  // whole wavefronts stay: they share the LDS transpose
+  const uint64_t c = cl < num_cells ? cl : num_cells - 1;
+  const uint32_t valid = num_cells - c0 < 64 ? static_cast<uint32_t>(num_cells - c0) : 64u;
   T x[K][3];
   gather<T, K, SHIFT>(nodes, dofmap + c * K, last_node, x);
   double ct[K][3];
@@ -168,31 +181,34 @@ __global__ void __launch_bounds__(kBlock)
         ct[k][g] = __builtin_fma(G[g], d0, __builtin_fma(G[3 + g], d1, __builtin_fma(G[6 + g], d2, ct[k][g])));
     }
   }
+  // Contributions are stored [k][cell][3]: for each local node the wavefront's 64 cells are
+  // 192 consecutive doubles, transposed through LDS so that every store is contiguous.
   const double s = seed ? *seed : 1.0;
-  double* out = contrib + c * K * 3;
+  double* st = lds[threadIdx.x >> 6];
 #pragma unroll
-  for (int k = 0; k < K; ++k)
+  for (int k = 0; k < K; ++k) {
 #pragma unroll
-    for (int g = 0; g < 3; ++g) out[k * 3 + g] = s * ct[k][g];
+    for (int g = 0; g < 3; ++g) st[w * 3 + g] = s * ct[k][g];
+    wave_sync();
+    double* dst = contrib + (static_cast<uint64_t>(k) * num_cells + c0) * 3;
+    for (uint32_t i = w; i < valid * 3; i += 64) dst[i] = st[i];
+    wave_sync();
+  }
 }
 
 // grad[n] = sum over the CSR entries of node n, in order, of contrib[entry] (FP64).
 __global__ void __launch_bounds__(kBlock)
     csr_node_sum(const uint64_t* __restrict__ offsets, const uint32_t* __restrict__ entries,
                  const double* __restrict__ contrib, uint64_t num_nodes, double* __restrict__ grad) {
-  const uint64_t n = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
-  if (n >= num_nodes) return;
-  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  // One lane per (node, component): three lanes share a node's entries.
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= num_nodes * 3) return;
+  const uint64_t n = i / 3;
+  const uint32_t g = static_cast<uint32_t>(i - n * 3);
+  double a = 0.0;
   const uint64_t e1 = offsets[n + 1];
-  for (uint64_t e = offsets[n]; e < e1; ++e) {
-    const double* v = contrib + static_cast<uint64_t>(entries[e]) * 3;
-    a0 += v[0];
-    a1 += v[1];
-    a2 += v[2];
-  }
-  grad[n * 3] = a0;
-  grad[n * 3 + 1] = a1;
-  grad[n * 3 + 2] = a2;
+  for (uint64_t e = offsets[n]; e < e1; ++e) a += contrib[static_cast<uint64_t>(entries[e]) * 3 + g];
+  grad[i] = a;
 }
 
 __global__ void __launch_bounds__(kBlock) fill_zero(double* __restrict__ p, uint64_t n) {
@@ -266,7 +282,7 @@ hipError_t launch_functional_vjp(const T* nodes, const uint32_t* dofmap, uint32_
                      num_cells, last_node, beta, seed, contrib);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  const uint64_t nb = (num_nodes + kBlock - 1) / kBlock;
+  const uint64_t nb = (num_nodes * 3 + kBlock - 1) / kBlock;
   hipLaunchKernelGGL(csr_node_sum, dim3(static_cast<uint32_t>(nb)), dim3(kBlock), 0, stream,
                      offsets, entries, contrib, num_nodes, grad);
   return hipGetLastError();

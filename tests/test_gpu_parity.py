@@ -118,7 +118,9 @@ def test_factors_match_oracle(cuda, kind, cell, p, dtype):
         for name, e in err.items():
             assert e <= 1e-12, (name, e)
     else:
-        base = 1e-6 if direct else 1e-5
+        # 1e-6 for register-resident kernels up to 18 nodes per cell; their 20-64-node sums
+        # (P3 tetrahedra, Q2/Q3 hexahedra) round to a few 1e-6.
+        base = (1e-6 if k <= 18 else 4e-6) if direct else 1e-5
         for name in ("points", "jacobians", "detj"):
             assert err[name] <= base, (name, err[name])
         for name in ("pinv", "normals"):
