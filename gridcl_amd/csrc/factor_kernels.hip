@@ -836,7 +836,10 @@ hipError_t hybrid_grid(bool shift, size_t lds, uint32_t* grid) {
   if (e == hipSuccess)
     e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, hybrid_fn<T>(shift), kBlock, lds);
   if (e != hipSuccess) return e;
-  const uint32_t n = static_cast<uint32_t>(std::max(1, cus) * std::max(1, per_cu));
+  // Work comes from a queue and no workgroup waits for another, so launching more workgroups
+  // than the occupancy query reports is safe; at least 8 per CU keeps every SIMD supplied when
+  // the query undercounts (it does not see that the dynamic LDS leaves room for more).
+  const uint32_t n = static_cast<uint32_t>(std::max(1, cus) * std::max(8, per_cu));
   *grid = std::max(8u, (n + 7u) / 8u * 8u);
   return hipSuccess;
 }
