@@ -5,6 +5,7 @@ element library is plain C++17. The result is ``gridcl_amd/lib/libtetraloma_geom
 """
 from __future__ import annotations
 
+import concurrent.futures
 import os
 import subprocess
 import sys
@@ -15,7 +16,12 @@ OUT_DIR = os.path.join(HERE, "lib")
 OBJ_DIR = os.path.join(HERE, "_build")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["factor_kernels.hip", "adjoint_kernels.hip"]
+# (source, extra defines): the factor kernels compile once per coordinate type.
+HIP_SOURCES = [
+    ("factor_kernels.hip", ["-DTG_FACTOR_TYPE=float", "-DTG_FACTOR_COMMON"]),
+    ("factor_kernels.hip", ["-DTG_FACTOR_TYPE=double"]),
+    ("adjoint_kernels.hip", []),
+]
 HOST_HIP_SOURCES = ["capi.cpp", "adjoint.cpp", "plan.cpp", "tuning.cpp"]  # host code that calls the HIP runtime
 HOST_SOURCES = ["refcell.cpp"]
 
@@ -39,23 +45,27 @@ def _run(cmd: list[str]) -> None:
 
 
 def build_library(verbose: bool = False) -> str:
-    """Compiles every source and links the shared library; returns its path."""
+    """Compiles every source (in parallel) and links the shared library; returns its path."""
     os.makedirs(OUT_DIR, exist_ok=True)
     os.makedirs(OBJ_DIR, exist_ok=True)
-    objs = []
-    for src in HIP_SOURCES:
-        obj = os.path.join(OBJ_DIR, src + ".o")
-        _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
-        objs.append(obj)
+    jobs = []
+    for src, defs in HIP_SOURCES:
+        tag = "".join(d.split("=")[-1][:1] for d in defs if "=" in d)
+        obj = os.path.join(OBJ_DIR, f"{src}{'.' + tag if tag else ''}.o")
+        jobs.append((obj, [HIPCC, *HIP_FLAGS, *defs, "-c", os.path.join(CSRC, src), "-o", obj]))
     for src in HOST_HIP_SOURCES:
         obj = os.path.join(OBJ_DIR, src + ".o")
-        _run([HIPCC, "-O3", "-std=c++17", "-fPIC", "-c", os.path.join(CSRC, src), "-o", obj])
-        objs.append(obj)
+        jobs.append((obj, [HIPCC, "-O3", "-std=c++17", "-fPIC", "-c", os.path.join(CSRC, src),
+                           "-o", obj]))
     for src in HOST_SOURCES:
         obj = os.path.join(OBJ_DIR, src + ".o")
-        _run(["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-c", os.path.join(CSRC, src),
-              "-o", obj])
-        objs.append(obj)
+        jobs.append((obj, ["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-c",
+                           os.path.join(CSRC, src), "-o", obj]))
+    workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with concurrent.futures.ThreadPoolExecutor(workers) as pool:
+        for f in [pool.submit(_run, cmd) for _, cmd in jobs]:
+            f.result()
+    objs = [obj for obj, _ in jobs]
     out = os.path.join(OUT_DIR, "libtetraloma_geom.so")
     tmp = out + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp])

@@ -7,9 +7,9 @@
 // Forward: one lane per cell; J in T (FP32: coordinates shifted to the cell's first node) at
 // each point, the integrand in T, accumulated per cell in FP64; a fixed tree per workgroup
 // writes one FP64 partial; a single-workgroup pass sums the partials in order.
-// Backward: J is recomputed (not stored), dF/dJ is formed in FP64 from it, and the lane writes
-// its cell's contributions dF/dx(k, g) = sum_q sum_d dF/dJ(g, d) dphi_k/dxi_d in FP64 (stored
-// [k][cell][3], so that a wavefront's stores are contiguous); a node
+// Backward: J is recomputed (not stored), dF/dJ is formed from it in T, and the lane writes its
+// cell's contributions dF/dx(k, g) = sum_q sum_d dF/dJ(g, d) dphi_k/dxi_d, widened to FP64
+// (stored [k][cell][3], so that a wavefront's stores are contiguous); a node
 // pass then sums each node's contributions in CSR order (ascending cell, then local node), so
 // the gradient is bitwise reproducible without atomics.
 #include <hip/hip_runtime.h>
@@ -140,17 +140,18 @@ __global__ void __launch_bounds__(kBlock)
   const uint32_t valid = num_cells - c0 < 64 ? static_cast<uint32_t>(num_cells - c0) : 64u;
   T x[K][3];
   gather<T, K, SHIFT>(nodes, dofmap + c * K, last_node, x);
-  double ct[K][3];
+  // Per-cell arithmetic in the geometry's type (J recomputed, dF/dJ, contributions summed over
+  // the points); the node reduction that follows is FP64.
+  T ct[K][3];
 #pragma unroll
-  for (int k = 0; k < K; ++k) ct[k][0] = ct[k][1] = ct[k][2] = 0.0;
+  for (int k = 0; k < K; ++k) ct[k][0] = ct[k][1] = ct[k][2] = T(0);
+  const T b = static_cast<T>(beta);
   for (uint32_t q = 0; q < nq; ++q) {
-    T jt[9];
-    jacobian<T, K, SHIFT>(x, tab + static_cast<uint64_t>(q) * K * 4, jt);
-    double j[9];
-#pragma unroll
-    for (int i = 0; i < 9; ++i) j[i] = static_cast<double>(jt[i]);
+    const T* __restrict__ t = tab + static_cast<uint64_t>(q) * K * 4;
+    T j[9];
+    jacobian<T, K, SHIFT>(x, t, j);
     // Cofactors cof(g, d) = d det / d J(g, d), stored like J at d * 3 + g.
-    double cof[9];
+    T cof[9];
     cof[0] = j[4] * j[8] - j[7] * j[5];
     cof[3] = -(j[1] * j[8] - j[7] * j[2]);
     cof[6] = j[1] * j[5] - j[4] * j[2];
@@ -160,25 +161,24 @@ __global__ void __launch_bounds__(kBlock)
     cof[2] = j[3] * j[7] - j[6] * j[4];
     cof[5] = -(j[0] * j[7] - j[6] * j[1]);
     cof[8] = j[0] * j[4] - j[3] * j[1];
-    const double det = j[0] * cof[0] + j[3] * cof[3] + j[6] * cof[6];
-    double n2 = 0.0;
+    const T det = j[0] * cof[0] + j[3] * cof[3] + j[6] * cof[6];
+    T n2 = T(0);
 #pragma unroll
-    for (int i = 0; i < 9; ++i) n2 = __builtin_fma(j[i], j[i], n2);
-    const double r = cbrt(det);
-    const double inv23 = 1.0 / (r * r);
-    const double w = wts[q];
-    const double a = w * (1.0 - beta * (2.0 / 3.0) * n2 * inv23 / det);
-    const double bb = w * beta * 2.0 * inv23;
-    double G[9];
+    for (int i = 0; i < 9; ++i) n2 = tfma(j[i], j[i], n2);
+    const T r = tcbrt(det);
+    const T inv23 = T(1) / (r * r);
+    const T w = static_cast<T>(wts[q]);
+    const T a = w * (T(1) - b * (T(2) / T(3)) * n2 * inv23 / det);
+    const T bb = w * b * T(2) * inv23;
+    T G[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) G[i] = a * cof[i] + bb * j[i];
-    const double* t = tab64 + static_cast<uint64_t>(q) * K * 4;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      const double d0 = t[4 * k + 1], d1 = t[4 * k + 2], d2 = t[4 * k + 3];
+      const T d0 = t[4 * k + 1], d1 = t[4 * k + 2], d2 = t[4 * k + 3];
 #pragma unroll
       for (int g = 0; g < 3; ++g)
-        ct[k][g] = __builtin_fma(G[g], d0, __builtin_fma(G[3 + g], d1, __builtin_fma(G[6 + g], d2, ct[k][g])));
+        ct[k][g] = tfma(G[g], d0, tfma(G[3 + g], d1, tfma(G[6 + g], d2, ct[k][g])));
     }
   }
   // Contributions are stored [k][cell][3]: for each local node the wavefront's 64 cells are
@@ -188,7 +188,7 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
   for (int k = 0; k < K; ++k) {
 #pragma unroll
-    for (int g = 0; g < 3; ++g) st[w * 3 + g] = s * ct[k][g];
+    for (int g = 0; g < 3; ++g) st[w * 3 + g] = s * static_cast<double>(ct[k][g]);
     wave_sync();
     double* dst = contrib + (static_cast<uint64_t>(k) * num_cells + c0) * 3;
     for (uint32_t i = w; i < valid * 3; i += 64) dst[i] = st[i];

@@ -553,28 +553,22 @@ __global__ void __launch_bounds__(kBlock)
   X(12, 2, 3, 4) X(13, 2, 3, 9) X(14, 2, 2, 3) X(15, 2, 2, 6) X(16, 2, 2, 10) X(17, 2, 2, 4) \
   X(18, 2, 2, 9)
 
-// One persistent launch over the work items of every block of a mesh. Items are split into
-// eight contiguous ranges, one per XCD (workgroup b runs on XCD b % 8); the workgroups of an
-// XCD take items of their range in order from one counter. An item is kBlock / 64 tiles of 16
-// cells (tiles of 16) or kBlock cells (other tiles). The last workgroup to finish resets the
-// counters, so the next launch (or graph replay) starts from zero.
+// One persistent launch over the work items of every block of a mesh: items, in block order,
+// are split on the host into one contiguous range of about equal work per workgroup, and the
+// ranges of the workgroups of one XCD are adjacent (workgroup b runs on XCD b % 8), so each XCD
+// walks one stretch of the mesh. An item is kBlock / 64 tiles of 16 cells (kMapTile) or kBlock
+// cells. No counters: the launch is deterministic and can be replayed from a graph as is.
 template <typename T, bool SHIFT>
 __global__ void __launch_bounds__(kBlock)
     factor_hybrid(const HybridBlock<T>* __restrict__ blocks, const WorkItem* __restrict__ items,
-                  const uint32_t* __restrict__ xcd_begin, uint32_t* counters) {
-  __shared__ uint32_t s_item;
+                  const uint32_t* __restrict__ ranges) {
   extern __shared__ __align__(16) unsigned char tg_lds[];
   T* stages = reinterpret_cast<T*>(tg_lds);
   T* ltab = stages + (kBlock / 64) * kStageElems;  // the table of the kMapTile block in hand
   uint32_t table_block = 0xffffffffu;
-  const uint32_t x = blockIdx.x & 7u;
-  const uint32_t begin = xcd_begin[x], end = xcd_begin[x + 1];
-  for (;;) {
-    if (threadIdx.x == 0) s_item = begin + atomicAdd(&counters[x], 1u);
-    __syncthreads();
-    const uint32_t it = __builtin_amdgcn_readfirstlane(s_item);
-    __syncthreads();
-    if (it >= end) break;
+  const uint32_t r = xcd_block(blockIdx.x, gridDim.x);
+  const uint32_t begin = ranges[r], end = ranges[r + 1];
+  for (uint32_t it = begin; it < end; ++it) {
     const WorkItem w = items[it];
     const HybridBlock<T>* __restrict__ hb = blocks + w.block;
     if (hb->p.mapping == kMapTile && w.block != table_block) {
@@ -632,13 +626,6 @@ __global__ void __launch_bounds__(kBlock)
             break;
         }
       }
-    }
-  }
-  if (threadIdx.x == 0) {
-    __threadfence();
-    if (atomicAdd(&counters[8], 1u) == gridDim.x - 1) {
-      for (int i = 0; i < 9; ++i) atomicExch(&counters[i], 0u);
-      __threadfence();
     }
   }
 }
@@ -811,6 +798,7 @@ hipError_t launch_factor(int tdim, int gdim, bool shift, bool generic, const T* 
   return hipGetLastError();
 }
 
+#ifdef TG_FACTOR_COMMON  // type-independent helpers: defined in one of the two objects
 size_t hybrid_lds_bytes(size_t elem, size_t max_table_bytes) {
   return (kBlock / 64) * kStageElems * elem + max_table_bytes;
 }
@@ -822,6 +810,7 @@ int hybrid_shape(int tdim, int gdim, uint32_t k) {
 #undef TG_HSHAPE
   return -1;
 }
+#endif
 
 template <typename T>
 static auto hybrid_fn(bool shift) {
@@ -830,37 +819,31 @@ static auto hybrid_fn(bool shift) {
 
 template <typename T>
 hipError_t hybrid_grid(bool shift, size_t lds, uint32_t* grid) {
-  int dev = 0, cus = 0, per_cu = 0;
+  // Resident workgroups per CU from the kernel's registers and LDS: 512 VGPRs per SIMD lane, one
+  // wavefront of each workgroup per SIMD, 160 KiB of LDS per CU, at most 8 wavefronts per SIMD.
+  int dev = 0, cus = 0;
+  hipFuncAttributes attr{};
   hipError_t e = hipGetDevice(&dev);
   if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   if (e == hipSuccess)
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, hybrid_fn<T>(shift), kBlock, lds);
+    e = hipFuncGetAttributes(&attr, reinterpret_cast<const void*>(hybrid_fn<T>(shift)));
   if (e != hipSuccess) return e;
-  // Work comes from a queue and no workgroup waits for another, so launching more workgroups
-  // than the occupancy query reports is safe; at least 8 per CU keeps every SIMD supplied when
-  // the query undercounts (it does not see that the dynamic LDS leaves room for more).
-  const uint32_t n = static_cast<uint32_t>(std::max(1, cus) * std::max(8, per_cu));
-  *grid = std::max(8u, (n + 7u) / 8u * 8u);
+  const int regs = std::max(8, (attr.numRegs + 7) / 8 * 8);
+  const int by_regs = std::min(8, 512 / regs);
+  const size_t per_wg = lds + attr.sharedSizeBytes;
+  const int by_lds = per_wg ? static_cast<int>((160 * 1024) / per_wg) : 8;
+  const int per_cu = std::max(1, std::min(by_regs, by_lds));
+  *grid = static_cast<uint32_t>(std::max(1, cus) * per_cu);
   return hipSuccess;
 }
 
 template <typename T>
 hipError_t launch_hybrid(bool shift, const HybridBlock<T>* blocks, const WorkItem* items,
-                         const uint32_t* xcd_begin, uint32_t* counters, uint32_t grid,
-                         size_t lds, hipStream_t stream) {
+                         const uint32_t* ranges, uint32_t grid, size_t lds, hipStream_t stream) {
   hipLaunchKernelGGL(hybrid_fn<T>(shift), dim3(grid), dim3(kBlock), static_cast<uint32_t>(lds),
-                     stream, blocks, items, xcd_begin, counters);
+                     stream, blocks, items, ranges);
   return hipGetLastError();
 }
-
-template hipError_t hybrid_grid<float>(bool, size_t, uint32_t*);
-template hipError_t hybrid_grid<double>(bool, size_t, uint32_t*);
-template hipError_t launch_hybrid<float>(bool, const HybridBlock<float>*, const WorkItem*,
-                                         const uint32_t*, uint32_t*, uint32_t, size_t,
-                                         hipStream_t);
-template hipError_t launch_hybrid<double>(bool, const HybridBlock<double>*, const WorkItem*,
-                                          const uint32_t*, uint32_t*, uint32_t, size_t,
-                                          hipStream_t);
 
 template <typename T>
 hipError_t launch_evector(const T* nodes, const uint32_t* dofmap, uint64_t num_cells,
@@ -873,17 +856,19 @@ hipError_t launch_evector(const T* nodes, const uint32_t* dofmap, uint64_t num_c
   return hipGetLastError();
 }
 
-template hipError_t launch_factor<float>(int, int, bool, bool, const float*, const uint32_t*,
-                                         const uint32_t*, const float*, const double*,
-                                         const float*, const FactorParams<float>&,
-                                         const FactorOutputs&, hipStream_t);
-template hipError_t launch_factor<double>(int, int, bool, bool, const double*, const uint32_t*,
-                                          const uint32_t*, const double*, const double*,
-                                          const double*, const FactorParams<double>&,
-                                          const FactorOutputs&, hipStream_t);
-template hipError_t launch_evector<float>(const float*, const uint32_t*, uint64_t, uint64_t,
-                                          uint32_t, int, uint32_t, float*, hipStream_t);
-template hipError_t launch_evector<double>(const double*, const uint32_t*, uint64_t, uint64_t,
-                                           uint32_t, int, uint32_t, double*, hipStream_t);
+// The build compiles this file once per coordinate type (TG_FACTOR_TYPE = float or double) so
+// that the two halves of the kernel set compile in parallel.
+#ifndef TG_FACTOR_TYPE
+#error "compile with -DTG_FACTOR_TYPE=float or -DTG_FACTOR_TYPE=double"
+#endif
+using FT = TG_FACTOR_TYPE;
+template hipError_t hybrid_grid<FT>(bool, size_t, uint32_t*);
+template hipError_t launch_hybrid<FT>(bool, const HybridBlock<FT>*, const WorkItem*,
+                                      const uint32_t*, uint32_t, size_t, hipStream_t);
+template hipError_t launch_factor<FT>(int, int, bool, bool, const FT*, const uint32_t*,
+                                      const uint32_t*, const FT*, const double*, const FT*,
+                                      const FactorParams<FT>&, const FactorOutputs&, hipStream_t);
+template hipError_t launch_evector<FT>(const FT*, const uint32_t*, uint64_t, uint64_t, uint32_t,
+                                       int, uint32_t, FT*, hipStream_t);
 
 }  // namespace tg

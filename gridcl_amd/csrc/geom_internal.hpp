@@ -78,7 +78,7 @@ struct HybridBlock {
   uint32_t pad_;
 };
 
-// One work item: 256 consecutive lanes (cells, padding included) of one block.
+// One work item: the cells of one workgroup (padding included) of one block.
 struct WorkItem {
   uint32_t block;
   uint32_t first_cell;
@@ -91,13 +91,14 @@ constexpr size_t kMaxLdsTableBytes = 32 * 1024;
 int hybrid_shape(int tdim, int gdim, uint32_t k);
 // Dynamic LDS of the mixed-mesh kernel: staging buffers plus the largest kMapTile table.
 size_t hybrid_lds_bytes(size_t elem, size_t max_table_bytes);
-// Workgroups that fit on the device at once for the mixed-mesh kernel (a multiple of 8).
+// Workgroups that are resident on the device at once for the mixed-mesh kernel.
 template <typename T>
 hipError_t hybrid_grid(bool shift, size_t lds, uint32_t* grid);
+// Launches the mixed-mesh kernel: workgroup b's items are [ranges[r], ranges[r + 1]) with r the
+// XCD-contiguous index of b (grid + 1 entries).
 template <typename T>
 hipError_t launch_hybrid(bool shift, const HybridBlock<T>* blocks, const WorkItem* items,
-                         const uint32_t* xcd_begin, uint32_t* counters, uint32_t grid,
-                         size_t lds, hipStream_t stream);
+                         const uint32_t* ranges, uint32_t grid, size_t lds, hipStream_t stream);
 
 // Launches the factor kernel for (T, tdim, gdim, k); `shift` selects the per-cell origin shift,
 // `generic` forces the runtime-K kernel. Returns hipSuccess or the launch error, and

@@ -33,8 +33,7 @@ struct tg_plan {
   std::vector<PlanBlock> blocks;
   void* d_blocks = nullptr;
   tg::WorkItem* d_items = nullptr;
-  uint32_t* d_xcd_begin = nullptr;
-  uint32_t* d_counters = nullptr;
+  uint32_t* d_ranges = nullptr;  // [grid + 1]: the item range of each workgroup
   hipStream_t capture = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -42,8 +41,7 @@ struct tg_plan {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (capture) (void)hipStreamDestroy(capture);
-    for (void* p : {d_blocks, static_cast<void*>(d_items), static_cast<void*>(d_xcd_begin),
-                    static_cast<void*>(d_counters)})
+    for (void* p : {d_blocks, static_cast<void*>(d_items), static_cast<void*>(d_ranges)})
       if (p) (void)hipFree(p);
   }
 };
@@ -109,40 +107,42 @@ int upload_persistent(tg_plan* plan) {
       items.push_back(tg::WorkItem{static_cast<uint32_t>(i), static_cast<uint32_t>(c)});
       item_cost.push_back(cost[i]);
     }
+  plan->num_items = static_cast<uint32_t>(items.size());
+  size_t max_table = 0;
+  for (const auto& h : hb)
+    if (h.p.mapping == tg::kMapTile)
+      max_table = std::max<size_t>(max_table, static_cast<size_t>(h.p.nq) * h.p.k * 4 * sizeof(T));
+  plan->lds = tg::hybrid_lds_bytes(sizeof(T), max_table);
+  hipError_t e = tg::hybrid_grid<T>(plan->shift, plan->lds, &plan->grid);
+  if (e != hipSuccess) return tg::hip_fail(e, "plan occupancy");
+  plan->grid = std::max(1u, std::min(plan->grid, plan->num_items));
+  // One contiguous range of about equal cost per workgroup, in item (block) order.
   double total = 0.0;
   for (double c : item_cost) total += c;
-  std::vector<uint32_t> begin(9, static_cast<uint32_t>(items.size()));
-  begin[0] = 0;
+  const uint32_t G = plan->grid;
+  std::vector<uint32_t> ranges(G + 1, static_cast<uint32_t>(items.size()));
+  ranges[0] = 0;
   double acc = 0.0;
-  int x = 1;
-  for (size_t i = 0; i < items.size() && x < 8; ++i) {
+  uint32_t r = 1;
+  for (size_t i = 0; i < items.size() && r < G; ++i) {
     acc += item_cost[i];
-    while (x < 8 && acc >= total * x / 8.0) begin[x++] = static_cast<uint32_t>(i + 1);
+    while (r < G && acc >= total * r / G) ranges[r++] = static_cast<uint32_t>(i + 1);
   }
-  plan->num_items = static_cast<uint32_t>(items.size());
-  hipError_t e = hipMalloc(&plan->d_blocks, std::max<size_t>(1, hb.size()) * sizeof(hb[0]));
+  e = hipMalloc(&plan->d_blocks, std::max<size_t>(1, hb.size()) * sizeof(hb[0]));
   if (e == hipSuccess)
     e = hipMalloc(reinterpret_cast<void**>(&plan->d_items),
                   std::max<size_t>(1, items.size()) * sizeof(tg::WorkItem));
-  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&plan->d_xcd_begin), 9 * sizeof(uint32_t));
-  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&plan->d_counters), 9 * sizeof(uint32_t));
+  if (e == hipSuccess)
+    e = hipMalloc(reinterpret_cast<void**>(&plan->d_ranges), (G + 1) * sizeof(uint32_t));
   if (e == hipSuccess && !hb.empty())
     e = hipMemcpy(plan->d_blocks, hb.data(), hb.size() * sizeof(hb[0]), hipMemcpyHostToDevice);
   if (e == hipSuccess && !items.empty())
     e = hipMemcpy(plan->d_items, items.data(), items.size() * sizeof(tg::WorkItem),
                   hipMemcpyHostToDevice);
   if (e == hipSuccess)
-    e = hipMemcpy(plan->d_xcd_begin, begin.data(), 9 * sizeof(uint32_t), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemset(plan->d_counters, 0, 9 * sizeof(uint32_t));
-  size_t max_table = 0;
-  for (const auto& h : hb)
-    if (h.p.mapping == tg::kMapTile)
-      max_table = std::max<size_t>(max_table, static_cast<size_t>(h.p.nq) * h.p.k * 4 * sizeof(T));
-  plan->lds = tg::hybrid_lds_bytes(sizeof(T), max_table);
-  if (e == hipSuccess) e = tg::hybrid_grid<T>(plan->shift, plan->lds, &plan->grid);
+    e = hipMemcpy(plan->d_ranges, ranges.data(), (G + 1) * sizeof(uint32_t),
+                  hipMemcpyHostToDevice);
   if (e != hipSuccess) return tg::hip_fail(e, "plan upload");
-  plan->grid = std::min(plan->grid, (plan->num_items + 15u) / 8u * 8u);
-  plan->grid = std::max(plan->grid, 8u);
   return TG_OK;
 }
 
@@ -167,15 +167,13 @@ hipError_t plan_enqueue(tg_plan* plan, hipStream_t s) {
     if (plan->persistent)
       return tg::launch_hybrid<float>(plan->shift,
                                       static_cast<const tg::HybridBlock<float>*>(plan->d_blocks),
-                                      plan->d_items, plan->d_xcd_begin, plan->d_counters,
-                                      plan->grid, plan->lds, s);
+                                      plan->d_items, plan->d_ranges, plan->grid, plan->lds, s);
     return launch_blocks<float>(plan, s);
   }
   if (plan->persistent)
     return tg::launch_hybrid<double>(plan->shift,
                                      static_cast<const tg::HybridBlock<double>*>(plan->d_blocks),
-                                     plan->d_items, plan->d_xcd_begin, plan->d_counters,
-                                     plan->grid, plan->lds, s);
+                                     plan->d_items, plan->d_ranges, plan->grid, plan->lds, s);
   return launch_blocks<double>(plan, s);
 }
 

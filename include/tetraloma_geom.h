@@ -219,8 +219,9 @@ int tg_plan_graph_launch(tg_plan* plan, void* stream);
  * F(X) = sum_c sum_q w_q det J_cq(X) + beta sum_c sum_q w_q |J_cq(X)|_F^2 / det(J_cq(X))^(2/3)
  * over one block of 3D cells in 3D at the points and weights of a point set. The forward runs in
  * the geometry's type (F32: J of coordinates shifted to each cell's first node) and reduces in
- * FP64 in a fixed order; the vector-Jacobian product recomputes J, forms dF/dJ in FP64 and sums
- * each node's per-(cell, local node) contributions in FP64 in CSR order, so both are bitwise
+ * FP64 in a fixed order; the vector-Jacobian product recomputes J, forms dF/dJ and each
+ * (cell, local node) contribution in the geometry's type, and sums each node's contributions in
+ * FP64 in CSR order (ascending cell, then local node), so both are bitwise
  * reproducible. The reduction follows Mesh::measure (tetraloma/src/mesh.rs:579;
  * tetraloma-geometry/src/measure.rs:17-102). */
 typedef struct tg_functional tg_functional;
