@@ -51,6 +51,22 @@ def time_steps(fn, steps: int, warmup: int):
     return start.elapsed_time(stop) / steps  # ms per step
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "factor_direct_p2tet_64cube_hbm_traffic.txt")
+
+
+def profiled_traffic():
+    """HBM bytes per launch of the FP32 kernel on the 64^3 x 6 mesh, as measured by the two
+    rocprofv3 counter runs (FETCH_SIZE and WRITE_SIZE, calibrated on a device copy) whose
+    summary is committed with the benchmark; None if that summary is absent."""
+    try:
+        for line in open(TRAFFIC_PROFILE):
+            if line.strip().startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def spin_up(fn, seconds: float) -> int:
     """Runs `fn` back to back for about `seconds` of wall time, untimed, so that the GPU clocks
     have left their idle state before the warm-up and timed steps. Returns the number of calls."""
@@ -317,7 +333,7 @@ def main() -> int:
         },
         "roofline": {"bound": "hbm", "achieved": r32["achieved_gbs"], "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": r32["achieved_gbs"] / PEAK_HBM_GBS,
-                     "traffic": None},
+                     "traffic": profiled_traffic() if args.n == 64 else None},
         "cpu_baseline": cpu,
         "fp64": {"evals_per_s": r64["evals_per_s"], "ms_per_step": r64["ms"],
                  "achieved_gbs": r64["achieved_gbs"],

@@ -1,0 +1,67 @@
+"""Short text summaries of rocprofv3 CSV output, for the committed profile summaries.
+
+    python -m gridcl_amd.profsum stats  <prefix>_kernel_stats.csv [--top N]
+    python -m gridcl_amd.profsum pmc    <prefix>_counter_collection.csv [--match SUBSTRING]
+
+``stats`` prints calls and average/min/max duration per kernel; ``pmc`` prints, per kernel and
+counter and grid size, the number of dispatches and the mean, min and max counter value per dispatch
+(FETCH_SIZE and WRITE_SIZE are in KiB).
+"""
+from __future__ import annotations
+
+import argparse
+import csv
+import re
+import sys
+from collections import defaultdict
+
+
+def short_name(name: str, width: int = 96) -> str:
+    name = name.replace("tg::(anonymous namespace)::", "").replace("tg::", "")
+    name = re.sub(r"\(.*\)$", "", name)  # drop the parameter list
+    return name if len(name) <= width else name[: width - 3] + "..."
+
+
+def stats(path: str, top: int) -> None:
+    rows = list(csv.DictReader(open(path)))
+    rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
+    print(f"{'kernel':<72} {'calls':>6} {'avg us':>9} {'min us':>9} {'max us':>9} {'%':>6}")
+    for r in rows[:top]:
+        print(f"{short_name(r['Name'], 72):<72} {int(r['Calls']):>6} "
+              f"{float(r['AverageNs']) / 1e3:>9.1f} {float(r['MinNs']) / 1e3:>9.1f} "
+              f"{float(r['MaxNs']) / 1e3:>9.1f} {float(r['Percentage']):>6.2f}")
+
+
+def pmc(path: str, match: str | None) -> None:
+    acc = defaultdict(list)  # (kernel, counter, grid size in work-items) -> values
+    for r in csv.DictReader(open(path)):
+        if match and match not in r["Kernel_Name"]:
+            continue
+        key = (short_name(r["Kernel_Name"], 72), r["Counter_Name"], r["Grid_Size"])
+        acc[key].append(float(r["Counter_Value"]))
+    print(f"{'kernel':<72} {'counter':<11} {'n':>5} {'grid':>9} {'mean':>14} {'min':>14} "
+          f"{'max':>14}")
+    for (name, counter, grid), v in sorted(acc.items(), key=lambda kv: -max(kv[1])):
+        print(f"{name:<72} {counter:<11} {len(v):>5} {grid:>9} "
+              f"{sum(v) / len(v):>14.1f} {min(v):>14.1f} {max(v):>14.1f}")
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    a = sub.add_parser("stats")
+    a.add_argument("csv")
+    a.add_argument("--top", type=int, default=12)
+    b = sub.add_parser("pmc")
+    b.add_argument("csv")
+    b.add_argument("--match")
+    args = ap.parse_args(argv)
+    if args.cmd == "stats":
+        stats(args.csv, args.top)
+    else:
+        pmc(args.csv, args.match)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -29,7 +29,8 @@ def test_plan_matches_per_block_and_oracle(cuda, hybrid_small, dtype):
     g = api.geometry_from_host(h, dtype=tdt)
     rule = api.QuadRule.for_grid(g, 4)
     plan = api.MeshPlan(g, rule, FACTORS)
-    assert plan.persistent and plan.grid % 8 == 0
+    # One workgroup per item at most; the small mesh has fewer items than resident workgroups.
+    assert plan.persistent and 1 <= plan.grid <= plan.num_items
     plan.launch()
     torch.cuda.synchronize()
     tol = 1e-6 if dtype == "f32" else 1e-12
