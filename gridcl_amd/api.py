@@ -54,6 +54,7 @@ OPT_FORCE_GENERIC = 4
 OPT_MAP_CELL = 8
 OPT_MAP_CELL_STAGED = 16
 OPT_MAP_TILE = 32
+OPT_NO_SUMFACT = 64
 
 
 # ---- Errors: one class per tetraloma_device::GridError variant of the path -------------------

@@ -20,6 +20,7 @@ ARCH = "gfx950"
 HIP_SOURCES = [
     ("factor_kernels.hip", ["-DTG_FACTOR_TYPE=float", "-DTG_FACTOR_COMMON"]),
     ("factor_kernels.hip", ["-DTG_FACTOR_TYPE=double"]),
+    ("sumfact_kernels.hip", []),
     ("adjoint_kernels.hip", []),
 ]
 HOST_HIP_SOURCES = ["capi.cpp", "adjoint.cpp", "plan.cpp", "tuning.cpp"]  # host code that calls the HIP runtime

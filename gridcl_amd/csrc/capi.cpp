@@ -2,6 +2,7 @@
 // with the reference's error variants, cached basis tables, and kernel dispatch.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -40,6 +41,8 @@ DeviceTables::~DeviceTables() {
   if (tab) (void)hipFree(tab);
   if (tab64) (void)hipFree(tab64);
   if (ref_normals) (void)hipFree(ref_normals);
+  if (sf_tab) (void)hipFree(sf_tab);
+  if (sf_lex) (void)hipFree(sf_lex);
 }
 
 }  // namespace tg
@@ -95,6 +98,69 @@ int upload(const std::vector<double>& src, void** dst) {
   return TG_OK;
 }
 
+// Sum-factorisation tables of a hexahedron of `degree` at the points of p, when the point set is
+// an m x m x m tensor product of one 1D set u (point i = (u[i % m], u[i / m % m], u[i / m^2]))
+// and the element's nodes lie on an n1^3 lattice: tab1d [2][m][n1] (1D Lagrange values, then
+// derivatives, on the sorted 1D node positions) and lex [k]. Returns false when not applicable.
+bool sumfact_tables(const tg_pointset* p, unsigned degree, int family, std::vector<double>& tab1d,
+                    std::vector<uint32_t>& lex, int* n1_out, int* m_out) {
+  if (p->cell != tg::kHexahedron || p->tdim != 3) return false;
+  std::vector<double> nodes;
+  if (!tg::lagrange_points(tg::kHexahedron, degree, static_cast<tg::Family>(family), nodes))
+    return false;
+  const size_t k = nodes.size() / 3;
+  std::vector<double> t1;
+  for (size_t i = 0; i < k; ++i) t1.push_back(nodes[i * 3]);
+  std::sort(t1.begin(), t1.end());
+  t1.erase(std::unique(t1.begin(), t1.end(), [](double a, double b) { return std::fabs(a - b) < 1e-12; }),
+           t1.end());
+  const int n1 = static_cast<int>(t1.size());
+  if (n1 < 2 || n1 > 4 || static_cast<size_t>(n1 * n1 * n1) != k) return false;
+  auto index_of = [&](double v) {
+    for (int i = 0; i < n1; ++i)
+      if (std::fabs(t1[i] - v) < 1e-12) return i;
+    return -1;
+  };
+  lex.assign(k, 0);
+  std::vector<char> seen(k, 0);
+  for (size_t i = 0; i < k; ++i) {
+    const int a = index_of(nodes[i * 3]), b = index_of(nodes[i * 3 + 1]), c = index_of(nodes[i * 3 + 2]);
+    if (a < 0 || b < 0 || c < 0) return false;
+    const uint32_t l = static_cast<uint32_t>((a * n1 + b) * n1 + c);
+    if (seen[l]) return false;
+    seen[l] = 1;
+    lex[i] = l;
+  }
+  const size_t nq = p->nq;
+  int m = 0;
+  while (static_cast<size_t>(m * m * m) < nq) ++m;
+  if (m < 1 || m > 4 || static_cast<size_t>(m * m * m) != nq) return false;
+  std::vector<double> u(m);
+  for (int i = 0; i < m; ++i) u[i] = p->points[i * 3];
+  for (size_t i = 0; i < nq; ++i) {
+    const double want[3] = {u[i % m], u[(i / m) % m], u[i / (m * m)]};
+    for (int d = 0; d < 3; ++d)
+      if (std::fabs(p->points[i * 3 + d] - want[d]) > 1e-14) return false;
+  }
+  // 1D Lagrange polynomial n on the nodes t1 and its derivative, in the product form.
+  tab1d.assign(2 * m * n1, 0.0);
+  for (int q = 0; q < m; ++q)
+    for (int n = 0; n < n1; ++n) {
+      double v = 1.0, dv = 0.0;
+      for (int j = 0; j < n1; ++j) {
+        if (j == n) continue;
+        const double f = (u[q] - t1[j]) / (t1[n] - t1[j]);
+        dv = dv * f + v / (t1[n] - t1[j]);
+        v *= f;
+      }
+      tab1d[q * n1 + n] = v;
+      tab1d[(m + q) * n1 + n] = dv;
+    }
+  *n1_out = n1;
+  *m_out = m;
+  return true;
+}
+
 }  // namespace
 
 namespace tg {
@@ -121,6 +187,19 @@ int device_tables(tg_pointset* p, unsigned degree, int family, int dtype, Device
     rc = dtype == TG_F32 ? upload<float>(p->normals, &t->ref_normals)
                          : upload<double>(p->normals, &t->ref_normals);
     if (rc) return rc;
+  }
+  std::vector<double> tab1d;
+  std::vector<uint32_t> lex;
+  int n1 = 0, m = 0;
+  if (sumfact_tables(p, degree, family, tab1d, lex, &n1, &m)) {
+    rc = dtype == TG_F32 ? upload<float>(tab1d, &t->sf_tab) : upload<double>(tab1d, &t->sf_tab);
+    if (rc) return rc;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&t->sf_lex), lex.size() * sizeof(uint32_t));
+    if (e == hipSuccess)
+      e = hipMemcpy(t->sf_lex, lex.data(), lex.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "sum-factorisation tables");
+    t->sf_n1 = n1;
+    t->sf_m = m;
   }
   *out = t.get();
   p->cache.emplace(key, std::move(t));
@@ -190,6 +269,15 @@ int eval_typed(const tg_geometry* g, const Block& b, tg_pointset* pts, DeviceTab
   tg::FactorOutputs o{out->points, out->jacobians, out->detj, out->pinv, out->facet_normals,
                       out->normals};
   const bool shift = f32 && !(options & TG_OPT_NO_CELL_SHIFT);
+  const uint32_t sf_factors = TG_POINTS | TG_JACOBIAN | TG_DETJ;
+  if (tabs->sf_m > 0 && tile == 16 && g->gdim == 3 && !(factors & ~sf_factors) &&
+      !(options & (TG_OPT_FORCE_GENERIC | TG_OPT_NO_SUMFACT))) {
+    hipError_t e = tg::launch_sumfact<T>(
+        tabs->sf_n1, tabs->sf_m, shift, static_cast<const T*>(g->nodes), b.dofmap,
+        static_cast<const T*>(tabs->sf_tab), tabs->sf_lex, tabs->tab64, p, o, stream);
+    if (e == hipSuccess) return TG_OK;
+    if (e != hipErrorNotSupported) return hip_fail(e, "sum-factorised kernel launch");
+  }
   hipError_t e = tg::launch_factor<T>(
       b.tdim, g->gdim, shift, (options & TG_OPT_FORCE_GENERIC) != 0,
       static_cast<const T*>(g->nodes), b.dofmap, b.flags, static_cast<const T*>(tabs->tab),

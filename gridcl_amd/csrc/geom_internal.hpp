@@ -109,6 +109,17 @@ hipError_t launch_factor(int tdim, int gdim, bool shift, bool generic, const T* 
                          const double* tab64, const T* ref_normals, const FactorParams<T>& p,
                          const FactorOutputs& out, hipStream_t stream);
 
+// Sum-factorised hexahedra (tiles of 16; X, J, det J only): degree n1 - 1 in each direction
+// (n1 = 2..4 nodes per direction) at an m x m x m tensor point set (m = 1..4, first coordinate
+// fastest). tab1d is [2][m][n1] (1D basis values, then derivatives, at the 1D points); lex[k]
+// is the lexicographic position (a n1 + b) n1 + c of A.5 node k, (a, b, c) indexing its
+// (xi, eta, zeta) node coordinates; tab64 is the full f64 table [q][k][4] for the FP64 fix-up.
+// Returns hipErrorNotSupported for other (n1, m).
+template <typename T>
+hipError_t launch_sumfact(int n1, int m, bool shift, const T* nodes, const uint32_t* dofmap,
+                          const T* tab1d, const uint32_t* lex, const double* tab64,
+                          const FactorParams<T>& p, const FactorOutputs& out, hipStream_t stream);
+
 template <typename T>
 hipError_t launch_evector(const T* nodes, const uint32_t* dofmap, uint64_t num_cells,
                           uint64_t padded, uint32_t k, int gdim, uint32_t last_node, T* out,

@@ -27,10 +27,18 @@ struct Block {
 // Device copies of one element's tables at one point set: interleaved [q][k][4] (value, then
 // the tdim derivatives, zero-padded) in the geometry's type, the same in f64, and the reference
 // facet normals [q][tdim] in the geometry's type.
+//
+// Hexahedra at a tensor-product point set also get the sum-factorisation tables: the 1D basis
+// values and derivatives at the 1D points [2][m][n1] in the geometry's type and the
+// lexicographic position of each A.5 node [k] (sf_m = 0 when the point set is not a tensor
+// product of one 1D set with the first coordinate fastest, or m > 4, or n1 > 4).
 struct DeviceTables {
   void* tab = nullptr;
   double* tab64 = nullptr;
   void* ref_normals = nullptr;
+  void* sf_tab = nullptr;
+  uint32_t* sf_lex = nullptr;
+  int sf_n1 = 0, sf_m = 0;
   ~DeviceTables();
 };
 

@@ -89,7 +89,11 @@ enum {
    * tile's rows. The last two need tiles of 16 (other tiles always run the first). */
   TG_OPT_MAP_CELL = 1u << 3,
   TG_OPT_MAP_CELL_STAGED = 1u << 4,
-  TG_OPT_MAP_TILE = 1u << 5
+  TG_OPT_MAP_TILE = 1u << 5,
+  /* Hexahedra at tensor-product point sets with tiles of 16 run the sum-factorised kernel (the
+   * reference's `sumfact` strategy, tetraloma-geometry/src/sumfact/tensor.rs:337) for X, J and
+   * det J; this option keeps them on the dense kernels. */
+  TG_OPT_NO_SUMFACT = 1u << 6
 };
 
 typedef struct tg_geometry tg_geometry;

@@ -40,3 +40,15 @@ def pointwise_rel(got, want):
     got = np.asarray(got).reshape(-1)
     want = np.asarray(want).reshape(-1)
     return float(np.max(np.abs(got - want) / np.maximum(np.abs(want), 1e-300)))
+
+
+def oracle_factors_sampled(g: api.Geometry, mesh: M.HostMesh, b: int, ps: api.PointSet, factors,
+                           cells: np.ndarray, threads=0):
+    """The oracle on the listed cells of block b only (cell-major [len(cells)][q][LEN])."""
+    blk = mesh.blocks[b]
+    nodes = api.stored_nodes_f64(g)
+    rows = np.ascontiguousarray(blk.cells[cells]).reshape(-1)
+    flags = None if blk.flags is None else np.ascontiguousarray(blk.flags[cells])
+    return O.eval_block(blk.cell, blk.degree, blk.family, mesh.gdim, nodes, rows, ps.points,
+                        factors=factors, flags=flags, ref_normals=ps.facet_normals,
+                        threads=threads)

@@ -389,3 +389,47 @@ def test_mesh_level_loops(cuda):
         assert np.abs(f.to_host_cell_major() - want).max() <= 1e-6 * np.abs(want).max()
     with pytest.raises(api.CellNotInRule):
         mesh.detj(api.QuadRule.gauss([TET], 2))
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+@pytest.mark.parametrize("m", [1, 2, 3, 4])
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_sumfact_hexahedra(cuda, p, m, dtype):
+    """Hexahedra at m^3 Gauss points with tiles of 16 run the sum-factorised kernel for X, J and
+    det J: against the oracle, and against the dense kernels (TG_OPT_NO_SUMFACT). 27 cells: the
+    second tile is padded. Tolerances: FP64 1e-12; FP32 1e-6 (X, J, det J)."""
+    import torch
+
+    mesh = volume_mesh(HEX, p, n=3)
+    tdt = torch.float64 if dtype == "f64" else torch.float32
+    g = api.geometry_from_host(mesh, dtype=tdt)
+    ps = api.PointSet.gauss(HEX, 2 * m - 1)
+    assert len(ps) == m ** 3
+    names = ["points", "jacobians", "detj"]
+    got = gpu_factors(g, 0, ps, names)
+    dense = gpu_factors(g, 0, ps, names, options=api.OPT_NO_SUMFACT)
+    want = oracle_factors(g, mesh, 0, ps, names)
+    err = rel_errs(got, want)
+    err_dense = rel_errs(dense, want)
+    print(f"hex p={p} m={m} {dtype} sumfact {err} dense {err_dense}")
+    tol = 1e-12 if dtype == "f64" else 1e-6
+    for name, e in err.items():
+        assert e <= tol, (name, e)
+    # The padding of the last tile holds the last cell (field.rs:234-253).
+    fields = api.geometric_factors_block(g, 0, ps, names)
+    torch.cuda.synchronize()
+    raw = fields["detj"].data.cpu().numpy().astype(np.float64).reshape(2, m ** 3, 1, 16)
+    assert np.array_equal(raw[1, :, 0, 11:], np.repeat(raw[1, :, 0, 10:11], 5, axis=1))
+    # Not a tensor-product point set: the dense kernel, same results to rounding.
+    pts = ps.points.copy()
+    pts[[0, -1]] = pts[[-1, 0]]
+    perm_ps = api.PointSet.custom(HEX, pts, ps.weights.copy())
+    got_p = gpu_factors(g, 0, perm_ps, names)
+    for name in names:
+        a = got_p[name][:, [-1] + list(range(1, m ** 3 - 1)) + [0]] if m > 1 else got_p[name]
+        assert O_rel(a, got[name]) <= (1e-12 if dtype == "f64" else 4e-6), name
+
+
+def O_rel(a, b):
+    """rel_err of a against b."""
+    return float(np.abs(a - b).max() / np.abs(b).max())
