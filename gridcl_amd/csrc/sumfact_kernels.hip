@@ -11,10 +11,11 @@
 //   X, J_xi, J_eta, J_zeta at (p, q, r) = sum_a phi^(t)_a(u_p) S(a, q)_w
 // which is 14-fold less work than the dense n1^3 x m^3 contraction for Q3 at 4^3 points.
 //
-// One wavefront per tile of 16 cells. The wavefront gathers its cells' nodes through the dofmap
-// (coalesced index loads, indices clamped to the last node) into LDS as [g][lex node][16 cells],
-// so that the reads of the sums are conflict-free and the lanes of one cell share them. Lanes are
-// 16 cells x 4 values of r; each lane produces the m^2 points (p, q) of its r, one physical
+// One workgroup of two wavefronts per tile of 16 cells. They gather the tile's nodes through
+// the dofmap (coalesced index loads, indices clamped to the last node) into LDS as
+// [g][lex node][16 cells], so that the reads of the sums are conflict-free and the lanes of one
+// cell share them. Lanes are 16 cells x 4 values of r, and each wavefront takes half of the
+// values of q; each lane produces its points (p, q) of its r, one physical
 // coordinate g at a time, and stores X_g and row g of J from registers. det J is built across
 // the three rows (row0 x row1, then row2 . that), so no lane holds the whole 3 x 3 J of all its
 // points at once. FP32: the coordinates are shifted to the mean of the cell's eight vertices,
@@ -27,7 +28,7 @@
 namespace tg {
 namespace {
 
-constexpr int kBlock = 64;  // one wavefront (one tile) per workgroup: LDS is per wavefront
+constexpr int kBlock = 128;  // two wavefronts per tile of 16 cells
 
 __device__ __forceinline__ float tfma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double tfma(double a, double b, double c) { return __builtin_fma(a, b, c); }
@@ -73,8 +74,10 @@ __device__ __noinline__ double detj_fp64_staged(const T* __restrict__ xs,
          j[6] * (j[1] * j[5] - j[4] * j[2]);
 }
 
-// N1 nodes and M points per direction; PC points of the p direction per pass (PC < M trades
-// recomputation of the first two sums for registers, for FP64).
+// N1 nodes and M points per direction. The two wavefronts of a workgroup share one tile: they
+// gather it together, then wavefront h takes the points q in [h QH, (h + 1) QH) of the eta
+// direction (QH = ceil(M / 2)), so that each lane keeps the sums of half the points. PC points of
+// the p direction per pass (PC < M recomputes the first two sums to save registers, for FP64).
 template <typename T, int N1, int M, int PC, bool SHIFT>
 __global__ void __launch_bounds__(kBlock)
     factor_sumfact(const T* __restrict__ nodes, const uint32_t* __restrict__ dofmap,
@@ -82,12 +85,13 @@ __global__ void __launch_bounds__(kBlock)
                    const double* __restrict__ tab64, FactorParams<T> p, FactorOutputs o) {
   constexpr int K = N1 * N1 * N1;
   constexpr int NQ = M * M * M;
+  constexpr int QH = (M + 1) / 2;
   __shared__ T xs[3 * K * 16];
-  const uint32_t w = threadIdx.x, cell = w & 15u, r = w >> 4;
+  const uint32_t t = threadIdx.x, cell = t & 15u, r = (t >> 4) & 3u, h = t >> 6;
   const uint64_t tile = blockIdx.x;
 
   // Gather: pair j = (cell, node) of the tile; the tile's dofmap rows are contiguous.
-  for (uint32_t j = w; j < 16u * K; j += 64u) {
+  for (uint32_t j = t; j < 16u * K; j += kBlock) {
     const uint32_t cc = j / K, node = j - cc * K;
     uint64_t c = tile * 16 + cc;
     c = c < p.num_cells ? c : p.num_cells - 1;
@@ -97,8 +101,9 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
     for (int g = 0; g < 3; ++g) xs[(g * K + l) * 16 + cc] = nodes[static_cast<uint64_t>(n) * 3 + g];
   }
-  wave_sync();
-  if (r >= static_cast<uint32_t>(M)) return;  // no LDS use follows: idle lanes may leave
+  __syncthreads();
+  const int q0 = static_cast<int>(h) * QH;
+  if (r >= static_cast<uint32_t>(M) || q0 >= M) return;  // no barrier follows
 
   T s[3];
 #pragma unroll
@@ -123,67 +128,89 @@ __global__ void __launch_bounds__(kBlock)
     dr[c] = tab1d[(M + r) * N1 + c];
   }
   const uint32_t mask = p.mask;
+  // The lane's first point (p, q, r) = (0, q0, r) and its field bases (tile layout, tile 16).
+  const uint32_t first_point = static_cast<uint32_t>(M * q0) + M * M * r;
+  const uint64_t row0 = tile * NQ + first_point;
+  T* __restrict__ xbase = static_cast<T*>(o.x) + ((row0 * 3) << 4) + cell;
+  T* __restrict__ jbase = static_cast<T*>(o.j) + ((row0 * 9) << 4) + cell;
+  T* __restrict__ dbase = static_cast<T*>(o.detj) + (row0 << 4) + cell;
 
 #pragma unroll
   for (int p0 = 0; p0 < M; p0 += PC) {
-    T carry[M][PC][3], n2[M][PC];
-#pragma unroll
+    T carry[QH][PC][3], n2[QH][PC];
+    // g and a stay rolled: unrolled, the compiler hoists the LDS reads of later iterations and
+    // the registers they hold cost occupancy. Sums start from zero (fma(x, y, 0) = x y).
+#pragma unroll 1
     for (int g = 0; g < 3; ++g) {
-      T acc[M][PC][4];
+      const T sg = g == 0 ? s[0] : (g == 1 ? s[1] : s[2]);
+      const T* __restrict__ xg = xs + g * K * 16 + cell;
+      T acc[QH][PC][4];
 #pragma unroll
+      for (int qq = 0; qq < QH; ++qq)
+#pragma unroll
+        for (int pp = 0; pp < PC; ++pp)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) acc[qq][pp][i] = T(0);
+#pragma unroll 1
       for (int a = 0; a < N1; ++a) {
-        T s2[M][3];
+        T s2[QH][3];
+#pragma unroll
+        for (int qq = 0; qq < QH; ++qq) s2[qq][0] = s2[qq][1] = s2[qq][2] = T(0);
 #pragma unroll
         for (int b = 0; b < N1; ++b) {
-          T s0, s1;
+          T s0 = T(0), s1 = T(0);
 #pragma unroll
           for (int c = 0; c < N1; ++c) {
-            const T x = xs[(g * K + (a * N1 + b) * N1 + c) * 16 + cell] - s[g];
-            s0 = c == 0 ? x * br[0] : tfma(x, br[c], s0);
-            s1 = c == 0 ? x * dr[0] : tfma(x, dr[c], s1);
+            const T x = xg[((a * N1 + b) * N1 + c) * 16] - sg;
+            s0 = tfma(x, br[c], s0);
+            s1 = tfma(x, dr[c], s1);
           }
 #pragma unroll
-          for (int q = 0; q < M; ++q) {
+          for (int qq = 0; qq < QH; ++qq) {
+            const int q = q0 + qq < M ? q0 + qq : M - 1;
             const T bq = tab1d[q * N1 + b], dq = tab1d[(M + q) * N1 + b];
-            s2[q][0] = b == 0 ? s0 * bq : tfma(s0, bq, s2[q][0]);
-            s2[q][1] = b == 0 ? s0 * dq : tfma(s0, dq, s2[q][1]);
-            s2[q][2] = b == 0 ? s1 * bq : tfma(s1, bq, s2[q][2]);
+            s2[qq][0] = tfma(s0, bq, s2[qq][0]);
+            s2[qq][1] = tfma(s0, dq, s2[qq][1]);
+            s2[qq][2] = tfma(s1, bq, s2[qq][2]);
           }
         }
 #pragma unroll
-        for (int q = 0; q < M; ++q)
+        for (int qq = 0; qq < QH; ++qq)
 #pragma unroll
           for (int pp = 0; pp < PC; ++pp) {
             if (p0 + pp >= M) continue;
             const T ba = tab1d[(p0 + pp) * N1 + a], da = tab1d[(M + p0 + pp) * N1 + a];
-            T* A = acc[q][pp];
-            A[0] = a == 0 ? s2[q][0] * ba : tfma(s2[q][0], ba, A[0]);
-            A[1] = a == 0 ? s2[q][0] * da : tfma(s2[q][0], da, A[1]);
-            A[2] = a == 0 ? s2[q][1] * ba : tfma(s2[q][1], ba, A[2]);
-            A[3] = a == 0 ? s2[q][2] * ba : tfma(s2[q][2], ba, A[3]);
+            T* A = acc[qq][pp];
+            A[0] = tfma(s2[qq][0], ba, A[0]);
+            A[1] = tfma(s2[qq][0], da, A[1]);
+            A[2] = tfma(s2[qq][1], ba, A[2]);
+            A[3] = tfma(s2[qq][2], ba, A[3]);
           }
       }
-      // Row g of J and X_g of the lane's points; the det J carry.
+      // Row g of J and X_g of the lane's points; the det J carry. Addresses: one per-lane base
+      // per field and compile-time offsets per point.
+      const T og = g == 0 ? p.origin[0] : (g == 1 ? p.origin[1] : p.origin[2]);
+      T* __restrict__ xb = xbase + g * 16;
+      T* __restrict__ jb = jbase + g * 16;
 #pragma unroll
-      for (int q = 0; q < M; ++q)
+      for (int qq = 0; qq < QH; ++qq)
 #pragma unroll
         for (int pp = 0; pp < PC; ++pp) {
-          if (p0 + pp >= M) continue;
-          const uint32_t qi = static_cast<uint32_t>(p0 + pp) + M * q + M * M * r;
-          const uint64_t row = tile * NQ + qi;
-          const T* A = acc[q][pp];
-          if (mask & kFPoints) st<T>(o.x, ((row * 3 + g) << 4) | cell, (A[0] + s[g]) + p.origin[g]);
+          if (p0 + pp >= M || q0 + qq >= M) continue;
+          const int pt = p0 + pp + M * qq;  // point offset from the lane's first point
+          const T* A = acc[qq][pp];
+          if (mask & kFPoints) st<T>(xb, pt * 3 * 16, (A[0] + sg) + og);
           if (mask & kFJac) {
 #pragma unroll
-            for (int d = 0; d < 3; ++d) st<T>(o.j, ((row * 9 + d * 3 + g) << 4) | cell, A[1 + d]);
+            for (int d = 0; d < 3; ++d) st<T>(jb, (pt * 9 + d * 3) * 16, A[1 + d]);
           }
-          T* cr = carry[q][pp];
+          T* cr = carry[qq][pp];
           const T nrm = A[1] * A[1] + A[2] * A[2] + A[3] * A[3];
           if (g == 0) {
             cr[0] = A[1];
             cr[1] = A[2];
             cr[2] = A[3];
-            n2[q][pp] = nrm;
+            n2[qq][pp] = nrm;
           } else if (g == 1) {  // row0 x row1
             const T c0 = cr[1] * A[3] - cr[2] * A[2];
             const T c1 = cr[2] * A[1] - cr[0] * A[3];
@@ -191,27 +218,28 @@ __global__ void __launch_bounds__(kBlock)
             cr[0] = c0;
             cr[1] = c1;
             cr[2] = c2;
-            n2[q][pp] += nrm;
+            n2[qq][pp] += nrm;
           } else {  // det J = row2 . (row0 x row1)
             cr[0] = A[1] * cr[0] + A[2] * cr[1] + A[3] * cr[2];
-            n2[q][pp] += nrm;
+            n2[qq][pp] += nrm;
           }
         }
     }
     if (mask & kFDetJ) {
 #pragma unroll
-      for (int q = 0; q < M; ++q)
+      for (int qq = 0; qq < QH; ++qq)
 #pragma unroll
         for (int pp = 0; pp < PC; ++pp) {
-          if (p0 + pp >= M) continue;
-          const uint32_t qi = static_cast<uint32_t>(p0 + pp) + M * q + M * M * r;
-          T det = carry[q][pp][0];
+          if (p0 + pp >= M || q0 + qq >= M) continue;
+          const int pt = p0 + pp + M * qq;
+          T det = carry[qq][pp][0];
           if constexpr (sizeof(T) == 4) {
-            const T nn = n2[q][pp];
+            const T nn = n2[qq][pp];
             if (p.fixup && tabs(det) < static_cast<T>(p.tau) * nn * tsqrt(nn))
-              det = static_cast<T>(detj_fp64_staged<T, K>(xs, lex, cell, tab64, qi));
+              det = static_cast<T>(
+                  detj_fp64_staged<T, K>(xs, lex, cell, tab64, first_point + static_cast<uint32_t>(pt)));
           }
-          st<T>(o.detj, ((tile * NQ + qi) << 4) | cell, det);
+          st<T>(dbase, pt * 16, det);
         }
     }
   }
@@ -221,10 +249,10 @@ template <typename T>
 using SumfactFn = void (*)(const T*, const uint32_t*, const T*, const uint32_t*, const double*,
                            FactorParams<T>, FactorOutputs);
 
-// FP32 keeps a whole r-slab of points in registers; FP64 takes the p direction two at a time.
+// FP32 keeps its points of an r-slab in registers; FP64 takes the p direction two at a time.
 template <typename T, int N1, int M, bool SHIFT>
 constexpr SumfactFn<T> sumfact_fn() {
-  constexpr int PC = sizeof(T) == 4 ? M : (M < 2 ? M : 2);
+  constexpr int PC = sizeof(T) == 4 ? M : (M < 2 ? M : 2);  // FP64: p two at a time
   return factor_sumfact<T, N1, M, PC, SHIFT>;
 }
 
