@@ -55,6 +55,7 @@ OPT_MAP_CELL = 8
 OPT_MAP_CELL_STAGED = 16
 OPT_MAP_TILE = 32
 OPT_NO_SUMFACT = 64
+OPT_PLAN_PER_BLOCK = 128
 
 
 # ---- Errors: one class per tetraloma_device::GridError variant of the path -------------------

@@ -247,10 +247,14 @@ uint32_t log2u(uint32_t v) {
   return r;
 }
 
+}  // namespace
+
+namespace tg {
+
 template <typename T>
-int eval_typed(const tg_geometry* g, const Block& b, tg_pointset* pts, DeviceTables* tabs,
-               uint32_t factors, uint32_t tile, uint32_t options, const tg_outputs* out,
-               hipStream_t stream) {
+int enqueue_block(const tg_geometry* g, const Block& b, tg_pointset* pts, DeviceTables* tabs,
+                  uint32_t factors, uint32_t tile, uint32_t options, const tg_outputs* out,
+                  hipStream_t stream) {
   tg::FactorParams<T> p{};
   p.num_cells = b.num_cells;
   p.padded_cells = (b.num_cells + tile - 1) / tile * tile;
@@ -288,7 +292,12 @@ int eval_typed(const tg_geometry* g, const Block& b, tg_pointset* pts, DeviceTab
   return TG_OK;
 }
 
-}  // namespace
+template int enqueue_block<float>(const tg_geometry*, const Block&, tg_pointset*, DeviceTables*,
+                                  uint32_t, uint32_t, uint32_t, const tg_outputs*, hipStream_t);
+template int enqueue_block<double>(const tg_geometry*, const Block&, tg_pointset*, DeviceTables*,
+                                   uint32_t, uint32_t, uint32_t, const tg_outputs*, hipStream_t);
+
+}  // namespace tg
 
 extern "C" {
 
@@ -517,8 +526,9 @@ int tg_eval_block(const tg_geometry* g, int32_t block, tg_pointset* pts, uint32_
   int rc = device_tables(pts, b.degree, b.family, g->dtype, &tabs);
   if (rc) return rc;
   hipStream_t s = static_cast<hipStream_t>(stream);
-  return g->dtype == TG_F32 ? eval_typed<float>(g, b, pts, tabs, factors, tile, options, out, s)
-                            : eval_typed<double>(g, b, pts, tabs, factors, tile, options, out, s);
+  return g->dtype == TG_F32
+             ? tg::enqueue_block<float>(g, b, pts, tabs, factors, tile, options, out, s)
+             : tg::enqueue_block<double>(g, b, pts, tabs, factors, tile, options, out, s);
 }
 
 uint64_t tg_evector_elems(const tg_geometry* g, int32_t block) {

@@ -553,13 +553,14 @@ __global__ void __launch_bounds__(kBlock)
   X(12, 2, 3, 4) X(13, 2, 3, 9) X(14, 2, 2, 3) X(15, 2, 2, 6) X(16, 2, 2, 10) X(17, 2, 2, 4) \
   X(18, 2, 2, 9)
 
-// One persistent launch over the work items of every block of a mesh: items, in block order,
+// One persistent launch over the work items of every block of a mesh (at least two waves per
+// SIMD: the kernel holds every shape's code, and unbounded it would take one): items, in block order,
 // are split on the host into one contiguous range of about equal work per workgroup, and the
 // ranges of the workgroups of one XCD are adjacent (workgroup b runs on XCD b % 8), so each XCD
 // walks one stretch of the mesh. An item is kBlock / 64 tiles of 16 cells (kMapTile) or kBlock
 // cells. No counters: the launch is deterministic and can be replayed from a graph as is.
 template <typename T, bool SHIFT>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock, 2)
     factor_hybrid(const HybridBlock<T>* __restrict__ blocks, const WorkItem* __restrict__ items,
                   const uint32_t* __restrict__ ranges) {
   extern __shared__ __align__(16) unsigned char tg_lds[];

@@ -75,4 +75,11 @@ namespace tg {
 int device_tables(tg_pointset* p, unsigned degree, int family, int dtype, DeviceTables** out);
 // The point weights on the device (f64), uploaded on first use.
 int device_weights(tg_pointset* p, const double** out);
+// Enqueues every factor in `factors` of block b at pts (the dispatch of tg_eval_block: the
+// sum-factorised kernel where it applies, else the register-resident or runtime-K kernel).
+// Returns a TG status.
+template <typename T>
+int enqueue_block(const tg_geometry* g, const Block& b, tg_pointset* pts, DeviceTables* tabs,
+                  uint32_t factors, uint32_t tile, uint32_t options, const struct tg_outputs* out,
+                  hipStream_t stream);
 }  // namespace tg

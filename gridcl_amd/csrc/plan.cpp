@@ -147,34 +147,29 @@ int upload_persistent(tg_plan* plan) {
 }
 
 template <typename T>
-hipError_t launch_blocks(tg_plan* plan, hipStream_t s) {
+int launch_blocks(tg_plan* plan, hipStream_t s) {
   const tg_geometry* g = plan->g;
   for (const PlanBlock& pb : plan->blocks) {
-    const tg::Block& b = g->blocks[pb.index];
-    tg::FactorParams<T> p = params<T>(g, b, pb.pts, pb.mask, plan->tile, plan->options);
-    hipError_t e = tg::launch_factor<T>(
-        b.tdim, g->gdim, plan->shift, (plan->options & TG_OPT_FORCE_GENERIC) != 0,
-        static_cast<const T*>(g->nodes), b.dofmap, b.flags, static_cast<const T*>(pb.tabs->tab),
-        pb.tabs->tab64, static_cast<const T*>(pb.tabs->ref_normals), p, outputs(pb.out), s);
-    if (e != hipSuccess) return e;
+    int rc = tg::enqueue_block<T>(g, g->blocks[pb.index], pb.pts, pb.tabs, pb.mask, plan->tile,
+                                  plan->options, &pb.out, s);
+    if (rc) return rc;
   }
-  return hipSuccess;
+  return TG_OK;
 }
 
-hipError_t plan_enqueue(tg_plan* plan, hipStream_t s) {
-  if (plan->num_items == 0 && plan->persistent) return hipSuccess;
-  if (plan->g->dtype == TG_F32) {
-    if (plan->persistent)
-      return tg::launch_hybrid<float>(plan->shift,
-                                      static_cast<const tg::HybridBlock<float>*>(plan->d_blocks),
+int plan_enqueue(tg_plan* plan, hipStream_t s) {
+  if (!plan->persistent)
+    return plan->g->dtype == TG_F32 ? launch_blocks<float>(plan, s) : launch_blocks<double>(plan, s);
+  if (plan->num_items == 0) return TG_OK;
+  hipError_t e =
+      plan->g->dtype == TG_F32
+          ? tg::launch_hybrid<float>(plan->shift,
+                                     static_cast<const tg::HybridBlock<float>*>(plan->d_blocks),
+                                     plan->d_items, plan->d_ranges, plan->grid, plan->lds, s)
+          : tg::launch_hybrid<double>(plan->shift,
+                                      static_cast<const tg::HybridBlock<double>*>(plan->d_blocks),
                                       plan->d_items, plan->d_ranges, plan->grid, plan->lds, s);
-    return launch_blocks<float>(plan, s);
-  }
-  if (plan->persistent)
-    return tg::launch_hybrid<double>(plan->shift,
-                                     static_cast<const tg::HybridBlock<double>*>(plan->d_blocks),
-                                     plan->d_items, plan->d_ranges, plan->grid, plan->lds, s);
-  return launch_blocks<double>(plan, s);
+  return e == hipSuccess ? TG_OK : tg::hip_fail(e, "persistent plan launch");
 }
 
 }  // namespace
@@ -220,7 +215,7 @@ int tg_plan_create(const tg_geometry* g, tg_pointset* const* pts, uint32_t facto
     if (tg::hybrid_shape(b.tdim, g->gdim, b.k) < 0) all_hybrid = false;
     plan->blocks.push_back(pb);
   }
-  plan->persistent = all_hybrid;
+  plan->persistent = all_hybrid && !(options & TG_OPT_PLAN_PER_BLOCK);
   if (plan->persistent) {
     int rc = g->dtype == TG_F32 ? upload_persistent<float>(plan.get())
                                 : upload_persistent<double>(plan.get());
@@ -245,8 +240,7 @@ int tg_plan_info(const tg_plan* plan, uint32_t* num_items, uint32_t* grid, int32
 
 int tg_plan_launch(tg_plan* plan, void* stream) {
   if (!plan) return tg::fail(TG_ERR_INVALID_ARGUMENT, "null plan");
-  hipError_t e = plan_enqueue(plan, static_cast<hipStream_t>(stream));
-  return e == hipSuccess ? TG_OK : tg::hip_fail(e, "plan launch");
+  return plan_enqueue(plan, static_cast<hipStream_t>(stream));
 }
 
 int tg_plan_graph_launch(tg_plan* plan, void* stream) {
@@ -256,11 +250,12 @@ int tg_plan_graph_launch(tg_plan* plan, void* stream) {
     // Capture on a private stream of default priority, once; replays launch on `stream`.
     e = hipStreamCreateWithFlags(&plan->capture, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamBeginCapture(plan->capture, hipStreamCaptureModeThreadLocal);
+    int rc = TG_OK;
     if (e == hipSuccess) {
-      hipError_t le = plan_enqueue(plan, plan->capture);
+      rc = plan_enqueue(plan, plan->capture);
       e = hipStreamEndCapture(plan->capture, &plan->graph);
-      if (le != hipSuccess) e = le;
     }
+    if (rc) return rc;
     if (e == hipSuccess) e = hipGraphInstantiate(&plan->exec, plan->graph, nullptr, nullptr, 0);
     if (e != hipSuccess) return tg::hip_fail(e, "plan graph capture");
   }

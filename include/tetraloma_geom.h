@@ -93,7 +93,10 @@ enum {
   /* Hexahedra at tensor-product point sets with tiles of 16 run the sum-factorised kernel (the
    * reference's `sumfact` strategy, tetraloma-geometry/src/sumfact/tensor.rs:337) for X, J and
    * det J; this option keeps them on the dense kernels. */
-  TG_OPT_NO_SUMFACT = 1u << 6
+  TG_OPT_NO_SUMFACT = 1u << 6,
+  /* Plans only: one launch per block (each with the kernel tg_eval_block would run) instead of
+   * the persistent kernel over the work list; tg_plan_graph_launch captures them in one graph. */
+  TG_OPT_PLAN_PER_BLOCK = 1u << 7
 };
 
 typedef struct tg_geometry tg_geometry;
@@ -202,9 +205,11 @@ uint64_t tg_evector_elems(const tg_geometry* g, int32_t block);
  * codimension 1, facet normals on blocks of codimension 0 with a facet point set; the others on
  * every block). Replaces the reference's per-block loop over launch_factor
  * (tetraloma/src/mesh.rs:249-257 per_block; tetraloma-geometry/src/layout.rs:488) with one
- * persistent kernel over a work list of 256-cell items of all blocks, split in eight ranges of
- * equal work, one per XCD. Blocks whose element has no register-resident kernel (more than 27
- * nodes per cell, or forced generic) make the plan fall back to one launch per block.
+ * persistent kernel over a work list of the items (one workgroup's cells) of all blocks, split on
+ * the host into one contiguous range of equal estimated work per resident workgroup (the ranges
+ * of one XCD adjacent). Blocks whose element has no register-resident kernel (more than 27
+ * nodes per cell, or forced generic), and TG_OPT_PLAN_PER_BLOCK, make the plan run one launch
+ * per block instead.
  * Outputs are laid out exactly as tg_eval_block's. */
 typedef struct tg_plan tg_plan;
 int tg_plan_create(const tg_geometry* g, tg_pointset* const* pts, uint32_t factors,

@@ -1,8 +1,9 @@
 """The mixed-mesh plan: one persistent launch over every block of the hybrid mesh, and its
 hipGraph replays, against the per-block launches and the oracle (needs the GPU).
 
-The persistent kernel runs the per-block kernels' arithmetic, so its fields are bitwise equal to
-tg_eval_block's; graph replays are checked against the eager result after each replay, with
+The persistent kernel runs the register-resident per-block kernels' arithmetic, so its fields are
+bitwise equal to tg_eval_block's with the sum-factorised hexahedron kernel switched off
+(TG_OPT_NO_SUMFACT); graph replays are checked against the eager result after each replay, with
 eager work and synchronisation in between."""
 import numpy as np
 import pytest
@@ -37,7 +38,7 @@ def test_plan_matches_per_block_and_oracle(cuda, hybrid_small, dtype):
     for b, blk in enumerate(h.blocks):
         names = list(plan.fields[b])
         ps = rule.get(blk.cell)
-        single = api.geometric_factors_block(g, b, ps, names)
+        single = api.geometric_factors_block(g, b, ps, names, options=api.OPT_NO_SUMFACT)
         torch.cuda.synchronize()
         for n in names:
             assert np.array_equal(plan.fields[b][n].to_host_cell_major(),
@@ -89,3 +90,28 @@ def test_plan_with_large_element_falls_back(cuda):
     torch.cuda.synchronize()
     assert np.array_equal(plan.fields[0]["detj"].to_host_cell_major(),
                           single["detj"].to_host_cell_major())
+
+
+def test_plan_per_block_option_matches_eval(cuda, hybrid_small):
+    """TG_OPT_PLAN_PER_BLOCK: one launch per block with tg_eval_block's kernels (the
+    sum-factorised one for the hexahedra), replayed from a graph; equal to per-block evaluation
+    bitwise."""
+    import torch
+
+    g = api.geometry_from_host(hybrid_small, dtype=torch.float32)
+    rule = api.QuadRule.for_grid(g, 4)
+    plan = api.MeshPlan(g, rule, FACTORS, options=api.OPT_PLAN_PER_BLOCK)
+    assert not plan.persistent
+    for _ in range(3):
+        for fb in plan.fields:
+            for f in fb.values():
+                f.data.zero_()
+        plan.graph_launch()
+        torch.cuda.synchronize()
+        for b, blk in enumerate(hybrid_small.blocks):
+            names = list(plan.fields[b])
+            single = api.geometric_factors_block(g, b, rule.get(blk.cell), names)
+            torch.cuda.synchronize()
+            for n in names:
+                assert np.array_equal(plan.fields[b][n].to_host_cell_major(),
+                                      single[n].to_host_cell_major()), (b, n)
