@@ -15,11 +15,11 @@ struct tg_functional {
   tg::DeviceTables* tabs = nullptr;
   const double* wts = nullptr;
   double* partials = nullptr;   // one FP64 per forward workgroup
-  double* contrib = nullptr;    // [k][cells][3] FP64
+  void* contrib = nullptr;      // [k][cells][3] in the geometry's type
   uint64_t* offsets = nullptr;  // [nodes + 1]
   uint32_t* entries = nullptr;  // [cells * k]: local node * cells + cell, grouped by node
   ~tg_functional() {
-    for (void* p : {static_cast<void*>(partials), static_cast<void*>(contrib),
+    for (void* p : {static_cast<void*>(partials), contrib,
                     static_cast<void*>(offsets), static_cast<void*>(entries)})
       if (p) (void)hipFree(p);
   }
@@ -93,7 +93,11 @@ int tg_functional_create(const tg_geometry* g, int32_t block, tg_pointset* pts,
   int rc = tg::device_tables(pts, b.degree, b.family, g->dtype, &f->tabs);
   if (!rc) rc = tg::device_weights(pts, &f->wts);
   if (!rc) rc = alloc(&f->partials, tg::functional_partials(b.num_cells), "hipMalloc(partials)");
-  if (!rc) rc = alloc(&f->contrib, b.num_cells * b.k * 3, "hipMalloc(contributions)");
+  if (!rc) {
+    const size_t elem = g->dtype == TG_F32 ? sizeof(float) : sizeof(double);
+    rc = alloc(reinterpret_cast<unsigned char**>(&f->contrib), b.num_cells * b.k * 3 * elem,
+               "hipMalloc(contributions)");
+  }
   if (!rc) rc = build_incidence(f.get());
   if (rc) return rc;
   *out = f.release();
@@ -141,13 +145,13 @@ int tg_functional_vjp(tg_functional* f, const void* nodes, double beta, const do
   } else if (f->g->dtype == TG_F32) {
     e = tg::launch_functional_vjp<float>(
         static_cast<const float*>(x), b.dofmap, b.k, static_cast<const float*>(f->tabs->tab),
-        f->tabs->tab64, f->wts, static_cast<uint32_t>(f->pts->nq), b.num_cells, last, beta, seed,
-        f->contrib, f->offsets, f->entries, nn, grad, s);
+        f->wts, static_cast<uint32_t>(f->pts->nq), b.num_cells, last, beta, seed,
+        static_cast<float*>(f->contrib), f->offsets, f->entries, nn, grad, s);
   } else {
     e = tg::launch_functional_vjp<double>(
         static_cast<const double*>(x), b.dofmap, b.k, static_cast<const double*>(f->tabs->tab),
-        f->tabs->tab64, f->wts, static_cast<uint32_t>(f->pts->nq), b.num_cells, last, beta, seed,
-        f->contrib, f->offsets, f->entries, nn, grad, s);
+        f->wts, static_cast<uint32_t>(f->pts->nq), b.num_cells, last, beta, seed,
+        static_cast<double*>(f->contrib), f->offsets, f->entries, nn, grad, s);
   }
   return e == hipSuccess ? TG_OK : tg::hip_fail(e, "functional vjp");
 }

@@ -20,12 +20,12 @@ hipError_t launch_functional_forward(const T* nodes, const uint32_t* dofmap, uin
                                      uint64_t num_cells, uint32_t last_node, double beta,
                                      double* partials, double* out, hipStream_t stream);
 
+// contrib: [k][cells][3] scratch of the geometry's type.
 template <typename T>
 hipError_t launch_functional_vjp(const T* nodes, const uint32_t* dofmap, uint32_t k,
-                                 const T* tab, const double* tab64, const double* wts,
-                                 uint32_t nq, uint64_t num_cells, uint32_t last_node, double beta,
-                                 const double* seed, double* contrib, const uint64_t* offsets,
-                                 const uint32_t* entries, uint64_t num_nodes, double* grad,
-                                 hipStream_t stream);
+                                 const T* tab, const double* wts, uint32_t nq, uint64_t num_cells,
+                                 uint32_t last_node, double beta, const double* seed, T* contrib,
+                                 const uint64_t* offsets, const uint32_t* entries,
+                                 uint64_t num_nodes, double* grad, hipStream_t stream);
 
 }  // namespace tg

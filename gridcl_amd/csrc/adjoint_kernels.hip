@@ -4,14 +4,14 @@
 // Jacobian arithmetic (tetraloma-geometry/src/layout.rs:138-213) and the reduction is
 // deterministic like measure.rs:17-102 (fixed-order per-block trees, then one ordered pass).
 //
-// Forward: one lane per cell; J in T (FP32: coordinates shifted to the cell's first node) at
-// each point, the integrand in T, accumulated per cell in FP64; a fixed tree per workgroup
-// writes one FP64 partial; a single-workgroup pass sums the partials in order.
-// Backward: J is recomputed (not stored), dF/dJ is formed from it in T, and the lane writes its
-// cell's contributions dF/dx(k, g) = sum_q sum_d dF/dJ(g, d) dphi_k/dxi_d, widened to FP64
-// (stored [k][cell][3], so that a wavefront's stores are contiguous); a node
-// pass then sums each node's contributions in CSR order (ascending cell, then local node), so
-// the gradient is bitwise reproducible without atomics.
+// Forward: two lanes per cell (the even and the odd points, in two wavefronts); J in T (FP32:
+// coordinates shifted to the cell's first node) at each point, the integrand in T, accumulated
+// per lane in FP64; a fixed tree per workgroup writes one FP64 partial; a single-workgroup pass
+// sums the partials in order. Backward: J is recomputed (not stored), dF/dJ is formed from it in
+// T, each lane sums its points' contributions dF/dx(k, g) = sum_q sum_d dF/dJ(g, d)
+// dphi_k/dxi_d, the two lanes of a cell are added (even + odd) and stored [k][cell][3] in T; a
+// node pass then sums each node's contributions in CSR order (ascending cell, then local node)
+// in FP64 and scales by the seed, so the gradient is bitwise reproducible without atomics.
 #include <hip/hip_runtime.h>
 
 #include "adjoint_internal.hpp"
@@ -85,6 +85,11 @@ __device__ __forceinline__ double block_sum(double v, double* lds) {
   return lds[0];
 }
 
+// Both kernels give each cell two lanes in two wavefronts of the workgroup: wavefronts 2i and
+// 2i + 1 hold the same 64 cells and take the even and the odd points, so the table addresses
+// stay uniform per wavefront (scalar loads) and twice as many wavefronts share the work.
+constexpr int kCells = kBlock / 2;  // cells per workgroup
+
 template <typename T, int K, bool SHIFT>
 __global__ void __launch_bounds__(kBlock)
     functional_forward(const T* __restrict__ nodes, const uint32_t* __restrict__ dofmap,
@@ -92,13 +97,15 @@ __global__ void __launch_bounds__(kBlock)
                        uint64_t num_cells, uint32_t last_node, double beta,
                        double* __restrict__ partials) {
   __shared__ double lds[kBlock];
-  const uint64_t c = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  // The wavefront index, made a scalar so that the table addresses are known to be uniform.
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), half = wave & 1u;
+  const uint64_t c = static_cast<uint64_t>(blockIdx.x) * kCells + (wave >> 1) * 64 + (threadIdx.x & 63u);
   double cell_sum = 0.0;
   if (c < num_cells) {
     T x[K][3];
     gather<T, K, SHIFT>(nodes, dofmap + c * K, last_node, x);
     const T b = static_cast<T>(beta);
-    for (uint32_t q = 0; q < nq; ++q) {
+    for (uint32_t q = half; q < nq; q += 2) {
       T j[9];
       jacobian<T, K, SHIFT>(x, tab + static_cast<uint64_t>(q) * K * 4, j);
       const T det = det3(j);
@@ -127,88 +134,120 @@ __global__ void __launch_bounds__(kBlock)
 template <typename T, int K, bool SHIFT>
 __global__ void __launch_bounds__(kBlock)
     functional_vjp_cells(const T* __restrict__ nodes, const uint32_t* __restrict__ dofmap,
-                         const T* __restrict__ tab, const double* __restrict__ tab64,
-                         const double* __restrict__ wts, uint32_t nq, uint64_t num_cells,
-                         uint32_t last_node, double beta, const double* __restrict__ seed,
-                         double* __restrict__ contrib) {
-  __shared__ double lds[kBlock / 64][64 * 3];
-  const uint64_t cl = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
-  const uint32_t w = threadIdx.x & 63u;
-  const uint64_t c0 = cl - w;  // the wavefront's first cell
-  if (c0 >= num_cells) return;  // whole wavefronts stay: they share the LDS transpose
+                         const T* __restrict__ tab, const double* __restrict__ wts, uint32_t nq,
+                         uint64_t num_cells, uint32_t last_node, double beta,
+                         T* __restrict__ contrib) {
+  __shared__ T lds[kBlock / 64][64 * 3];  // per wavefront: one local node of its 64 cells
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), half = wave & 1u,
+                 w = threadIdx.x & 63u;
+  const uint64_t c0 = static_cast<uint64_t>(blockIdx.x) * kCells + (wave >> 1) * 64;
+  const uint64_t cl = c0 + w;
   const uint64_t c = cl < num_cells ? cl : num_cells - 1;
-  const uint32_t valid = num_cells - c0 < 64 ? static_cast<uint32_t>(num_cells - c0) : 64u;
-  T x[K][3];
-  gather<T, K, SHIFT>(nodes, dofmap + c * K, last_node, x);
-  // Per-cell arithmetic in the geometry's type (J recomputed, dF/dJ, contributions summed over
-  // the points); the node reduction that follows is FP64.
+  const uint32_t valid =
+      c0 >= num_cells ? 0u : (num_cells - c0 < 64 ? static_cast<uint32_t>(num_cells - c0) : 64u);
   T ct[K][3];
 #pragma unroll
   for (int k = 0; k < K; ++k) ct[k][0] = ct[k][1] = ct[k][2] = T(0);
-  const T b = static_cast<T>(beta);
-  for (uint32_t q = 0; q < nq; ++q) {
-    const T* __restrict__ t = tab + static_cast<uint64_t>(q) * K * 4;
-    T j[9];
-    jacobian<T, K, SHIFT>(x, t, j);
-    // Cofactors cof(g, d) = d det / d J(g, d), stored like J at d * 3 + g.
-    T cof[9];
-    cof[0] = j[4] * j[8] - j[7] * j[5];
-    cof[3] = -(j[1] * j[8] - j[7] * j[2]);
-    cof[6] = j[1] * j[5] - j[4] * j[2];
-    cof[1] = -(j[3] * j[8] - j[6] * j[5]);
-    cof[4] = j[0] * j[8] - j[6] * j[2];
-    cof[7] = -(j[0] * j[5] - j[3] * j[2]);
-    cof[2] = j[3] * j[7] - j[6] * j[4];
-    cof[5] = -(j[0] * j[7] - j[6] * j[1]);
-    cof[8] = j[0] * j[4] - j[3] * j[1];
-    const T det = j[0] * cof[0] + j[3] * cof[3] + j[6] * cof[6];
-    T n2 = T(0);
+  if (valid > 0) {
+    T x[K][3];
+    gather<T, K, SHIFT>(nodes, dofmap + c * K, last_node, x);
+    // Per-cell arithmetic in the geometry's type (J recomputed, dF/dJ, contributions summed over
+    // the wavefront's points); the node reduction that follows is FP64.
+    const T b = static_cast<T>(beta);
+    for (uint32_t q = half; q < nq; q += 2) {
+      const T* __restrict__ t = tab + static_cast<uint64_t>(q) * K * 4;
+      T j[9];
+      jacobian<T, K, SHIFT>(x, t, j);
+      // Cofactors cof(g, d) = d det / d J(g, d), stored like J at d * 3 + g.
+      T cof[9];
+      cof[0] = j[4] * j[8] - j[7] * j[5];
+      cof[3] = -(j[1] * j[8] - j[7] * j[2]);
+      cof[6] = j[1] * j[5] - j[4] * j[2];
+      cof[1] = -(j[3] * j[8] - j[6] * j[5]);
+      cof[4] = j[0] * j[8] - j[6] * j[2];
+      cof[7] = -(j[0] * j[5] - j[3] * j[2]);
+      cof[2] = j[3] * j[7] - j[6] * j[4];
+      cof[5] = -(j[0] * j[7] - j[6] * j[1]);
+      cof[8] = j[0] * j[4] - j[3] * j[1];
+      const T det = j[0] * cof[0] + j[3] * cof[3] + j[6] * cof[6];
+      T n2 = T(0);
 #pragma unroll
-    for (int i = 0; i < 9; ++i) n2 = tfma(j[i], j[i], n2);
-    const T r = tcbrt(det);
-    const T inv23 = T(1) / (r * r);
-    const T w = static_cast<T>(wts[q]);
-    const T a = w * (T(1) - b * (T(2) / T(3)) * n2 * inv23 / det);
-    const T bb = w * b * T(2) * inv23;
-    T G[9];
+      for (int i = 0; i < 9; ++i) n2 = tfma(j[i], j[i], n2);
+      const T r = tcbrt(det);
+      const T inv23 = T(1) / (r * r);
+      const T wq = static_cast<T>(wts[q]);
+      const T a = wq * (T(1) - b * (T(2) / T(3)) * n2 * inv23 / det);
+      const T bb = wq * b * T(2) * inv23;
+      T G[9];
 #pragma unroll
-    for (int i = 0; i < 9; ++i) G[i] = a * cof[i] + bb * j[i];
+      for (int i = 0; i < 9; ++i) G[i] = a * cof[i] + bb * j[i];
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const T d0 = t[4 * k + 1], d1 = t[4 * k + 2], d2 = t[4 * k + 3];
+      for (int k = 0; k < K; ++k) {
+        const T d0 = t[4 * k + 1], d1 = t[4 * k + 2], d2 = t[4 * k + 3];
 #pragma unroll
-      for (int g = 0; g < 3; ++g)
-        ct[k][g] = tfma(G[g], d0, tfma(G[3 + g], d1, tfma(G[6 + g], d2, ct[k][g])));
+        for (int g = 0; g < 3; ++g)
+          ct[k][g] = tfma(G[g], d0, tfma(G[3 + g], d1, tfma(G[6 + g], d2, ct[k][g])));
+      }
     }
   }
-  // Contributions are stored [k][cell][3]: for each local node the wavefront's 64 cells are
-  // 192 consecutive doubles, transposed through LDS so that every store is contiguous.
-  const double s = seed ? *seed : 1.0;
-  double* st = lds[threadIdx.x >> 6];
+  // Contributions are stored [k][cell][3] in T: for each local node the 64 cells of a wavefront
+  // pair are 192 consecutive values. Both wavefronts stage their part in LDS; the pair's 128
+  // lanes add them (even points + odd points) and store contiguously.
+  T* even = lds[wave & ~1u];
+  T* odd = lds[wave | 1u];
+  const uint32_t pl = threadIdx.x & 127u;  // lane within the pair
 #pragma unroll
   for (int k = 0; k < K; ++k) {
 #pragma unroll
-    for (int g = 0; g < 3; ++g) st[w * 3 + g] = s * static_cast<double>(ct[k][g]);
-    wave_sync();
-    double* dst = contrib + (static_cast<uint64_t>(k) * num_cells + c0) * 3;
-    for (uint32_t i = w; i < valid * 3; i += 64) dst[i] = st[i];
-    wave_sync();
+    for (int g = 0; g < 3; ++g) lds[wave][w * 3 + g] = ct[k][g];
+    __syncthreads();
+    T* dst = contrib + (static_cast<uint64_t>(k) * num_cells + c0) * 3;
+    for (uint32_t i = pl; i < valid * 3; i += 128) dst[i] = even[i] + odd[i];
+    __syncthreads();
   }
 }
 
-// grad[n] = sum over the CSR entries of node n, in order, of contrib[entry] (FP64).
+// grad[n] = seed * (sum over the CSR entries of node n, in order, of contrib[entry]) in FP64.
+// One lane per node; the entries' indices and values are loaded eight at a time before they
+// are added, so the loads of a node overlap.
+template <typename T>
 __global__ void __launch_bounds__(kBlock)
     csr_node_sum(const uint64_t* __restrict__ offsets, const uint32_t* __restrict__ entries,
-                 const double* __restrict__ contrib, uint64_t num_nodes, double* __restrict__ grad) {
-  // One lane per (node, component): three lanes share a node's entries.
-  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
-  if (i >= num_nodes * 3) return;
-  const uint64_t n = i / 3;
-  const uint32_t g = static_cast<uint32_t>(i - n * 3);
-  double a = 0.0;
-  const uint64_t e1 = offsets[n + 1];
-  for (uint64_t e = offsets[n]; e < e1; ++e) a += contrib[static_cast<uint64_t>(entries[e]) * 3 + g];
-  grad[i] = a;
+                 const T* __restrict__ contrib, uint64_t num_nodes, const double* __restrict__ seed,
+                 double* __restrict__ grad) {
+  const uint64_t n = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (n >= num_nodes) return;
+  const uint64_t e0 = offsets[n], e1 = offsets[n + 1];
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+  for (uint64_t e = e0; e < e1; e += 8) {
+    uint32_t idx[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) idx[i] = e + i < e1 ? entries[e + i] : 0xffffffffu;
+    T v[8][3];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      if (idx[i] != 0xffffffffu) {
+        const T* src = contrib + static_cast<uint64_t>(idx[i]) * 3;
+        v[i][0] = src[0];
+        v[i][1] = src[1];
+        v[i][2] = src[2];
+      } else {
+        v[i][0] = v[i][1] = v[i][2] = T(0);
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      if (idx[i] != 0xffffffffu) {
+        a0 += static_cast<double>(v[i][0]);
+        a1 += static_cast<double>(v[i][1]);
+        a2 += static_cast<double>(v[i][2]);
+      }
+    }
+  }
+  const double s = seed ? *seed : 1.0;
+  grad[n * 3] = s * a0;
+  grad[n * 3 + 1] = s * a1;
+  grad[n * 3 + 2] = s * a2;
 }
 
 __global__ void __launch_bounds__(kBlock) fill_zero(double* __restrict__ p, uint64_t n) {
@@ -220,8 +259,8 @@ template <typename T>
 using FwdFn =void (*)(const T*, const uint32_t*, const T*, const double*, uint32_t, uint64_t,
                        uint32_t, double, double*);
 template <typename T>
-using VjpFn = void (*)(const T*, const uint32_t*, const T*, const double*, const double*, uint32_t,
-                       uint64_t, uint32_t, double, const double*, double*);
+using VjpFn = void (*)(const T*, const uint32_t*, const T*, const double*, uint32_t, uint64_t,
+                       uint32_t, double, T*);
 
 // Register-resident 3D elements: P1-P3 tetrahedra, Q1-Q2 hexahedra, P1-P2 prisms, P1 pyramids.
 #define TG_ADJ_SHAPES(X) X(4) X(10) X(20) X(8) X(27) X(6) X(18) X(5)
@@ -249,7 +288,7 @@ VjpFn<T> vjp_kernel(uint32_t k) {
 }  // namespace
 
 uint32_t functional_partials(uint64_t num_cells) {
-  return static_cast<uint32_t>((num_cells + kBlock - 1) / kBlock);
+  return static_cast<uint32_t>((num_cells + kCells - 1) / kCells);
 }
 
 template <typename T>
@@ -270,21 +309,20 @@ hipError_t launch_functional_forward(const T* nodes, const uint32_t* dofmap, uin
 
 template <typename T>
 hipError_t launch_functional_vjp(const T* nodes, const uint32_t* dofmap, uint32_t k,
-                                 const T* tab, const double* tab64, const double* wts,
-                                 uint32_t nq, uint64_t num_cells, uint32_t last_node, double beta,
-                                 const double* seed, double* contrib, const uint64_t* offsets,
-                                 const uint32_t* entries, uint64_t num_nodes, double* grad,
-                                 hipStream_t stream) {
+                                 const T* tab, const double* wts, uint32_t nq, uint64_t num_cells,
+                                 uint32_t last_node, double beta, const double* seed, T* contrib,
+                                 const uint64_t* offsets, const uint32_t* entries,
+                                 uint64_t num_nodes, double* grad, hipStream_t stream) {
   VjpFn<T> fn = vjp_kernel<T>(k);
   if (!fn) return hipErrorNotSupported;
   const uint32_t blocks = functional_partials(num_cells);
-  hipLaunchKernelGGL(fn, dim3(blocks), dim3(kBlock), 0, stream, nodes, dofmap, tab, tab64, wts, nq,
-                     num_cells, last_node, beta, seed, contrib);
+  hipLaunchKernelGGL(fn, dim3(blocks), dim3(kBlock), 0, stream, nodes, dofmap, tab, wts, nq,
+                     num_cells, last_node, beta, contrib);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  const uint64_t nb = (num_nodes * 3 + kBlock - 1) / kBlock;
-  hipLaunchKernelGGL(csr_node_sum, dim3(static_cast<uint32_t>(nb)), dim3(kBlock), 0, stream,
-                     offsets, entries, contrib, num_nodes, grad);
+  const uint64_t nb = (num_nodes + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(csr_node_sum<T>, dim3(static_cast<uint32_t>(nb)), dim3(kBlock), 0, stream,
+                     offsets, entries, contrib, num_nodes, seed, grad);
   return hipGetLastError();
 }
 
@@ -307,14 +345,14 @@ template hipError_t launch_functional_forward<double>(const double*, const uint3
                                                       uint64_t, uint32_t, double, double*,
                                                       double*, hipStream_t);
 template hipError_t launch_functional_vjp<float>(const float*, const uint32_t*, uint32_t,
-                                                 const float*, const double*, const double*,
-                                                 uint32_t, uint64_t, uint32_t, double,
-                                                 const double*, double*, const uint64_t*,
-                                                 const uint32_t*, uint64_t, double*, hipStream_t);
+                                                 const float*, const double*, uint32_t, uint64_t,
+                                                 uint32_t, double, const double*, float*,
+                                                 const uint64_t*, const uint32_t*, uint64_t,
+                                                 double*, hipStream_t);
 template hipError_t launch_functional_vjp<double>(const double*, const uint32_t*, uint32_t,
-                                                  const double*, const double*, const double*,
-                                                  uint32_t, uint64_t, uint32_t, double,
-                                                  const double*, double*, const uint64_t*,
-                                                  const uint32_t*, uint64_t, double*, hipStream_t);
+                                                  const double*, const double*, uint32_t, uint64_t,
+                                                  uint32_t, double, const double*, double*,
+                                                  const uint64_t*, const uint32_t*, uint64_t,
+                                                  double*, hipStream_t);
 
 }  // namespace tg

@@ -236,8 +236,8 @@ int tg_plan_graph_launch(tg_plan* plan, void* stream);
 typedef struct tg_functional tg_functional;
 
 /* Binds the functional of block `block` at `pts`: builds the node incidence in CSR form (the
- * block's dofmap is copied to the host once) and allocates the FP64 work arrays (one partial
- * per 256 cells, [k][cells][3] contributions). Register-resident kernels cover P1-P3
+ * block's dofmap is copied to the host once) and allocates the work arrays (one FP64 partial
+ * per 128 cells, [k][cells][3] contributions in the geometry's type). Register-resident kernels cover P1-P3
  * tetrahedra, Q1-Q2 hexahedra, P1-P2 prisms and P1 pyramids (UnsupportedDegree otherwise). */
 int tg_functional_create(const tg_geometry* g, int32_t block, tg_pointset* pts,
                          tg_functional** out);
