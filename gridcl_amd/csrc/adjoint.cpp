@@ -17,7 +17,7 @@ struct tg_functional {
   double* partials = nullptr;   // one FP64 per forward workgroup
   void* contrib = nullptr;      // [k][cells][3] in the geometry's type
   uint64_t* offsets = nullptr;  // [nodes + 1]
-  uint32_t* entries = nullptr;  // [cells * k]: local node * cells + cell, grouped by node
+  uint32_t* entries = nullptr;  // [cells][k]: CSR position of each (cell, local node) pair
   ~tg_functional() {
     for (void* p : {static_cast<void*>(partials), contrib,
                     static_cast<void*>(offsets), static_cast<void*>(entries)})
@@ -33,8 +33,8 @@ int alloc(T** p, size_t n, const char* what) {
   return e == hipSuccess ? TG_OK : tg::hip_fail(e, what);
 }
 
-// Node incidence in CSR form from the block's dofmap (copied to the host once): for every node
-// the entries cell * k + local node in ascending order, indices clamped like the kernels' gather.
+// Node incidence in CSR form from the block's dofmap (copied to the host once): per node the
+// range of its (cell, local node) pairs, indices clamped like the kernels' gather.
 int build_incidence(tg_functional* f) {
   const tg::Block& b = f->block;
   const uint64_t n_entries = b.num_cells * b.k;
@@ -49,19 +49,20 @@ int build_incidence(tg_functional* f) {
   for (uint32_t v : dofmap) ++offsets[(v < last ? v : last) + 1];
   for (uint64_t i = 0; i < nn; ++i) offsets[i + 1] += offsets[i];
   std::vector<uint64_t> fill(offsets.begin(), offsets.end() - 1);
-  std::vector<uint32_t> entries(n_entries);
-  // Entries in ascending (cell, local node) order per node; each names the contribution's slot
-  // k * cells + cell of the [k][cell][3] contribution array.
+  // The CSR position of each (cell, local node) pair: each node's pairs in ascending (cell,
+  // local node) order. The vjp writes the pair's contribution there, so the node pass reads
+  // each node's contributions as one contiguous run.
+  std::vector<uint32_t> slot(n_entries);
   for (uint64_t i = 0; i < n_entries; ++i) {
     const uint32_t v = dofmap[i] < last ? dofmap[i] : last;
-    entries[fill[v]++] = static_cast<uint32_t>((i % b.k) * b.num_cells + i / b.k);
+    slot[i] = static_cast<uint32_t>(fill[v]++);
   }
   int rc = alloc(&f->offsets, nn + 1, "hipMalloc(offsets)");
-  if (!rc) rc = alloc(&f->entries, n_entries, "hipMalloc(entries)");
+  if (!rc) rc = alloc(&f->entries, n_entries, "hipMalloc(slots)");
   if (rc) return rc;
   e = hipMemcpy(f->offsets, offsets.data(), (nn + 1) * sizeof(uint64_t), hipMemcpyHostToDevice);
   if (e == hipSuccess)
-    e = hipMemcpy(f->entries, entries.data(), n_entries * sizeof(uint32_t), hipMemcpyHostToDevice);
+    e = hipMemcpy(f->entries, slot.data(), n_entries * sizeof(uint32_t), hipMemcpyHostToDevice);
   return e == hipSuccess ? TG_OK : tg::hip_fail(e, "hipMemcpy(incidence)");
 }
 

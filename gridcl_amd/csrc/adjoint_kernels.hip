@@ -9,9 +9,10 @@
 // per lane in FP64; a fixed tree per workgroup writes one FP64 partial; a single-workgroup pass
 // sums the partials in order. Backward: J is recomputed (not stored), dF/dJ is formed from it in
 // T, each lane sums its points' contributions dF/dx(k, g) = sum_q sum_d dF/dJ(g, d)
-// dphi_k/dxi_d, the two lanes of a cell are added (even + odd) and stored [k][cell][3] in T; a
-// node pass then sums each node's contributions in CSR order (ascending cell, then local node)
-// in FP64 and scales by the seed, so the gradient is bitwise reproducible without atomics.
+// dphi_k/dxi_d, the two lanes of a cell are added (even + odd) and each (cell, local node)
+// contribution is stored in T at its CSR position (node by node, ascending cell then local
+// node); a node pass then sums each node's contiguous run in FP64 and scales by the seed, so the
+// gradient is bitwise reproducible without atomics.
 #include <hip/hip_runtime.h>
 
 #include "adjoint_internal.hpp"
@@ -136,7 +137,7 @@ __global__ void __launch_bounds__(kBlock)
     functional_vjp_cells(const T* __restrict__ nodes, const uint32_t* __restrict__ dofmap,
                          const T* __restrict__ tab, const double* __restrict__ wts, uint32_t nq,
                          uint64_t num_cells, uint32_t last_node, double beta,
-                         T* __restrict__ contrib) {
+                         const uint32_t* __restrict__ slot, T* __restrict__ contrib) {
   __shared__ T lds[kBlock / 64][64 * 3];  // per wavefront: one local node of its 64 cells
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), half = wave & 1u,
                  w = threadIdx.x & 63u;
@@ -190,54 +191,49 @@ __global__ void __launch_bounds__(kBlock)
       }
     }
   }
-  // Contributions are stored [k][cell][3] in T: for each local node the 64 cells of a wavefront
-  // pair are 192 consecutive values. Both wavefronts stage their part in LDS; the pair's 128
-  // lanes add them (even points + odd points) and store contiguously.
-  T* even = lds[wave & ~1u];
-  T* odd = lds[wave | 1u];
-  const uint32_t pl = threadIdx.x & 127u;  // lane within the pair
+  // The cell's contributions (even points + odd points) go to their CSR positions: the odd
+  // wavefront stages its part in LDS, the even one adds and stores each pair's three values.
+  const uint32_t* __restrict__ slots = slot + c * K;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
+    if (half) {
 #pragma unroll
-    for (int g = 0; g < 3; ++g) lds[wave][w * 3 + g] = ct[k][g];
+      for (int g = 0; g < 3; ++g) lds[wave][w * 3 + g] = ct[k][g];
+    }
     __syncthreads();
-    T* dst = contrib + (static_cast<uint64_t>(k) * num_cells + c0) * 3;
-    for (uint32_t i = pl; i < valid * 3; i += 128) dst[i] = even[i] + odd[i];
+    if (!half && cl < num_cells) {
+      const T* odd = lds[wave | 1u];
+      T* dst = contrib + static_cast<uint64_t>(slots[k]) * 3;
+#pragma unroll
+      for (int g = 0; g < 3; ++g) dst[g] = ct[k][g] + odd[w * 3 + g];
+    }
     __syncthreads();
   }
 }
 
-// grad[n] = seed * (sum over the CSR entries of node n, in order, of contrib[entry]) in FP64.
-// One lane per node; the entries' indices and values are loaded eight at a time before they
-// are added, so the loads of a node overlap.
+// grad[n] = seed * (sum of node n's contributions, in CSR order) in FP64: one lane per node,
+// reading its contiguous run of contributions.
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
-    csr_node_sum(const uint64_t* __restrict__ offsets, const uint32_t* __restrict__ entries,
-                 const T* __restrict__ contrib, uint64_t num_nodes, const double* __restrict__ seed,
-                 double* __restrict__ grad) {
+    csr_node_sum(const uint64_t* __restrict__ offsets, const T* __restrict__ contrib,
+                 uint64_t num_nodes, const double* __restrict__ seed, double* __restrict__ grad) {
   const uint64_t n = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
   if (n >= num_nodes) return;
   const uint64_t e0 = offsets[n], e1 = offsets[n + 1];
   double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-  for (uint64_t e = e0; e < e1; e += 8) {
-    uint32_t idx[8];
+  // Four entries' loads are issued before their (in-order) additions.
+  for (uint64_t e = e0; e < e1; e += 4) {
+    T v[4][3];
 #pragma unroll
-    for (int i = 0; i < 8; ++i) idx[i] = e + i < e1 ? entries[e + i] : 0xffffffffu;
-    T v[8][3];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      if (idx[i] != 0xffffffffu) {
-        const T* src = contrib + static_cast<uint64_t>(idx[i]) * 3;
-        v[i][0] = src[0];
-        v[i][1] = src[1];
-        v[i][2] = src[2];
-      } else {
-        v[i][0] = v[i][1] = v[i][2] = T(0);
-      }
+    for (int i = 0; i < 4; ++i) {
+      const T* src = contrib + (e + i < e1 ? e + i : e0) * 3;
+      v[i][0] = src[0];
+      v[i][1] = src[1];
+      v[i][2] = src[2];
     }
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      if (idx[i] != 0xffffffffu) {
+    for (int i = 0; i < 4; ++i) {
+      if (e + i < e1) {
         a0 += static_cast<double>(v[i][0]);
         a1 += static_cast<double>(v[i][1]);
         a2 += static_cast<double>(v[i][2]);
@@ -260,7 +256,7 @@ using FwdFn =void (*)(const T*, const uint32_t*, const T*, const double*, uint32
                        uint32_t, double, double*);
 template <typename T>
 using VjpFn = void (*)(const T*, const uint32_t*, const T*, const double*, uint32_t, uint64_t,
-                       uint32_t, double, T*);
+                       uint32_t, double, const uint32_t*, T*);
 
 // Register-resident 3D elements: P1-P3 tetrahedra, Q1-Q2 hexahedra, P1-P2 prisms, P1 pyramids.
 #define TG_ADJ_SHAPES(X) X(4) X(10) X(20) X(8) X(27) X(6) X(18) X(5)
@@ -311,18 +307,18 @@ template <typename T>
 hipError_t launch_functional_vjp(const T* nodes, const uint32_t* dofmap, uint32_t k,
                                  const T* tab, const double* wts, uint32_t nq, uint64_t num_cells,
                                  uint32_t last_node, double beta, const double* seed, T* contrib,
-                                 const uint64_t* offsets, const uint32_t* entries,
+                                 const uint64_t* offsets, const uint32_t* slots,
                                  uint64_t num_nodes, double* grad, hipStream_t stream) {
   VjpFn<T> fn = vjp_kernel<T>(k);
   if (!fn) return hipErrorNotSupported;
   const uint32_t blocks = functional_partials(num_cells);
   hipLaunchKernelGGL(fn, dim3(blocks), dim3(kBlock), 0, stream, nodes, dofmap, tab, wts, nq,
-                     num_cells, last_node, beta, contrib);
+                     num_cells, last_node, beta, slots, contrib);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const uint64_t nb = (num_nodes + kBlock - 1) / kBlock;
   hipLaunchKernelGGL(csr_node_sum<T>, dim3(static_cast<uint32_t>(nb)), dim3(kBlock), 0, stream,
-                     offsets, entries, contrib, num_nodes, seed, grad);
+                     offsets, contrib, num_nodes, seed, grad);
   return hipGetLastError();
 }
 
