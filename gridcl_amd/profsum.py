@@ -2,10 +2,13 @@
 
     python -m gridcl_amd.profsum stats  <prefix>_kernel_stats.csv [--top N]
     python -m gridcl_amd.profsum pmc    <prefix>_counter_collection.csv [--match SUBSTRING]
+    python -m gridcl_amd.profsum trace  <prefix>_kernel_trace.csv [--match SUBSTRING] [--last N]
 
-``stats`` prints calls and average/min/max duration per kernel; ``pmc`` prints, per kernel and
-counter and grid size, the number of dispatches and the mean, min and max counter value per dispatch
-(FETCH_SIZE and WRITE_SIZE are in KiB).
+``stats`` prints calls and average/min/max duration per kernel; ``pmc`` prints, per kernel,
+counter and grid size, the number of dispatches and the mean, min and max counter value per
+dispatch (FETCH_SIZE and WRITE_SIZE are in KiB); ``trace`` prints, per kernel and grid size, the
+dispatch count and the average, min and max duration, optionally of the last N dispatches only
+(the timed steps of a run).
 """
 from __future__ import annotations
 
@@ -46,6 +49,22 @@ def pmc(path: str, match: str | None) -> None:
               f"{sum(v) / len(v):>14.1f} {min(v):>14.1f} {max(v):>14.1f}")
 
 
+def trace(path: str, match: str | None, last: int) -> None:
+    acc = defaultdict(list)  # (kernel, grid size in work-items) -> durations in ns
+    for r in csv.DictReader(open(path)):
+        if match and match not in r["Kernel_Name"]:
+            continue
+        grid = int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) * int(r["Grid_Size_Z"])
+        acc[(short_name(r["Kernel_Name"], 72), grid)].append(
+            int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+    print(f"{'kernel':<72} {'grid':>10} {'n':>6} {'avg us':>9} {'min us':>9} {'max us':>9}")
+    for (name, grid), v in sorted(acc.items(), key=lambda kv: -sum(kv[1])):
+        if last:
+            v = v[-last:]
+        print(f"{name:<72} {grid:>10} {len(v):>6} {sum(v) / len(v) / 1e3:>9.1f} "
+              f"{min(v) / 1e3:>9.1f} {max(v) / 1e3:>9.1f}")
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -55,11 +74,17 @@ def main(argv=None) -> int:
     b = sub.add_parser("pmc")
     b.add_argument("csv")
     b.add_argument("--match")
+    c = sub.add_parser("trace")
+    c.add_argument("csv")
+    c.add_argument("--match")
+    c.add_argument("--last", type=int, default=0)
     args = ap.parse_args(argv)
     if args.cmd == "stats":
         stats(args.csv, args.top)
-    else:
+    elif args.cmd == "pmc":
         pmc(args.csv, args.match)
+    else:
+        trace(args.csv, args.match, args.last)
     return 0
 
 
