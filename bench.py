@@ -215,7 +215,9 @@ def bench_q3(api, M, steps, warmup, n=96):
     out = {nm: torch.empty(int(api._lib.load().tg_field_elems(C, 64, api.factor_len(nm, 3, 3),
                                                                16)),
                            dtype=torch.float32, device="cuda") for nm in names}
-    ms = time_steps(lambda: api.geometric_factors_block(g, 0, ps, names, out=out), steps, warmup)
+    step = lambda: api.geometric_factors_block(g, 0, ps, names, out=out)
+    spin_up(step, 0.3)
+    ms = time_steps(step, steps, warmup)
     ms_dense = time_steps(lambda: api.geometric_factors_block(
         g, 0, ps, names, out=out, options=api.OPT_NO_SUMFACT), max(2, steps // 5), 1)
     nbytes = algorithmic_bytes(64, C, N, 64, 4)
