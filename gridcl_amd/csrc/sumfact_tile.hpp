@@ -1,0 +1,289 @@
+// The sum-factorised evaluation of one 16-cell tile of hexahedra by one 256-thread workgroup, as a
+// device function over caller-provided LDS (the per-block kernel in sumfact_kernels.hip calls
+// it). The arithmetic is described in sumfact_kernels.hip, the schedule at tile() below.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "geom_internal.hpp"
+
+namespace tg {
+namespace sf {
+
+constexpr int kThreads = 256;  // four wavefronts per tile of 16 cells
+
+__device__ __forceinline__ float tfma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double tfma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float tsqrt(float x) { return __builtin_sqrtf(x); }
+__device__ __forceinline__ double tsqrt(double x) { return __builtin_sqrt(x); }
+__device__ __forceinline__ float tabs(float x) { return __builtin_fabsf(x); }
+__device__ __forceinline__ double tabs(double x) { return __builtin_fabs(x); }
+
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+  typedef float type __attribute__((ext_vector_type(4)));
+};
+template <>
+struct Vec16<double> {
+  typedef double type __attribute__((ext_vector_type(2)));
+};
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// FP64 det J of cell c at point q from the stored coordinates (offsets from node 0 in FP64),
+// for near-degenerate points of FP32 geometries.
+template <typename T, int K>
+__device__ __noinline__ double detj_fp64(const T* __restrict__ nodes,
+                                         const uint32_t* __restrict__ row, uint32_t last_node,
+                                         const double* __restrict__ tab64, uint32_t q) {
+  double x0[3], j[9];
+  const uint32_t n0 = row[0] < last_node ? row[0] : last_node;
+#pragma unroll
+  for (int g = 0; g < 3; ++g) x0[g] = static_cast<double>(nodes[static_cast<uint64_t>(n0) * 3 + g]);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) j[i] = 0.0;
+  for (int k = 1; k < K; ++k) {
+    const uint32_t n = row[k] < last_node ? row[k] : last_node;
+    const double* t = tab64 + (static_cast<uint64_t>(q) * K + k) * 4;
+#pragma unroll
+    for (int g = 0; g < 3; ++g) {
+      const double d = static_cast<double>(nodes[static_cast<uint64_t>(n) * 3 + g]) - x0[g];
+#pragma unroll
+      for (int dd = 0; dd < 3; ++dd) j[dd * 3 + g] = __builtin_fma(d, t[1 + dd], j[dd * 3 + g]);
+    }
+  }
+  return j[0] * (j[4] * j[8] - j[7] * j[5]) - j[3] * (j[1] * j[8] - j[7] * j[2]) +
+         j[6] * (j[1] * j[5] - j[4] * j[2]);
+}
+
+// Copies n elements (a multiple of 16 bytes) from LDS to global memory with 16-byte
+// non-temporal stores spread over the wavefront's lanes.
+template <typename T>
+__device__ __forceinline__ void flush_run(const T* __restrict__ src, T* __restrict__ dst, int n,
+                                          uint32_t lane) {
+  using V = typename Vec16<T>::type;
+  constexpr int E = 16 / sizeof(T);
+  for (int i = static_cast<int>(lane); i < n / E; i += 64) {
+    const V v = *reinterpret_cast<const V*>(src + i * E);
+    __builtin_nontemporal_store(v, reinterpret_cast<V*>(dst + i * E));
+  }
+}
+
+// LDS of one tile (elements of T): the gathered nodes [3][16][K + 1], the cells' vertex means
+// [3][16], and one output stage per wavefront.
+template <int N1, int M, int PC>
+struct Lds {
+  static constexpr int K = N1 * N1 * N1;
+  static constexpr int kXs = 3 * 16 * (K + 1);
+  static constexpr int kShift = 3 * 16;
+  static constexpr int kStage = PC * 13 * 16;  // per wavefront
+  static constexpr int kTotal = kXs + kShift + 4 * kStage;
+};
+
+// One tile: N1 nodes and M points per direction; PC points of the p direction per pass (PC < M
+// recomputes the first two sums to save registers, for FP64).
+//
+// One workgroup of four wavefronts per tile of 16 cells. Gather: the workgroup loads the tile's
+// nodes through the dofmap (coalesced index loads, indices clamped to the last node) and, for
+// FP32, subtracts each cell's vertex mean before staging them in LDS as [g][16 cells][lex node]
+// with rows of K + 1 (the gather's writes and the sums' reads both fall in distinct banks).
+// Wavefront w takes the points with q = w; its lanes are 16 cells x 4 values of r, and each lane
+// computes X, J and det J at its points (p, w, r) for all three coordinates at once. The
+// results leave through a per-wavefront LDS stage, one r at a time: the points (0..M-1, q, r)
+// are consecutive in the point order, so X, J and det J of one r are three contiguous runs of
+// the QuadField (tiles of 16), written with 16-byte stores in order.
+// `lds` holds Lds<N1, M, PC>::kTotal elements (16-byte aligned); every thread of the 256-thread
+// workgroup calls with the same tile.
+template <typename T, int N1, int M, int PC, bool SHIFT>
+__device__ __forceinline__ void tile(const T* __restrict__ nodes, const uint32_t* __restrict__ dofmap,
+                                     const T* __restrict__ tab1d, const uint32_t* __restrict__ lex,
+                                     const double* __restrict__ tab64, const FactorParams<T>& p,
+                                     const FactorOutputs& o, uint64_t tile, T* __restrict__ lds) {
+  using L = Lds<N1, M, PC>;
+  constexpr int K = N1 * N1 * N1;
+  constexpr int NQ = M * M * M;
+  constexpr int kBlock = kThreads;
+  constexpr int PAIRS = (16 * K + kBlock - 1) / kBlock;  // (cell, node) pairs per thread
+  constexpr int SX = 0, SJ = PC * 3 * 16, SD = PC * 12 * 16, SLEN = L::kStage;
+  T* __restrict__ xs = lds;
+  T* __restrict__ shift = lds + L::kXs;
+  T* __restrict__ stage_base = lds + L::kXs + L::kShift;
+  const uint32_t t = threadIdx.x, lane = t & 63u, cell = t & 15u, r_lane = (t >> 4) & 3u,
+                 w = t >> 6;
+
+  struct P3 {
+    T v[3];
+  };
+  T held[PAIRS][3];
+  uint32_t held_at[PAIRS];
+#pragma unroll
+  for (int i = 0; i < PAIRS; ++i) {
+    const uint32_t j = t + kBlock * i;
+    held_at[i] = 0xffffffffu;
+    if (j < 16u * K) {
+      const uint32_t cc = j / K, node = j - cc * K;
+      uint64_t c = tile * 16 + cc;
+      c = c < p.num_cells ? c : p.num_cells - 1;
+      uint32_t n = dofmap[c * K + node];
+      n = n < p.last_node ? n : p.last_node;
+      const P3 x = *reinterpret_cast<const P3*>(nodes + static_cast<uint64_t>(n) * 3);
+      held_at[i] = cc * (K + 1) + lex[node];
+#pragma unroll
+      for (int g = 0; g < 3; ++g) held[i][g] = x.v[g];
+    }
+  }
+  if constexpr (SHIFT) {
+    // The eight vertices (A.5 nodes 0-7) of each cell, then each cell's mean of them.
+#pragma unroll
+    for (int i = 0; i < PAIRS; ++i)
+      if (held_at[i] != 0xffffffffu && (t + kBlock * i) % K < 8)
+#pragma unroll
+        for (int g = 0; g < 3; ++g) xs[g * 16 * (K + 1) + held_at[i]] = held[i][g];
+    __syncthreads();
+    if (t < 48) {
+      const uint32_t cc = t & 15u, g = t >> 4;
+      T acc = T(0);
+#pragma unroll
+      for (int v = 0; v < 8; ++v) {
+        const int l = (((v & 1) * (N1 - 1)) * N1 + ((v >> 1) & 1) * (N1 - 1)) * N1 + (v >> 2) * (N1 - 1);
+        acc += xs[(g * 16 + cc) * (K + 1) + l];
+      }
+      shift[g * 16 + cc] = acc * T(0.125);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < PAIRS; ++i)
+    if (held_at[i] != 0xffffffffu) {
+      const uint32_t cc = held_at[i] / (K + 1);
+#pragma unroll
+      for (int g = 0; g < 3; ++g)
+        xs[g * 16 * (K + 1) + held_at[i]] = SHIFT ? held[i][g] - shift[g * 16 + cc] : held[i][g];
+    }
+  __syncthreads();
+  const int q = static_cast<int>(w);
+  if (q >= M) return;  // the whole wavefront; no barrier follows in this function
+  // Lanes with r >= M compute r = M - 1 and stage nothing.
+  const bool r_live = r_lane < static_cast<uint32_t>(M);
+  const uint32_t r = r_live ? r_lane : M - 1;
+  T* __restrict__ stage = stage_base + w * SLEN;
+  T sg[3];
+#pragma unroll
+  for (int g = 0; g < 3; ++g) sg[g] = SHIFT ? shift[g * 16 + cell] : T(0);
+  T br[N1], dr[N1];
+#pragma unroll
+  for (int c = 0; c < N1; ++c) {
+    br[c] = tab1d[r * N1 + c];
+    dr[c] = tab1d[(M + r) * N1 + c];
+  }
+  const uint32_t mask = p.mask;
+  uint64_t c_own = tile * 16 + cell;
+  c_own = c_own < p.num_cells ? c_own : p.num_cells - 1;
+
+#pragma unroll
+  for (int p0 = 0; p0 < M; p0 += PC) {
+    constexpr int kPC = PC;
+    const int npp = M - p0 < kPC ? M - p0 : kPC;
+    // acc[pp][v][g]: v = 0 X, 1..3 the derivatives along xi, eta, zeta; sums from zero.
+    T acc[PC][4][3];
+#pragma unroll
+    for (int pp = 0; pp < PC; ++pp)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+#pragma unroll
+        for (int g = 0; g < 3; ++g) acc[pp][v][g] = T(0);
+#pragma unroll 1
+    for (int a = 0; a < N1; ++a) {
+      T s2[3][3];  // [w][g]: w = 0 (value, value), 1 (d eta), 2 (d zeta)
+#pragma unroll
+      for (int i = 0; i < 3; ++i) s2[i][0] = s2[i][1] = s2[i][2] = T(0);
+#pragma unroll
+      for (int b = 0; b < N1; ++b) {
+        const T bq = tab1d[q * N1 + b], dq = tab1d[(M + q) * N1 + b];
+#pragma unroll
+        for (int g = 0; g < 3; ++g) {
+          const T* __restrict__ xr = xs + (g * 16 + cell) * (K + 1) + (a * N1 + b) * N1;
+          T s0 = T(0), s1 = T(0);
+#pragma unroll
+          for (int c = 0; c < N1; ++c) {
+            s0 = tfma(xr[c], br[c], s0);
+            s1 = tfma(xr[c], dr[c], s1);
+          }
+          s2[0][g] = tfma(s0, bq, s2[0][g]);
+          s2[1][g] = tfma(s0, dq, s2[1][g]);
+          s2[2][g] = tfma(s1, bq, s2[2][g]);
+        }
+      }
+#pragma unroll
+      for (int pp = 0; pp < PC; ++pp) {
+        if (p0 + pp >= M) continue;
+        const T ba = tab1d[(p0 + pp) * N1 + a], da = tab1d[(M + p0 + pp) * N1 + a];
+#pragma unroll
+        for (int g = 0; g < 3; ++g) {
+          acc[pp][0][g] = tfma(s2[0][g], ba, acc[pp][0][g]);
+          acc[pp][1][g] = tfma(s2[0][g], da, acc[pp][1][g]);
+          acc[pp][2][g] = tfma(s2[1][g], ba, acc[pp][2][g]);
+          acc[pp][3][g] = tfma(s2[2][g], ba, acc[pp][3][g]);
+        }
+      }
+    }
+    // det J (first-row cofactor expansion of the column-major J, layout.rs:186-213).
+    T det[PC];
+#pragma unroll
+    for (int pp = 0; pp < PC; ++pp) {
+      const T(&A)[4][3] = acc[pp];
+      // J(g, d) = A[1 + d][g]
+      det[pp] = A[1][0] * (A[2][1] * A[3][2] - A[3][1] * A[2][2]) -
+                A[2][0] * (A[1][1] * A[3][2] - A[3][1] * A[1][2]) +
+                A[3][0] * (A[1][1] * A[2][2] - A[2][1] * A[1][2]);
+      if constexpr (sizeof(T) == 4) {
+        if (p.fixup && p0 + pp < M) {
+          T n2 = T(0);
+#pragma unroll
+          for (int d = 1; d < 4; ++d)
+#pragma unroll
+            for (int g = 0; g < 3; ++g) n2 = tfma(A[d][g], A[d][g], n2);
+          if (tabs(det[pp]) < static_cast<T>(p.tau) * n2 * tsqrt(n2))
+            det[pp] = static_cast<T>(detj_fp64<T, K>(
+                nodes, dofmap + c_own * K, p.last_node, tab64,
+                static_cast<uint32_t>(p0 + pp + M * q) + M * M * r));
+        }
+      }
+    }
+    // One r at a time: stage the points (p0.., q, r) and write their three runs.
+#pragma unroll 1
+    for (int rr = 0; rr < M; ++rr) {
+      wave_sync();
+      if (r_live && r == static_cast<uint32_t>(rr)) {
+#pragma unroll
+        for (int pp = 0; pp < PC; ++pp) {
+          if (p0 + pp >= M) continue;
+#pragma unroll
+          for (int g = 0; g < 3; ++g) {
+            stage[SX + (pp * 3 + g) * 16 + cell] = (acc[pp][0][g] + sg[g]) + p.origin[g];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) stage[SJ + (pp * 9 + d * 3 + g) * 16 + cell] = acc[pp][1 + d][g];
+          }
+          stage[SD + pp * 16 + cell] = det[pp];
+        }
+      }
+      wave_sync();
+      const uint64_t pt0 = tile * NQ + static_cast<uint32_t>(p0 + M * q + M * M * rr);
+      if (mask & kFPoints)
+        flush_run<T>(stage + SX, static_cast<T*>(o.x) + ((pt0 * 3) << 4), npp * 3 * 16, lane);
+      if (mask & kFJac)
+        flush_run<T>(stage + SJ, static_cast<T*>(o.j) + ((pt0 * 9) << 4), npp * 9 * 16, lane);
+      if (mask & kFDetJ)
+        flush_run<T>(stage + SD, static_cast<T*>(o.detj) + (pt0 << 4), npp * 16, lane);
+    }
+  }
+}
+
+}  // namespace sf
+}  // namespace tg
