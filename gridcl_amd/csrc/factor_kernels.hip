@@ -269,9 +269,9 @@ __device__ __forceinline__ TabRow<T> load_row(const T* __restrict__ t) {
 // X (without the origin) and J of the gathered cell at the point whose table rows start at t
 // (16-byte aligned); FP32 adds node 0 back to X after the contraction of the offsets. Each sum
 // starts from its first product and runs in ascending node order.
-template <typename T, int TD, int GD, int K, bool SHIFT>
-__device__ __forceinline__ void contract(const T (&x)[K][GD], const T* __restrict__ t,
-                                         bool want_x, bool want_j, T* xv, T* j) {
+template <typename T, int TD, int GD, int K, bool SHIFT, bool WX, bool WJ>
+__device__ __forceinline__ void contract_fixed(const T (&x)[K][GD], const T* __restrict__ t, T* xv,
+                                               T* j) {
   constexpr int K0 = SHIFT ? 1 : 0;  // first node whose offset is not identically zero
   T ax[GD], aj[TD * GD];
 #pragma unroll
@@ -279,8 +279,8 @@ __device__ __forceinline__ void contract(const T (&x)[K][GD], const T* __restric
     const TabRow<T> r = load_row<T>(t + 4 * k);
 #pragma unroll
     for (int g = 0; g < GD; ++g) {
-      if (want_x) ax[g] = k == K0 ? x[k][g] * r.v[0] : tfma(x[k][g], r.v[0], ax[g]);
-      if (want_j) {
+      if constexpr (WX) ax[g] = k == K0 ? x[k][g] * r.v[0] : tfma(x[k][g], r.v[0], ax[g]);
+      if constexpr (WJ) {
 #pragma unroll
         for (int d = 0; d < TD; ++d)
           aj[d * GD + g] =
@@ -288,14 +288,27 @@ __device__ __forceinline__ void contract(const T (&x)[K][GD], const T* __restric
       }
     }
   }
-  if (want_x) {
+  if constexpr (WX) {
 #pragma unroll
     for (int g = 0; g < GD; ++g) xv[g] = SHIFT ? x[0][g] + ax[g] : ax[g];
   }
-  if (want_j) {
+  if constexpr (WJ) {
 #pragma unroll
     for (int i = 0; i < TD * GD; ++i) j[i] = aj[i];
   }
+}
+
+// The factors asked for pick one straight-line contraction (a branch on uniform flags rather
+// than a select per product).
+template <typename T, int TD, int GD, int K, bool SHIFT>
+__device__ __forceinline__ void contract(const T (&x)[K][GD], const T* __restrict__ t,
+                                         bool want_x, bool want_j, T* xv, T* j) {
+  if (want_x && want_j)
+    contract_fixed<T, TD, GD, K, SHIFT, true, true>(x, t, xv, j);
+  else if (want_j)
+    contract_fixed<T, TD, GD, K, SHIFT, false, true>(x, t, xv, j);
+  else
+    contract_fixed<T, TD, GD, K, SHIFT, true, false>(x, t, xv, j);
 }
 
 // det J, recomputed in FP64 from the stored coordinates where |det J| < tau |J|_F^3 (FP32).
