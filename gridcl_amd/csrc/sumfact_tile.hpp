@@ -75,6 +75,17 @@ __device__ __forceinline__ void flush_run(const T* __restrict__ src, T* __restri
   }
 }
 
+// Lane v's value of x (a wavefront-uniform copy).
+__device__ __forceinline__ uint32_t readlane_bits(float x, int v) {
+  return static_cast<uint32_t>(__builtin_amdgcn_readlane(__builtin_bit_cast(int, x), v));
+}
+__device__ __forceinline__ uint64_t readlane_bits(double x, int v) {
+  const uint64_t b = __builtin_bit_cast(uint64_t, x);
+  const uint32_t lo = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(b), v));
+  const uint32_t hi = static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(b >> 32), v));
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
 // LDS of one tile (elements of T): the gathered nodes [3][16][K + 1], the cells' vertex means
 // [3][16], and one output stage per wavefront.
 template <int N1, int M, int PC>
@@ -89,10 +100,11 @@ struct Lds {
 // One tile: N1 nodes and M points per direction; PC points of the p direction per pass (PC < M
 // recomputes the first two sums to save registers, for FP64).
 //
-// One workgroup of four wavefronts per tile of 16 cells. Gather: the workgroup loads the tile's
-// nodes through the dofmap (coalesced index loads, indices clamped to the last node) and, for
-// FP32, subtracts each cell's vertex mean before staging them in LDS as [g][16 cells][lex node]
-// with rows of K + 1 (the gather's writes and the sums' reads both fall in distinct banks).
+// One workgroup of four wavefronts per tile of 16 cells. Gather: each wavefront loads four of the
+// tile's cells, one node per lane, through the dofmap (indices clamped to the last node) and,
+// for FP32, subtracts each cell's vertex mean (from lanes 0-7, without a barrier) before staging
+// them in LDS as [g][16 cells][lex node] with rows of K + 1 (the gather's writes and the sums'
+// reads both fall in distinct banks).
 // Wavefront w takes the points with q = w; its lanes are 16 cells x 4 values of r, and each lane
 // computes X, J and det J at its points (p, w, r) for all three coordinates at once. The
 // results leave through a per-wavefront LDS stage, one r at a time: the points (0..M-1, q, r)
@@ -109,7 +121,6 @@ __device__ __forceinline__ void tile(const T* __restrict__ nodes, const uint32_t
   constexpr int K = N1 * N1 * N1;
   constexpr int NQ = M * M * M;
   constexpr int kBlock = kThreads;
-  constexpr int PAIRS = (16 * K + kBlock - 1) / kBlock;  // (cell, node) pairs per thread
   constexpr int SX = 0, SJ = PC * 3 * 16, SD = PC * 12 * 16, SLEN = L::kStage;
   T* __restrict__ xs = lds;
   T* __restrict__ shift = lds + L::kXs;
@@ -120,52 +131,43 @@ __device__ __forceinline__ void tile(const T* __restrict__ nodes, const uint32_t
   struct P3 {
     T v[3];
   };
-  T held[PAIRS][3];
-  uint32_t held_at[PAIRS];
+  // Gather: wavefront w takes the cells w, w + 4, w + 8, w + 12 of the tile, lane k < K node k
+  // (A.5 order: the eight vertices are lanes 0-7).
+  T held[4][3];
+  const bool node_lane = lane < static_cast<uint32_t>(K);
+  const uint32_t lex_lane = node_lane ? lex[lane] : 0u;
 #pragma unroll
-  for (int i = 0; i < PAIRS; ++i) {
-    const uint32_t j = t + kBlock * i;
-    held_at[i] = 0xffffffffu;
-    if (j < 16u * K) {
-      const uint32_t cc = j / K, node = j - cc * K;
-      uint64_t c = tile * 16 + cc;
-      c = c < p.num_cells ? c : p.num_cells - 1;
-      uint32_t n = dofmap[c * K + node];
+  for (int i = 0; i < 4; ++i) {
+    const uint32_t cc = w + 4u * i;
+    uint64_t c = tile * 16 + cc;
+    c = c < p.num_cells ? c : p.num_cells - 1;
+    held[i][0] = held[i][1] = held[i][2] = T(0);
+    if (node_lane) {
+      uint32_t n = dofmap[c * K + lane];
       n = n < p.last_node ? n : p.last_node;
       const P3 x = *reinterpret_cast<const P3*>(nodes + static_cast<uint64_t>(n) * 3);
-      held_at[i] = cc * (K + 1) + lex[node];
 #pragma unroll
       for (int g = 0; g < 3; ++g) held[i][g] = x.v[g];
     }
   }
-  if constexpr (SHIFT) {
-    // The eight vertices (A.5 nodes 0-7) of each cell, then each cell's mean of them.
+  // FP32: each cell's vertex mean, summed in vertex order from lanes 0-7 (wavefront-uniform),
+  // subtracted before the nodes are staged.
 #pragma unroll
-    for (int i = 0; i < PAIRS; ++i)
-      if (held_at[i] != 0xffffffffu && (t + kBlock * i) % K < 8)
+  for (int i = 0; i < 4; ++i) {
+    const uint32_t cc = w + 4u * i;
 #pragma unroll
-        for (int g = 0; g < 3; ++g) xs[g * 16 * (K + 1) + held_at[i]] = held[i][g];
-    __syncthreads();
-    if (t < 48) {
-      const uint32_t cc = t & 15u, g = t >> 4;
-      T acc = T(0);
+    for (int g = 0; g < 3; ++g) {
+      T sg = T(0);
+      if constexpr (SHIFT) {
+        T acc = __builtin_bit_cast(T, readlane_bits(held[i][g], 0));
 #pragma unroll
-      for (int v = 0; v < 8; ++v) {
-        const int l = (((v & 1) * (N1 - 1)) * N1 + ((v >> 1) & 1) * (N1 - 1)) * N1 + (v >> 2) * (N1 - 1);
-        acc += xs[(g * 16 + cc) * (K + 1) + l];
+        for (int v = 1; v < 8; ++v) acc += __builtin_bit_cast(T, readlane_bits(held[i][g], v));
+        sg = acc * T(0.125);
+        if (lane == 0) shift[g * 16 + cc] = sg;
       }
-      shift[g * 16 + cc] = acc * T(0.125);
+      if (node_lane) xs[(g * 16 + cc) * (K + 1) + lex_lane] = held[i][g] - sg;
     }
-    __syncthreads();
   }
-#pragma unroll
-  for (int i = 0; i < PAIRS; ++i)
-    if (held_at[i] != 0xffffffffu) {
-      const uint32_t cc = held_at[i] / (K + 1);
-#pragma unroll
-      for (int g = 0; g < 3; ++g)
-        xs[g * 16 * (K + 1) + held_at[i]] = SHIFT ? held[i][g] - shift[g * 16 + cc] : held[i][g];
-    }
   __syncthreads();
   const int q = static_cast<int>(w);
   if (q >= M) return;  // the whole wavefront; no barrier follows in this function
