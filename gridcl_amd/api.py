@@ -56,6 +56,8 @@ OPT_MAP_CELL_STAGED = 16
 OPT_MAP_TILE = 32
 OPT_NO_SUMFACT = 64
 OPT_PLAN_PER_BLOCK = 128
+OPT_PLAN_SERIAL = 256
+OPT_SUMFACT_ALWAYS = 512
 
 
 # ---- Errors: one class per tetraloma_device::GridError variant of the path -------------------

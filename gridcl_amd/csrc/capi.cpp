@@ -251,6 +251,18 @@ uint32_t log2u(uint32_t v) {
 
 namespace tg {
 
+// Whether the sum-factorised kernel is the faster one for hexahedra with n1 nodes and m points
+// per direction (measured on MI355X, X + J + det J, tiles of 16, 0.26-0.88 M cells):
+//   FP32: degree 3 at every m; degrees 1 and 2 at m = 4, and degree 2 at m = 3 (alone the dense
+//         kernel is up to 8 % faster there, but overlapped with other blocks in a plan's graph
+//         the sum-factorised one gives the shorter total); otherwise the dense register kernels
+//         (1.1-4x faster at m <= 2, and for degree 1 at m = 3);
+//   FP64: degrees 2 and 3 at every m; degree 1 never (within 5 % either way).
+static bool sumfact_pays(bool f32, int n1, int m) {
+  if (n1 >= 4) return true;
+  return f32 ? (m >= 4 || (n1 == 3 && m == 3)) : n1 >= 3;
+}
+
 template <typename T>
 int enqueue_block(const tg_geometry* g, const Block& b, tg_pointset* pts, DeviceTables* tabs,
                   uint32_t factors, uint32_t tile, uint32_t options, const tg_outputs* out,
@@ -275,7 +287,8 @@ int enqueue_block(const tg_geometry* g, const Block& b, tg_pointset* pts, Device
   const bool shift = f32 && !(options & TG_OPT_NO_CELL_SHIFT);
   const uint32_t sf_factors = TG_POINTS | TG_JACOBIAN | TG_DETJ;
   if (tabs->sf_m > 0 && tile == 16 && g->gdim == 3 && !(factors & ~sf_factors) &&
-      !(options & (TG_OPT_FORCE_GENERIC | TG_OPT_NO_SUMFACT))) {
+      !(options & (TG_OPT_FORCE_GENERIC | TG_OPT_NO_SUMFACT)) &&
+      ((options & TG_OPT_SUMFACT_ALWAYS) || sumfact_pays(f32, tabs->sf_n1, tabs->sf_m))) {
     hipError_t e = tg::launch_sumfact<T>(
         tabs->sf_n1, tabs->sf_m, shift, static_cast<const T*>(g->nodes), b.dofmap,
         static_cast<const T*>(tabs->sf_tab), tabs->sf_lex, tabs->tab64, p, o, stream);

@@ -37,10 +37,16 @@ struct tg_plan {
   hipStream_t capture = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // Per-block plans captured in a graph: blocks 1.. run on aux[i - 1] (default priority), forked
+  // from and joined to the capturing stream by events, so that the blocks' kernels overlap.
+  std::vector<hipStream_t> aux;
+  std::vector<hipEvent_t> events;  // [0] fork, [i] block i done
   ~tg_plan() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     if (capture) (void)hipStreamDestroy(capture);
+    for (hipStream_t a : aux) (void)hipStreamDestroy(a);
+    for (hipEvent_t ev : events) (void)hipEventDestroy(ev);
     for (void* p : {d_blocks, static_cast<void*>(d_items), static_cast<void*>(d_ranges)})
       if (p) (void)hipFree(p);
   }
@@ -146,20 +152,52 @@ int upload_persistent(tg_plan* plan) {
   return TG_OK;
 }
 
+// Output bytes of a block (the order in which a forked per-block plan launches).
+uint64_t block_bytes(const tg_geometry* g, const PlanBlock& pb) {
+  const tg::Block& b = g->blocks[pb.index];
+  uint64_t len = 0;
+  const uint32_t lens[6] = {static_cast<uint32_t>(g->gdim),
+                            static_cast<uint32_t>(g->gdim * b.tdim), 1,
+                            static_cast<uint32_t>(g->gdim * b.tdim), static_cast<uint32_t>(g->gdim),
+                            static_cast<uint32_t>(g->gdim)};
+  for (int i = 0; i < 6; ++i)
+    if (pb.mask & (1u << i)) len += lens[i];
+  return len * b.num_cells * pb.pts->nq;
+}
+
 template <typename T>
-int launch_blocks(tg_plan* plan, hipStream_t s) {
+int launch_blocks(tg_plan* plan, hipStream_t s, bool fork) {
   const tg_geometry* g = plan->g;
-  for (const PlanBlock& pb : plan->blocks) {
+  fork = fork && !plan->aux.empty();
+  if (fork && hipEventRecord(plan->events[0], s) != hipSuccess)
+    return tg::hip_fail(hipGetLastError(), "plan fork");
+  for (size_t i = 0; i < plan->blocks.size(); ++i) {
+    const PlanBlock& pb = plan->blocks[i];
+    hipStream_t bs = s;
+    if (fork && i > 0) {
+      bs = plan->aux[i - 1];
+      hipError_t e = hipStreamWaitEvent(bs, plan->events[0], 0);
+      if (e != hipSuccess) return tg::hip_fail(e, "plan fork");
+    }
     int rc = tg::enqueue_block<T>(g, g->blocks[pb.index], pb.pts, pb.tabs, pb.mask, plan->tile,
-                                  plan->options, &pb.out, s);
+                                  plan->options, &pb.out, bs);
     if (rc) return rc;
+    if (fork && i > 0) {
+      hipError_t e = hipEventRecord(plan->events[i], bs);
+      if (e == hipSuccess) e = hipStreamWaitEvent(s, plan->events[i], 0);
+      if (e != hipSuccess) return tg::hip_fail(e, "plan join");
+    }
   }
   return TG_OK;
 }
 
-int plan_enqueue(tg_plan* plan, hipStream_t s) {
+// `fork`: per-block plans launch blocks 1.. on the auxiliary streams (used under graph capture,
+// where the fork and join become graph edges; eagerly the extra event calls cost more host time
+// than the overlap gains).
+int plan_enqueue(tg_plan* plan, hipStream_t s, bool fork) {
   if (!plan->persistent)
-    return plan->g->dtype == TG_F32 ? launch_blocks<float>(plan, s) : launch_blocks<double>(plan, s);
+    return plan->g->dtype == TG_F32 ? launch_blocks<float>(plan, s, fork)
+                                    : launch_blocks<double>(plan, s, fork);
   if (plan->num_items == 0) return TG_OK;
   hipError_t e =
       plan->g->dtype == TG_F32
@@ -216,6 +254,23 @@ int tg_plan_create(const tg_geometry* g, tg_pointset* const* pts, uint32_t facto
     plan->blocks.push_back(pb);
   }
   plan->persistent = all_hybrid && !(options & TG_OPT_PLAN_PER_BLOCK);
+  if (!plan->persistent && plan->blocks.size() > 1 && !(options & TG_OPT_PLAN_SERIAL)) {
+    // Largest block first, on the capturing stream; the others on their own streams.
+    std::stable_sort(plan->blocks.begin(), plan->blocks.end(),
+                     [&](const PlanBlock& a, const PlanBlock& b) {
+                       return block_bytes(g, a) > block_bytes(g, b);
+                     });
+    plan->events.resize(plan->blocks.size(), nullptr);
+    for (hipEvent_t& ev : plan->events) {
+      hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+      if (e != hipSuccess) return tg::hip_fail(e, "plan events");
+    }
+    plan->aux.resize(plan->blocks.size() - 1, nullptr);
+    for (hipStream_t& a : plan->aux) {
+      hipError_t e = hipStreamCreateWithFlags(&a, hipStreamNonBlocking);
+      if (e != hipSuccess) return tg::hip_fail(e, "plan streams");
+    }
+  }
   if (plan->persistent) {
     int rc = g->dtype == TG_F32 ? upload_persistent<float>(plan.get())
                                 : upload_persistent<double>(plan.get());
@@ -240,7 +295,7 @@ int tg_plan_info(const tg_plan* plan, uint32_t* num_items, uint32_t* grid, int32
 
 int tg_plan_launch(tg_plan* plan, void* stream) {
   if (!plan) return tg::fail(TG_ERR_INVALID_ARGUMENT, "null plan");
-  return plan_enqueue(plan, static_cast<hipStream_t>(stream));
+  return plan_enqueue(plan, static_cast<hipStream_t>(stream), false);
 }
 
 int tg_plan_graph_launch(tg_plan* plan, void* stream) {
@@ -252,7 +307,7 @@ int tg_plan_graph_launch(tg_plan* plan, void* stream) {
     if (e == hipSuccess) e = hipStreamBeginCapture(plan->capture, hipStreamCaptureModeThreadLocal);
     int rc = TG_OK;
     if (e == hipSuccess) {
-      rc = plan_enqueue(plan, plan->capture);
+      rc = plan_enqueue(plan, plan->capture, true);
       e = hipStreamEndCapture(plan->capture, &plan->graph);
     }
     if (rc) return rc;

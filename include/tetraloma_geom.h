@@ -92,11 +92,20 @@ enum {
   TG_OPT_MAP_TILE = 1u << 5,
   /* Hexahedra at tensor-product point sets with tiles of 16 run the sum-factorised kernel (the
    * reference's `sumfact` strategy, tetraloma-geometry/src/sumfact/tensor.rs:337) for X, J and
-   * det J; this option keeps them on the dense kernels. */
+   * det J where it is the faster one (FP32: degree 3, degree 2 from 3 points per direction,
+   * degree 1 at 4; FP64: degree >= 2); this option keeps them on the dense kernels. */
   TG_OPT_NO_SUMFACT = 1u << 6,
   /* Plans only: one launch per block (each with the kernel tg_eval_block would run) instead of
    * the persistent kernel over the work list; tg_plan_graph_launch captures them in one graph. */
-  TG_OPT_PLAN_PER_BLOCK = 1u << 7
+  TG_OPT_PLAN_PER_BLOCK = 1u << 7,
+  /* Plans with one launch per block: capture the blocks one after another in the graph (by
+   * default tg_plan_graph_launch's graph runs them as parallel branches, so they overlap;
+   * tg_plan_launch always launches them in order on the caller's stream). */
+  TG_OPT_PLAN_SERIAL = 1u << 8,
+  /* Hexahedra at tensor-product point sets: run the sum-factorised kernel wherever it applies,
+   * also for the (degree, points) pairs where the dense kernels measure faster and run by
+   * default (FP32: degree 1 below 4 points per direction, degree 2 below 3; FP64: degree 1). */
+  TG_OPT_SUMFACT_ALWAYS = 1u << 9
 };
 
 typedef struct tg_geometry tg_geometry;
@@ -209,7 +218,9 @@ uint64_t tg_evector_elems(const tg_geometry* g, int32_t block);
  * the host into one contiguous range of equal estimated work per resident workgroup (the ranges
  * of one XCD adjacent). Blocks whose element has no register-resident kernel (more than 27
  * nodes per cell, or forced generic), and TG_OPT_PLAN_PER_BLOCK, make the plan run one launch
- * per block instead.
+ * per block instead: tg_plan_launch launches them in order on the caller's stream;
+ * tg_plan_graph_launch's graph holds them as parallel branches (captured on streams of their own,
+ * default priority, forked and joined by events; TG_OPT_PLAN_SERIAL captures them in order).
  * Outputs are laid out exactly as tg_eval_block's. */
 typedef struct tg_plan tg_plan;
 int tg_plan_create(const tg_geometry* g, tg_pointset* const* pts, uint32_t factors,

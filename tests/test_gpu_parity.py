@@ -395,9 +395,10 @@ def test_mesh_level_loops(cuda):
 @pytest.mark.parametrize("m", [1, 2, 3, 4])
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
 def test_sumfact_hexahedra(cuda, p, m, dtype):
-    """Hexahedra at m^3 Gauss points with tiles of 16 run the sum-factorised kernel for X, J and
-    det J: against the oracle, and against the dense kernels (TG_OPT_NO_SUMFACT). 27 cells: the
-    second tile is padded. Tolerances: FP64 1e-12; FP32 1e-6 (X, J, det J)."""
+    """Hexahedra at m^3 Gauss points with tiles of 16: the sum-factorised kernel for X, J and
+    det J (forced by TG_OPT_SUMFACT_ALWAYS, for every (p, m)), the default choice and the dense
+    kernels (TG_OPT_NO_SUMFACT), each against the oracle. 27 cells: the second tile is padded.
+    Tolerances: FP64 1e-12; FP32 1e-6 (X, J, det J), 4e-6 for the dense kernels on Q3."""
     import torch
 
     mesh = volume_mesh(HEX, p, n=3)
@@ -406,17 +407,22 @@ def test_sumfact_hexahedra(cuda, p, m, dtype):
     ps = api.PointSet.gauss(HEX, 2 * m - 1)
     assert len(ps) == m ** 3
     names = ["points", "jacobians", "detj"]
-    got = gpu_factors(g, 0, ps, names)
+    got = gpu_factors(g, 0, ps, names, options=api.OPT_SUMFACT_ALWAYS)
+    default = gpu_factors(g, 0, ps, names)
     dense = gpu_factors(g, 0, ps, names, options=api.OPT_NO_SUMFACT)
     want = oracle_factors(g, mesh, 0, ps, names)
     err = rel_errs(got, want)
     err_dense = rel_errs(dense, want)
     print(f"hex p={p} m={m} {dtype} sumfact {err} dense {err_dense}")
     tol = 1e-12 if dtype == "f64" else 1e-6
-    for name, e in err.items():
-        assert e <= tol, (name, e)
+    # The dense FP32 contraction over 64 nodes (shifted to node 0) rounds about twice as much
+    # as the sum-factorised one; it runs for Q3 only under TG_OPT_NO_SUMFACT.
+    tol_dense = 4e-6 if (dtype == "f32" and p == 3) else tol
+    for res, t in ((got, tol), (default, tol), (dense, tol_dense)):
+        for name, e in rel_errs(res, want).items():
+            assert e <= t, (name, e)
     # The padding of the last tile holds the last cell (field.rs:234-253).
-    fields = api.geometric_factors_block(g, 0, ps, names)
+    fields = api.geometric_factors_block(g, 0, ps, names, options=api.OPT_SUMFACT_ALWAYS)
     torch.cuda.synchronize()
     raw = fields["detj"].data.cpu().numpy().astype(np.float64).reshape(2, m ** 3, 1, 16)
     assert np.array_equal(raw[1, :, 0, 11:], np.repeat(raw[1, :, 0, 10:11], 5, axis=1))
@@ -424,7 +430,7 @@ def test_sumfact_hexahedra(cuda, p, m, dtype):
     pts = ps.points.copy()
     pts[[0, -1]] = pts[[-1, 0]]
     perm_ps = api.PointSet.custom(HEX, pts, ps.weights.copy())
-    got_p = gpu_factors(g, 0, perm_ps, names)
+    got_p = gpu_factors(g, 0, perm_ps, names, options=api.OPT_SUMFACT_ALWAYS)
     for name in names:
         a = got_p[name][:, [-1] + list(range(1, m ** 3 - 1)) + [0]] if m > 1 else got_p[name]
         assert O_rel(a, got[name]) <= (1e-12 if dtype == "f64" else 4e-6), name

@@ -92,21 +92,25 @@ def test_plan_with_large_element_falls_back(cuda):
                           single["detj"].to_host_cell_major())
 
 
-def test_plan_per_block_option_matches_eval(cuda, hybrid_small):
-    """TG_OPT_PLAN_PER_BLOCK: one launch per block with tg_eval_block's kernels (the
-    sum-factorised one for the hexahedra), replayed from a graph; equal to per-block evaluation
-    bitwise."""
+@pytest.mark.parametrize("serial", [False, True])
+@pytest.mark.parametrize("graph", [False, True])
+def test_plan_per_block_option_matches_eval(cuda, hybrid_small, serial, graph):
+    """TG_OPT_PLAN_PER_BLOCK: one launch per block with tg_eval_block's kernels, forked onto
+    streams of their own and joined back (or, with TG_OPT_PLAN_SERIAL, in order on the caller's
+    stream), eagerly or replayed from a graph (three replays, outputs cleared before each);
+    equal to per-block evaluation bitwise."""
     import torch
 
     g = api.geometry_from_host(hybrid_small, dtype=torch.float32)
     rule = api.QuadRule.for_grid(g, 4)
-    plan = api.MeshPlan(g, rule, FACTORS, options=api.OPT_PLAN_PER_BLOCK)
+    opts = api.OPT_PLAN_PER_BLOCK | (api.OPT_PLAN_SERIAL if serial else 0)
+    plan = api.MeshPlan(g, rule, FACTORS, options=opts)
     assert not plan.persistent
     for _ in range(3):
         for fb in plan.fields:
             for f in fb.values():
                 f.data.zero_()
-        plan.graph_launch()
+        plan.graph_launch() if graph else plan.launch()
         torch.cuda.synchronize()
         for b, blk in enumerate(hybrid_small.blocks):
             names = list(plan.fields[b])
