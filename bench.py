@@ -51,6 +51,27 @@ def time_steps(fn, steps: int, warmup: int):
     return start.elapsed_time(stop) / steps  # ms per step
 
 
+def time_graph(fn, replays: int, calls: int = 20):
+    """ms per call of `fn`, from `replays` replays of a graph that holds `calls` calls."""
+    import torch
+
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        fn()
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        for _ in range(calls):
+            fn()
+    graph.replay()
+    torch.cuda.synchronize()
+    ms = time_steps(graph.replay, max(1, replays), 1)
+    del graph
+    return ms / calls
+
+
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "factor_direct_p2tet_64cube_hbm_traffic.txt")
 
 
@@ -198,7 +219,18 @@ def bench_adjoint(api, M, steps, warmup):
 
         ms_f = time_steps(fwd, steps, warmup)
         ms_b = time_steps(vjp, steps, warmup)
-        out[dtype] = {"ms_forward": ms_f, "ms_vjp": ms_b, "vjp_over_forward": ms_b / ms_f}
+        vjp()
+        torch.cuda.synchronize()
+        grad_eager = grad.clone()
+        # The same calls replayed from captured graphs (20 calls each): device time only, without
+        # the host's launch overhead, which at ~20-60 us per call is of the same order.
+        ms_fg = time_graph(fwd, steps)
+        ms_bg = time_graph(vjp, steps)
+        torch.cuda.synchronize()
+        assert torch.equal(grad, grad_eager), "graph-replayed vjp differs from the eager one"
+        out[dtype] = {"ms_forward": ms_f, "ms_vjp": ms_b, "vjp_over_forward": ms_b / ms_f,
+                      "ms_forward_graph": ms_fg, "ms_vjp_graph": ms_bg,
+                      "vjp_over_forward_graph": ms_bg / ms_fg}
         del fn, g
     out["cells"] = h.blocks[0].num_cells
     out["nodes"] = h.num_nodes
@@ -243,8 +275,8 @@ def main() -> int:
                     help="skip the Q3, mixed-mesh and adjoint legs")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU sharding "
-              "of this path is described in DESIGN.md, not run.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}): cell blocks would shard "
+              "across GPUs by cell ranges, which this benchmark does not run.", file=sys.stderr)
         return 2
 
     import torch

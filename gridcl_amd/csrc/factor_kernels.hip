@@ -20,6 +20,7 @@
 #include <cmath>
 
 #include "geom_internal.hpp"
+#include "xcd.hpp"
 
 namespace tg {
 namespace {
@@ -30,14 +31,6 @@ __device__ __forceinline__ float tsqrt(float x) { return __builtin_sqrtf(x); }
 __device__ __forceinline__ double tsqrt(double x) { return __builtin_sqrt(x); }
 __device__ __forceinline__ float tfma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double tfma(double a, double b, double c) { return __builtin_fma(a, b, c); }
-
-// Hardware dispatch places workgroup b on XCD b % 8. Give XCD x the contiguous range of logical
-// blocks [x * per + min(x, rem), ...): a bijection of [0, nb).
-__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
-  const uint32_t x = b & 7u, i = b >> 3;
-  const uint32_t per = nb >> 3, rem = nb & 7u;
-  return x * per + (x < rem ? x : rem) + i;
-}
 
 template <typename T>
 __device__ __forceinline__ void st(void* base, uint64_t idx, T v) {

@@ -30,6 +30,8 @@ __global__ void __launch_bounds__(sf::kThreads)
                    const T* __restrict__ tab1d, const uint32_t* __restrict__ lex,
                    const double* __restrict__ tab64, FactorParams<T> p, FactorOutputs o) {
   __shared__ __align__(16) T lds[sf::Lds<N1, M, PC>::kTotal];
+  // Tiles in dispatch order: giving each XCD a contiguous range of tiles instead (as the
+  // per-block dense kernels do) measured 9-14 % slower here.
   sf::tile<T, N1, M, PC, SHIFT>(nodes, dofmap, tab1d, lex, tab64, p, o, blockIdx.x, lds);
 }
 
