@@ -460,6 +460,15 @@ def normals_block(g: Geometry, b: int, pts: PointSet, tile: int = 16) -> QuadFie
     return geometric_factors_block(g, b, pts, ["normals"], tile)["normals"]
 
 
+def measure_block(g: Geometry, b: int, pts: PointSet, options: int = 0) -> float:
+    """The length, area or volume of block b, sum_c sum_q w_q det J (measure.rs:141): the
+    reference's fixed-order sums in the geometry's type on the device, widened to FP64 and read
+    back once (the call synchronises, as the reference's read does)."""
+    val = torch.empty((), dtype=torch.float64, device="cuda")
+    _check(_lib.load().tg_measure_block(g.handle, b, pts._h, options, val.data_ptr(), _stream()))
+    return float(val.item())
+
+
 class MeshPlan:
     """Every block of a geometry evaluated in one persistent launch (or one launch per block
     when an element has no register-resident kernel), optionally replayed from a hipGraph.
@@ -554,6 +563,16 @@ class Mesh:
 
     def facet_normals(self, rule: FacetQuadRule, tile: int = 16):
         return self._per_block(rule, "facet_normals", tile)
+
+    def measure(self, degree: int) -> float:
+        """The sum over blocks of measure_block at QuadRule.for_grid(degree) (mesh.rs:579-586;
+        one rank, so no reduction over ranks)."""
+        g = self.geometry
+        rule = QuadRule.for_grid(g, degree)
+        local = 0.0
+        for b, blk in enumerate(g.blocks):
+            local += measure_block(g, b, rule.get(blk.element.cell))
+        return local
 
 
 class MeshFunctional:

@@ -22,8 +22,9 @@ HIP_SOURCES = [
     ("factor_kernels.hip", ["-DTG_FACTOR_TYPE=double"]),
     ("sumfact_kernels.hip", []),
     ("adjoint_kernels.hip", []),
+    ("measure_kernels.hip", []),
 ]
-HOST_HIP_SOURCES = ["capi.cpp", "adjoint.cpp", "plan.cpp", "tuning.cpp"]  # host code that calls the HIP runtime
+HOST_HIP_SOURCES = ["capi.cpp", "adjoint.cpp", "plan.cpp", "tuning.cpp", "measure.cpp"]  # host code that calls the HIP runtime
 HOST_SOURCES = ["refcell.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

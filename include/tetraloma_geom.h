@@ -234,6 +234,20 @@ int tg_plan_launch(tg_plan* plan, void* stream);
  * `stream`. The plan's pointers are fixed at capture: later replays write the same outputs. */
 int tg_plan_graph_launch(tg_plan* plan, void* stream);
 
+/* ---- Measure ----------------------------------------------------------------------------------
+ *
+ * measure_block (tetraloma-geometry/src/measure.rs:141): the length, area or volume of block
+ * `block` at `pts`, sum_c sum_q w_q det J(xi_q) (the pseudo-determinant sqrt(det(J^T J)) on a
+ * curve or surface), written to *value, a device double. The arithmetic is the reference's:
+ * det J as tg_eval_block computes it (tiles of 16, `options` as there), each cell's sum over its
+ * points in the geometry's type (q in order from q = 0, the weights narrowed to that type;
+ * measure.rs:17-41), then passes of fixed-order sums of 64 values in that type (measure.rs:44-96),
+ * and the total widened to FP64. No atomics: the value is the same on every run. An empty block
+ * or point set gives 0 (measure.rs:159-161). The work arrays (the det J field and the per-cell
+ * sums) are allocated and freed on `stream` (hipMallocAsync / hipFreeAsync). */
+int tg_measure_block(const tg_geometry* g, int32_t block, tg_pointset* pts, uint32_t options,
+                     double* value, void* stream);
+
 /* ---- The mesh functional and its adjoint (no reference counterpart) --------------------------
  *
  * F(X) = sum_c sum_q w_q det J_cq(X) + beta sum_c sum_q w_q |J_cq(X)|_F^2 / det(J_cq(X))^(2/3)
