@@ -64,6 +64,10 @@ SIGNATURES = {
     "tg_plan_launch": (c_int32, [c_void_p, c_void_p]),
     "tg_plan_graph_launch": (c_int32, [c_void_p, c_void_p]),
     "tg_measure_block": (c_int32, [c_void_p, c_int32, c_void_p, c_uint32, c_void_p, c_void_p]),
+    "tg_pull_back_block": (
+        c_int32, [c_void_p, c_int32, c_void_p, c_uint64, c_uint32, c_uint32, c_double, c_uint32,
+                  c_void_p, c_void_p, c_void_p]
+    ),
     "tg_functional_create": (c_int32, [c_void_p, c_int32, c_void_p, POINTER(c_void_p)]),
     "tg_functional_destroy": (c_int32, [c_void_p]),
     "tg_functional_forward": (c_int32, [c_void_p, c_void_p, c_double, c_void_p, c_void_p]),

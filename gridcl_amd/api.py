@@ -469,6 +469,41 @@ def measure_block(g: Geometry, b: int, pts: PointSet, options: int = 0) -> float
     return float(val.item())
 
 
+def pull_back_block(g: Geometry, b: int, x: QuadField, max_iterations: int = 16,
+                    tolerance: float | None = None, options: int = 0
+                    ) -> tuple[QuadField, QuadField]:
+    """The reference points of the physical points x (a field of block b with gdim components,
+    absolute coordinates) and each point's last Newton step, as fields of x's tiling
+    (pull_back.rs:566). tolerance None: 1e-13 in FP64, 1e-5 in FP32 (pull_back.rs:49-51)."""
+    lib = _lib.load()
+    if b < 0 or b >= len(g.blocks):
+        raise InvalidArgument("block index out of range")
+    ncells = g.block_cells(b)
+    if x.num_cells != ncells:  # pull_back.rs:578-583
+        raise LengthMismatch(f"LengthMismatch: expected {ncells} cells, found {x.num_cells}")
+    if x.len != g.gdim or x.data.dtype != g.torch_dtype:
+        raise InvalidArgument("x must be a field of gdim components in the geometry's type")
+    td = TDIM[CellType(g.blocks[b].element.cell)]
+    nq, tile = x.num_points, x.tile
+    n_xi = int(lib.tg_field_elems(ncells, nq, td, tile))
+    n_st = int(lib.tg_field_elems(ncells, nq, 1, tile))
+    xi = torch.empty(max(n_xi, 1), dtype=g.torch_dtype, device=g.nodes.device)
+    st = torch.empty(max(n_st, 1), dtype=g.torch_dtype, device=g.nodes.device)
+    tol = -1.0 if tolerance is None else float(tolerance)
+    _check(lib.tg_pull_back_block(g.handle, b, _dptr(x.data), nq, tile, int(max_iterations), tol,
+                                  options, _dptr(xi), _dptr(st), _stream()))
+    return QuadField(xi, ncells, nq, td, tile), QuadField(st, ncells, nq, 1, tile)
+
+
+class PullBack:
+    """Mesh.pull_back's result (pull_back.rs:62-99): per block, the reference points and the
+    size of each point's last step."""
+
+    def __init__(self, reference: list[QuadField], last_step: list[QuadField]):
+        self.reference = reference
+        self.last_step = last_step
+
+
 class MeshPlan:
     """Every block of a geometry evaluated in one persistent launch (or one launch per block
     when an element has no register-resident kernel), optionally replayed from a hipGraph.
@@ -563,6 +598,15 @@ class Mesh:
 
     def facet_normals(self, rule: FacetQuadRule, tile: int = 16):
         return self._per_block(rule, "facet_normals", tile)
+
+    def pull_back(self, x: list[QuadField], max_iterations: int = 16,
+                  tolerance: float | None = None) -> PullBack:
+        """pull_back_block of every block's points (one field per block)."""
+        g = self.geometry
+        if len(x) != len(g.blocks):
+            raise LengthMismatch(f"LengthMismatch: expected {len(g.blocks)} fields, found {len(x)}")
+        out = [pull_back_block(g, b, f, max_iterations, tolerance) for b, f in enumerate(x)]
+        return PullBack([o[0] for o in out], [o[1] for o in out])
 
     def measure(self, degree: int) -> float:
         """The sum over blocks of measure_block at QuadRule.for_grid(degree) (mesh.rs:579-586;

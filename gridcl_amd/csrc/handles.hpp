@@ -57,6 +57,11 @@ struct tg_geometry {
   double origin[3];
   std::vector<tg::Block> blocks;
   double tau = 1e-2;
+  // The pull-back's coefficient matrices per (cell, degree, family), uploaded on first use.
+  std::map<std::tuple<int, unsigned, int>, void*> pull_back_coef;
+  ~tg_geometry() {
+    for (auto& kv : pull_back_coef) (void)hipFree(kv.second);
+  }
 };
 
 struct tg_pointset {

@@ -78,6 +78,10 @@ class LagrangeElement {
   // Values (nderivs = 0) or values and first derivatives (nderivs = 1) at points [q][tdim]:
   // out[((deriv)*nq + q)*dim + k], deriv 0 = value, 1+d = d/dxi_d.
   void tabulate(const double* pts, size_t nq, int nderivs, std::vector<double>& out) const;
+  // The spanning set: exponents or polynomial indices (a, b, c) of function j at [3 j], and the
+  // inverse Vandermonde, phi_i = sum_j coef()[j * dim + i] f_j.
+  const std::vector<int>& exps() const { return exps_; }
+  const std::vector<double>& coef() const { return coef_; }
 
  private:
   // Spanning set values and first derivatives at one point: f[(deriv)*dim + j].

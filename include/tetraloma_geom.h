@@ -248,6 +248,27 @@ int tg_plan_graph_launch(tg_plan* plan, void* stream);
 int tg_measure_block(const tg_geometry* g, int32_t block, tg_pointset* pts, uint32_t options,
                      double* value, void* stream);
 
+/* ---- Pull-back --------------------------------------------------------------------------------
+ *
+ * pull_back_block (tetraloma-geometry/src/pull_back.rs:566): the reference points of physical
+ * points. `x` is a field of block `block` (layout of tg_field_elems with LEN = gdim, tiles of
+ * `tile`, num_points points per cell, absolute coordinates, the geometry's type), e.g. the output
+ * of TG_POINTS; `xi` (LEN = tdim) and `step` (LEN = 1) are fields of the same tiling. Every point
+ * takes `max_iterations` Newton steps (the reference's default is 16) from the reference cell's
+ * vertex centroid rounded to f32: Delta = K(xi) (x - X(xi)) with K the pseudo-inverse of J
+ * (least squares on curves and surfaces), xi + Delta clamped to [-0.1, 1.1] per component, and
+ * |Delta|_inf (non-finite: the largest finite value) recorded in `step`; a point freezes, keeping
+ * xi and step, once |Delta|_inf <= tolerance (pull_back.rs:26-180). `tolerance` < 0 takes the
+ * reference's default, 1e-13 in FP64 and 1e-5 in FP32. There is no early exit: every point runs
+ * all steps, so a point's result does not depend on the others. A point with step above the
+ * tolerance has not converged (its xi is an iterate). FP32 geometries work relative to each
+ * cell's node 0 unless TG_OPT_NO_CELL_SHIFT. Elements: degrees 1-3 of intervals (in 1D and 2D),
+ * triangles and quadrilaterals (2D and 3D), tetrahedra, hexahedra and prisms (3D), and degree-1
+ * pyramids; others are UnsupportedDegree. An empty block or point set launches nothing. */
+int tg_pull_back_block(const tg_geometry* g, int32_t block, const void* x, uint64_t num_points,
+                       uint32_t tile, uint32_t max_iterations, double tolerance,
+                       uint32_t options, void* xi, void* step, void* stream);
+
 /* ---- The mesh functional and its adjoint (no reference counterpart) --------------------------
  *
  * F(X) = sum_c sum_q w_q det J_cq(X) + beta sum_c sum_q w_q |J_cq(X)|_F^2 / det(J_cq(X))^(2/3)

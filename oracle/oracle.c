@@ -3,6 +3,7 @@
  * the reference's order with separately rounded products. */
 #include "oracle.h"
 
+#include <float.h>
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -785,6 +786,88 @@ double or_functional(int cell, int degree, int family, const double* nodes, size
   free(phi);
   free(dphi);
   return total;
+}
+
+/* ---- Pull-back (pull_back.rs:26-180) -------------------------------------------------------- */
+
+int or_pull_back(int cell, int degree, int family, int gdim, const double* nodes,
+                 size_t num_nodes, const uint32_t* dofmap, size_t num_cells, const double* x,
+                 size_t nq, int max_it, double tol, double* xi, double* step) {
+  if (cell < 1 || cell > 7) return -1;
+  const int td = TDIM[cell], gd = gdim;
+  const long k = (long)or_num_nodes(cell, degree);
+  if (k <= 0 || k > 729 || gd < td || gd > 3) return -1;
+  /* The start: the vertex centroid, rounded to f32 as the reference's centroid() returns it. */
+  double start[3] = {0, 0, 0};
+  for (int d = 0; d < td; ++d) {
+    double acc = 0;
+    for (int v = 0; v < NV[cell]; ++v) acc += VX[cell][v][d];
+    start[d] = (double)(float)(acc / NV[cell]);
+  }
+  const double lo = (double)-0.1f, hi = (double)1.1f;
+  double* phi = (double*)malloc(sizeof(double) * k);
+  double* dphi = (double*)malloc(sizeof(double) * k * 3);
+  double* xc = (double*)malloc(sizeof(double) * k * 3);
+  int rc = 0;
+  for (size_t c = 0; c < num_cells && rc == 0; ++c) {
+    for (long i = 0; i < k; ++i) {
+      uint32_t nidx = dofmap[c * k + i];
+      if (nidx >= num_nodes) nidx = (uint32_t)(num_nodes - 1);
+      for (int g = 0; g < gd; ++g) xc[i * gd + g] = nodes[(size_t)nidx * gd + g];
+    }
+    for (size_t q = 0; q < nq && rc == 0; ++q) {
+      const double* y = &x[(c * nq + q) * gd];
+      double p[3] = {start[0], start[1], start[2]};
+      double last = DBL_MAX;
+      int frozen = 0;
+      for (int it = 0; it < max_it; ++it) {
+        if (or_tabulate(cell, degree, family, p, 1, phi, dphi)) {
+          rc = -1;
+          break;
+        }
+        double X[3], J[9], K[9], delta[3];
+        for (int g = 0; g < gd; ++g) {
+          double acc = xc[g] * phi[0];
+          for (long i = 1; i < k; ++i) acc += xc[i * gd + g] * phi[i];
+          X[g] = acc;
+        }
+        for (int d = 0; d < td; ++d)
+          for (int g = 0; g < gd; ++g) {
+            double acc = xc[g] * dphi[d];
+            for (long i = 1; i < k; ++i) acc += xc[i * gd + g] * dphi[i * td + d];
+            J[d * gd + g] = acc;
+          }
+        pinv_of(J, td, gd, K);
+        double s = 0;
+        int bad = 0;
+        for (int t = 0; t < td; ++t) {
+          double acc = K[t] * (y[0] - X[0]);
+          for (int g = 1; g < gd; ++g) acc += K[g * td + t] * (y[g] - X[g]);
+          delta[t] = acc;
+          const double m = fabs(acc);
+          if (t == 0 || m > s) s = m;
+          if (!(m <= DBL_MAX)) bad = 1;
+        }
+        if (bad) s = DBL_MAX;
+        if (!frozen) {
+          for (int d = 0; d < td; ++d) {
+            double v = p[d] + delta[d];
+            v = v < lo ? lo : v;
+            v = v > hi ? hi : v;
+            p[d] = v;
+          }
+          last = s;
+          frozen = s <= tol;
+        }
+      }
+      for (int d = 0; d < td; ++d) xi[(c * nq + q) * td + d] = p[d];
+      step[c * nq + q] = last;
+    }
+  }
+  free(phi);
+  free(dphi);
+  free(xc);
+  return rc;
 }
 
 double or_rel_err(const double* got, const double* want, size_t n) {

@@ -18,12 +18,18 @@
  *   - the E-vector gather with index clamping (tetraloma-geometry/src/evector.rs:197-236);
  *   - the block error metric rel_err = max|got - want| / max|want| (reference.rs:776-817);
  *   - the mesh functional of the adjoint and its FP64 gradient with respect to node
- *     coordinates (no reference counterpart; pinned by finite differences and closed forms).
+ *     coordinates (no reference counterpart; pinned by finite differences and closed forms);
+ *   - the pull-back of physical points by a fixed number of Newton steps from the reference
+ *     cell's centroid, with the reference's step, box clamp and freezing rule
+ *     (tetraloma-geometry/src/pull_back.rs:26-180), evaluated with this oracle's own bases.
  *
  * Pinning: node layouts against the reference's lagrange_points.json / tables.json fixtures;
  * bases by the Kronecker property, partition of unity and polynomial reproduction; factors by
  * affine closed forms, unit-volume sums, K J = I and the reference's doc-test examples; the
- * pyramid of degree >= 2 (Vandermonde path) by Kronecker/partition/reproduction only. */
+ * pyramid of degree >= 2 (Vandermonde path) by Kronecker/partition/reproduction only; the
+ * pull-back by round trips (the physical points of known reference points come back), by
+ * affine cells (exact after one step) and by the closest point of a surface for points off
+ * it. */
 #ifndef GRIDCL_ORACLE_H
 #define GRIDCL_ORACLE_H
 
@@ -64,6 +70,17 @@ int or_cpu_factors(int tdim, int gdim, int k, const double* nodes, size_t num_no
 double or_functional(int cell, int degree, int family, const double* nodes, size_t num_nodes,
                      const uint32_t* dofmap, size_t num_cells, const double* pts,
                      const double* wts, size_t nq, double beta, double* grad, int threads);
+
+/* Pull-back (pull_back.rs:26-180): for each cell c and point q, the reference point xi of the
+ * physical point x [c][q][gdim] (absolute coordinates) by max_it Newton steps from the cell's
+ * vertex centroid rounded to f32 (pull_back.rs:100-104): Delta = K(xi) (x - X(xi)), K the
+ * pseudo-inverse of J; s = |Delta|_inf (non-finite components count as DBL_MAX); a point not yet
+ * frozen moves to xi + Delta clamped to [-0.1f, 1.1f] per component, records s in step and
+ * freezes once s <= tol; frozen points keep xi and step. Outputs xi [c][q][tdim], step [c][q].
+ * Returns 0, or -1 for an unsupported element. */
+int or_pull_back(int cell, int degree, int family, int gdim, const double* nodes,
+                 size_t num_nodes, const uint32_t* dofmap, size_t num_cells, const double* x,
+                 size_t nq, int max_it, double tol, double* xi, double* step);
 
 /* rel_err = max|got - want| / max|want| over n values (reference.rs:776-817). */
 double or_rel_err(const double* got, const double* want, size_t n);

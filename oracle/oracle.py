@@ -49,6 +49,10 @@ def load() -> ctypes.CDLL:
         lib.or_functional.argtypes = [c_int, c_int, c_int, POINTER(c_double), c_size_t,
                                       c_void_p, c_size_t, POINTER(c_double), POINTER(c_double),
                                       c_size_t, c_double, c_void_p, c_int]
+        lib.or_pull_back.restype = c_int
+        lib.or_pull_back.argtypes = [c_int, c_int, c_int, c_int, POINTER(c_double), c_size_t,
+                                     c_void_p, c_size_t, POINTER(c_double), c_size_t, c_int,
+                                     c_double, c_void_p, c_void_p]
         lib.or_rel_err.restype = c_double
         lib.or_rel_err.argtypes = [POINTER(c_double), POINTER(c_double), c_size_t]
         _lib = lib
@@ -147,6 +151,24 @@ def functional(cell, degree, family, nodes, dofmap, pts, wts, beta=0.1, grad=Tru
     f = lib.or_functional(cell, degree, family, _d(nodes), nodes.shape[0], _p(dofmap),
                           dofmap.size // k, _d(pts), _d(wts), len(wts), beta, _p(g), 0)
     return f, g
+
+
+def pull_back(cell, degree, family, gdim, nodes, dofmap, x, max_it=16, tol=1e-13):
+    """Reference points xi [c][q][tdim] and last steps [c][q] of the physical points x
+    [c][q][gdim] (absolute), by the Newton iteration of or_pull_back (pull_back.rs:26-180)."""
+    lib = load()
+    nodes = np.ascontiguousarray(nodes, dtype=np.float64)
+    dofmap = np.ascontiguousarray(dofmap, dtype=np.uint32)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    td = lib.or_tdim(cell)
+    c, nq = x.shape[0], x.shape[1]
+    xi = np.zeros((c, nq, td))
+    step = np.zeros((c, nq))
+    rc = lib.or_pull_back(cell, degree, family, gdim, _d(nodes), nodes.shape[0], _p(dofmap), c,
+                          _d(x), nq, max_it, tol, _p(xi), _p(step))
+    if rc:
+        raise ValueError("or_pull_back failed")
+    return xi, step
 
 
 def rel_err(got, want) -> float:

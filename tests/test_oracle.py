@@ -100,3 +100,50 @@ def test_functional_single_affine_tet():
     dphi = np.array([[-1, -1, -1], [1, 0, 0], [0, 1, 0], [0, 0, 1]], dtype=float)
     want_g = (dphi @ cof.T) / 6  # [k][g] = sum_d cof[g, d] dphi[k, d] / 6
     assert np.abs(g0 - want_g).max() <= 1e-15
+
+
+@pytest.mark.parametrize("cell,p", [(TET, 1), (TET, 2), (TET, 3), (HEX, 2), (PRISM, 2), (7, 1)])
+def test_pull_back_round_trip(cell, p):
+    """or_pull_back (pull_back.rs:26-180) inverts the oracle's own X: the physical points of
+    reference points come back within 1e-13, converged (last step <= 1e-13)."""
+    mesh = M.jitter(M.cube_box(cell, 2, p), 0.05 / (2 * p), 3, interior_only=True)
+    blk = mesh.blocks[0]
+    pts = np.array([[0.1, 0.2, 0.15], [0.2, 0.2, 0.2], [0.05, 0.3, 0.4], [0.3, 0.1, 0.05]])
+    x = O.eval_block(cell, p, 0, 3, mesh.nodes, blk.cells.reshape(-1), pts,
+                     factors=("points",))["points"]
+    xi, step = O.pull_back(cell, p, 0, 3, mesh.nodes, blk.cells.reshape(-1), x)
+    assert np.abs(xi - pts[None]).max() <= 1e-13
+    assert step.max() <= 1e-13
+
+
+def test_pull_back_affine_one_step_and_far_points():
+    """Affine tetrahedra: after one step the next is zero, so two iterations converge; a point
+    far outside never converges and its iterates are clamped to [-0.1f, 1.1f]."""
+    mesh = M.config1_mesh(n=2)
+    blk = mesh.blocks[0]
+    pts = PointSet.tet_14().points
+    x = O.eval_block(TET, 1, 0, 3, mesh.nodes, blk.cells.reshape(-1), pts,
+                     factors=("points",))["points"]
+    xi, step = O.pull_back(TET, 1, 0, 3, mesh.nodes, blk.cells.reshape(-1), x, max_it=2)
+    assert np.abs(xi - pts[None]).max() <= 1e-14 and step.max() <= 1e-13
+    far = x + 10.0
+    xi, step = O.pull_back(TET, 1, 0, 3, mesh.nodes, blk.cells.reshape(-1), far)
+    assert (step > 1e-13).all()
+    assert xi.min() >= float(np.float32(-0.1)) and xi.max() <= float(np.float32(1.1))
+
+
+def test_pull_back_surface_closest_point():
+    """A point at distance 0.01 along the normal of a curved surface pulls back (least squares)
+    to the reference point of its foot: the residual x - X(xi) is normal to both tangents."""
+    m = M.square_box(2, 2, 2, gdim=3)
+    m.nodes[:, 2] = 0.2 * np.sin(np.pi * m.nodes[:, 0]) * np.sin(np.pi * m.nodes[:, 1])
+    blk = m.blocks[0]
+    pts = np.array([[0.2, 0.3], [0.6, 0.1], [0.1, 0.1]])
+    f = O.eval_block(2, 2, 0, 3, m.nodes, blk.cells.reshape(-1), pts,
+                     factors=("points", "normals", "jacobians"))
+    x = f["points"] + 0.01 * f["normals"]
+    xi, step = O.pull_back(2, 2, 0, 3, m.nodes, blk.cells.reshape(-1), x)
+    assert np.abs(xi - pts[None]).max() <= 1e-12 and step.max() <= 1e-13
+    J = f["jacobians"].reshape(*f["jacobians"].shape[:2], 2, 3)
+    r = x - f["points"]
+    assert np.abs(np.einsum("cqdg,cqg->cqd", J, r)).max() <= 1e-15
