@@ -237,6 +237,33 @@ def bench_adjoint(api, M, steps, warmup):
     return out
 
 
+def bench_pull_back(api, M, steps, warmup, n=64):
+    """Pull-back of the 14 Gauss points of every cell of the curved P2 tetrahedra 64^3 x 6
+    (the headline mesh), 16 Newton steps per point (the reference's default), FP32 and FP64:
+    the physical points are computed once, then pulled back; the round-trip error is reported."""
+    import torch
+
+    h = M.config2_mesh(n=n)
+    ps = api.PointSet.tet_14()
+    out = {"cells": h.blocks[0].num_cells, "points_per_cell": len(ps), "iterations": 16}
+    for dtype, tdt in (("f32", torch.float32), ("f64", torch.float64)):
+        g = api.geometry_from_host(h, dtype=tdt)
+        x = api.physical_points_block(g, 0, ps)
+        step = lambda: api.pull_back_block(g, 0, x)
+        ms = time_steps(step, max(2, steps // 5), 1)
+        xi, last = api.pull_back_block(g, 0, x)
+        torch.cuda.synchronize()
+        # Tiles of 16 cells: [tile][q][3][16]; every cell's points should come back.
+        want = torch.tensor(ps.points, dtype=tdt, device="cuda")[None, :, :, None]
+        err = float((xi.data.view(-1, len(ps), 3, 16) - want).abs().max())
+        pts = h.blocks[0].num_cells * len(ps)
+        out[dtype] = {"ms": ms, "points_per_s": pts / (ms * 1e-3), "max_xi_error": err,
+                      "max_last_step": float(last.data.max())}
+        del g, x, xi, last
+        torch.cuda.empty_cache()
+    return out
+
+
 def bench_q3(api, M, steps, warmup, n=96):
     """Q3 hexahedra (96^3 cells, 64 nodes and 64 Gauss points per cell), X + J + det J, FP32."""
     import torch
@@ -357,6 +384,9 @@ def main() -> int:
         torch.cuda.empty_cache()
         extras["q3_hex_96"] = bench_q3(api, M, max(5, args.steps // 5), 2)
         log(f"q3: {extras['q3_hex_96']}")
+        torch.cuda.empty_cache()
+        extras["pull_back_p2tet_64"] = bench_pull_back(api, M, args.steps, args.warmup)
+        log(f"pull-back: {extras['pull_back_p2tet_64']}")
 
     r32, r64 = results["f32"], results["f64"]
     line = {

@@ -8,7 +8,8 @@
 // takes max_iterations steps from the reference cell's centroid, Delta = K(xi) (y - X(xi)) with K
 // the pseudo-inverse of J, the new xi clamped to [-0.1, 1.1] per component, |Delta|_inf recorded
 // as the last step, and the point frozen (xi and step kept) once that step is at most the
-// tolerance. There is no early exit, so a point's result does not depend on its neighbours.
+// tolerance. A point's result does not depend on its neighbours: the wavefront stops early only
+// when all of its points are frozen, after which no step would change anything.
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -140,6 +141,9 @@ __global__ void __launch_bounds__(kLanes)
         last = s;
         frozen = s <= p.tol;
       }
+      // Frozen points keep xi and step, so once every point of the wavefront is frozen the
+      // remaining steps change nothing: leave them out (the results are those of all steps).
+      if (__all(frozen)) break;
     }
 #pragma unroll
     for (int d = 0; d < TD; ++d) xi_out[(((tile * p.nq + q) * TD + d) << tl) | in_tile] = xi[d];

@@ -259,8 +259,9 @@ int tg_measure_block(const tg_geometry* g, int32_t block, tg_pointset* pts, uint
  * (least squares on curves and surfaces), xi + Delta clamped to [-0.1, 1.1] per component, and
  * |Delta|_inf (non-finite: the largest finite value) recorded in `step`; a point freezes, keeping
  * xi and step, once |Delta|_inf <= tolerance (pull_back.rs:26-180). `tolerance` < 0 takes the
- * reference's default, 1e-13 in FP64 and 1e-5 in FP32. There is no early exit: every point runs
- * all steps, so a point's result does not depend on the others. A point with step above the
+ * reference's default, 1e-13 in FP64 and 1e-5 in FP32. The results are those of running every
+ * step for every point (frozen points do not move; a wavefront stops once all of its points are
+ * frozen), so a point's result does not depend on the others. A point with step above the
  * tolerance has not converged (its xi is an iterate). FP32 geometries work relative to each
  * cell's node 0 unless TG_OPT_NO_CELL_SHIFT. Elements: degrees 1-3 of intervals (in 1D and 2D),
  * triangles and quadrilaterals (2D and 3D), tetrahedra, hexahedra and prisms (3D), and degree-1
