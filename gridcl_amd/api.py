@@ -58,6 +58,7 @@ OPT_NO_SUMFACT = 64
 OPT_PLAN_PER_BLOCK = 128
 OPT_PLAN_SERIAL = 256
 OPT_SUMFACT_ALWAYS = 512
+OPT_POINTS_RELATIVE = 1024
 
 
 # ---- Errors: one class per tetraloma_device::GridError variant of the path -------------------
@@ -467,6 +468,14 @@ def measure_block(g: Geometry, b: int, pts: PointSet, options: int = 0) -> float
     val = torch.empty((), dtype=torch.float64, device="cuda")
     _check(_lib.load().tg_measure_block(g.handle, b, pts._h, options, val.data_ptr(), _stream()))
     return float(val.item())
+
+
+def equispaced_nodes(g: Geometry) -> torch.Tensor:
+    """The node array with every GLL block's nodes moved to the equispaced nodes of the same
+    geometry (equispaced.rs:158): [num_nodes][gdim], relative to the origin like g.nodes."""
+    out = torch.empty_like(g.nodes)
+    _check(_lib.load().tg_equispaced_nodes(g.handle, _dptr(out), _stream()))
+    return out
 
 
 def pull_back_block(g: Geometry, b: int, x: QuadField, max_iterations: int = 16,

@@ -24,8 +24,10 @@ HIP_SOURCES = [
     ("adjoint_kernels.hip", []),
     ("measure_kernels.hip", []),
     ("pullback_kernels.hip", []),
+    ("equispaced_kernels.hip", []),
 ]
-HOST_HIP_SOURCES = ["capi.cpp", "adjoint.cpp", "plan.cpp", "tuning.cpp", "measure.cpp", "pullback.cpp"]  # host code that calls the HIP runtime
+HOST_HIP_SOURCES = ["capi.cpp", "adjoint.cpp", "plan.cpp", "tuning.cpp", "measure.cpp", "pullback.cpp",
+                    "equispaced.cpp"]  # host code that calls the HIP runtime
 HOST_SOURCES = ["refcell.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

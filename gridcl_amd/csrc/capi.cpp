@@ -281,7 +281,9 @@ int enqueue_block(const tg_geometry* g, const Block& b, tg_pointset* pts, Device
   p.tau = g->tau;
   p.mapping = tg::choose_mapping(g->dtype, b.tdim, g->gdim, b.k, p.nq, tile, options);
   p.xcd_remap = tg::choose_xcd_remap();
-  for (int i = 0; i < 3; ++i) p.origin[i] = static_cast<T>(i < g->gdim ? g->origin[i] : 0.0);
+  const bool relative = (options & TG_OPT_POINTS_RELATIVE) != 0;
+  for (int i = 0; i < 3; ++i)
+    p.origin[i] = static_cast<T>(i < g->gdim && !relative ? g->origin[i] : 0.0);
   tg::FactorOutputs o{out->points, out->jacobians, out->detj, out->pinv, out->facet_normals,
                       out->normals};
   const bool shift = f32 && !(options & TG_OPT_NO_CELL_SHIFT);

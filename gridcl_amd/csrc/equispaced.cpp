@@ -1,0 +1,115 @@
+// C-ABI entry point of the GLL -> equispaced node conversion
+// (tetraloma-geometry/src/equispaced.rs:158, equispaced_nodes).
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/tetraloma_geom.h"
+#include "equispaced_internal.hpp"
+#include "handles.hpp"
+
+namespace {
+
+// The equispaced lattice of (cell, degree) as a point set (point j = local node j in A.5 order),
+// kept with the geometry so that its device tables outlive the launches that read them.
+int lattice(tg_geometry* g, const tg::Block& b, tg_pointset** out) {
+  const auto key = std::make_pair(static_cast<int>(b.cell), b.degree);
+  auto hit = g->lattices.find(key);
+  if (hit != g->lattices.end()) {
+    *out = hit->second.get();
+    return TG_OK;
+  }
+  auto p = std::make_unique<tg_pointset>();
+  p->cell = b.cell;
+  p->tdim = tg::tdim(b.cell);
+  if (!tg::lagrange_points(b.cell, b.degree, tg::kEquispaced, p->points))
+    return tg::fail(TG_ERR_UNSUPPORTED_DEGREE, "UnsupportedDegree: no equispaced lattice");
+  p->nq = p->points.size() / p->tdim;
+  p->weights.assign(p->nq, 1.0);
+  *out = p.get();
+  g->lattices.emplace(key, std::move(p));
+  return TG_OK;
+}
+
+struct Occurrences {
+  size_t block;
+  uint64_t start, len;
+};
+
+template <typename T>
+int convert(tg_geometry* g, const std::vector<Occurrences>& occ, uint32_t* first, void* out,
+            hipStream_t s) {
+  for (const Occurrences& o : occ) {
+    const tg::Block& b = g->blocks[o.block];
+    tg_pointset* pts = nullptr;
+    int rc = lattice(g, b, &pts);
+    if (rc) return rc;
+    tg::DeviceTables* tabs = nullptr;
+    rc = tg::device_tables(pts, b.degree, b.family, g->dtype, &tabs);
+    if (rc) return rc;
+    const uint64_t elems = (b.num_cells + 15) / 16 * 16 * pts->nq * g->gdim;
+    void* field = nullptr;
+    hipError_t e = hipMallocAsync(&field, elems * sizeof(T), s);
+    if (e != hipSuccess) return tg::hip_fail(e, "equispaced conversion field");
+    tg_outputs outs{};
+    outs.points = field;
+    // X at the lattice relative to the origin, like the node array (equispaced.rs:7-10).
+    rc = tg::enqueue_block<T>(g, b, pts, tabs, TG_POINTS, 16, TG_OPT_POINTS_RELATIVE, &outs, s);
+    if (rc == TG_OK) {
+      e = tg::launch_select_first<T>(g->gdim, b.dofmap, o.len, static_cast<uint32_t>(o.start),
+                                     b.k, static_cast<const T*>(field), first, g->num_nodes,
+                                     static_cast<T*>(out), s);
+      if (e != hipSuccess) rc = tg::hip_fail(e, "equispaced conversion kernel launch");
+    }
+    const hipError_t ef = hipFreeAsync(field, s);
+    if (rc) return rc;
+    if (ef != hipSuccess) return tg::hip_fail(ef, "equispaced conversion field");
+  }
+  return TG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tg_equispaced_nodes(const tg_geometry* g, void* out, void* stream) {
+  using tg::fail;
+  if (!g || !out) return fail(TG_ERR_INVALID_ARGUMENT, "null geometry or output");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const size_t elem = g->dtype == TG_F32 ? sizeof(float) : sizeof(double);
+  hipError_t e = hipMemcpyAsync(out, g->nodes, g->num_nodes * g->gdim * elem,
+                                hipMemcpyDeviceToDevice, s);
+  if (e != hipSuccess) return tg::hip_fail(e, "equispaced conversion copy");
+  // The GLL blocks' occurrences in flat (block, cell, local node) order.
+  std::vector<Occurrences> occ;
+  uint64_t total = 0;
+  for (size_t bi = 0; bi < g->blocks.size(); ++bi) {
+    const tg::Block& b = g->blocks[bi];
+    if (b.family != tg::kGll || b.num_cells == 0) continue;
+    const uint64_t len = b.num_cells * b.k;
+    occ.push_back({bi, total, len});
+    total += len;
+  }
+  if (total == 0) return TG_OK;
+  if (total > 0xffffffffull) return fail(TG_ERR_TOO_LARGE, "TooLarge: too many node occurrences");
+  tg_geometry* gm = const_cast<tg_geometry*>(g);  // only its lattice cache changes
+  void* first = nullptr;
+  e = hipMallocAsync(&first, g->num_nodes * sizeof(uint32_t), s);
+  if (e != hipSuccess) return tg::hip_fail(e, "equispaced conversion occurrences");
+  uint32_t* f = static_cast<uint32_t*>(first);
+  e = tg::launch_first_init(f, g->num_nodes, s);
+  for (const Occurrences& o : occ)
+    if (e == hipSuccess)
+      e = tg::launch_first_occurrence(g->blocks[o.block].dofmap, o.len,
+                                      static_cast<uint32_t>(o.start), g->num_nodes, f, s);
+  int rc = e == hipSuccess ? TG_OK : tg::hip_fail(e, "equispaced conversion kernel launch");
+  if (rc == TG_OK)
+    rc = g->dtype == TG_F32 ? convert<float>(gm, occ, f, out, s) : convert<double>(gm, occ, f, out, s);
+  const hipError_t ef = hipFreeAsync(first, s);
+  if (rc == TG_OK && ef != hipSuccess) rc = tg::hip_fail(ef, "equispaced conversion occurrences");
+  return rc;
+}
+
+}  // extern "C"

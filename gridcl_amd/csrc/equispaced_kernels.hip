@@ -1,0 +1,99 @@
+// GLL -> equispaced conversion of the node array (tetraloma-geometry/src/equispaced.rs:1-12,
+// equispaced_nodes): each node of a GLL block takes the value of its first occurrence in
+// (block, cell, local node) order from the block's physical points at the equispaced lattice.
+// The first occurrence of every node is the minimum of its flat occurrence indices, taken with
+// atomicMin, which gives the same node-by-node answer in any execution order; each node is then
+// written once, by that occurrence.
+#include <hip/hip_runtime.h>
+
+#include "equispaced_internal.hpp"
+
+namespace tg {
+namespace {
+
+constexpr uint32_t kBlock = 256;
+
+__global__ void __launch_bounds__(kBlock) fill_none(uint32_t* __restrict__ first, uint64_t n) {
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i < n) first[i] = 0xffffffffu;
+}
+
+// first[node] = min over the block's occurrences i of start + i (nodes past the array skipped,
+// as equispaced.rs:82 skips them).
+__global__ void __launch_bounds__(kBlock)
+    first_occurrence(const uint32_t* __restrict__ dofmap, uint64_t len, uint32_t start,
+                     uint64_t num_nodes, uint32_t* __restrict__ first) {
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= len) return;
+  const uint32_t node = dofmap[i];
+  if (node < num_nodes) atomicMin(&first[node], start + static_cast<uint32_t>(i));
+}
+
+// out[node] = the field's value at the node's first occurrence, for the occurrences of one
+// block: occurrence i is point q = i % k of cell c = i / k of the field (tiles of 16).
+template <typename T, int GD>
+__global__ void __launch_bounds__(kBlock)
+    select_first(const uint32_t* __restrict__ dofmap, uint64_t len, uint32_t start, uint32_t k,
+                 const T* __restrict__ field, const uint32_t* __restrict__ first,
+                 uint64_t num_nodes, T* __restrict__ out) {
+  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= len) return;
+  const uint32_t node = dofmap[i];
+  if (node >= num_nodes || first[node] != start + static_cast<uint32_t>(i)) return;
+  const uint64_t c = i / k, q = i - c * k;
+#pragma unroll
+  for (int g = 0; g < GD; ++g)
+    out[static_cast<uint64_t>(node) * GD + g] = field[(((c >> 4) * k + q) * GD + g) * 16 + (c & 15u)];
+}
+
+uint32_t grid_of(uint64_t n) { return static_cast<uint32_t>((n + kBlock - 1) / kBlock); }
+
+}  // namespace
+
+hipError_t launch_first_init(uint32_t* first, uint64_t num_nodes, hipStream_t stream) {
+  if (num_nodes == 0) return hipSuccess;
+  hipLaunchKernelGGL(fill_none, dim3(grid_of(num_nodes)), dim3(kBlock), 0, stream, first,
+                     num_nodes);
+  return hipGetLastError();
+}
+
+hipError_t launch_first_occurrence(const uint32_t* dofmap, uint64_t len, uint32_t start,
+                                   uint64_t num_nodes, uint32_t* first, hipStream_t stream) {
+  if (len == 0) return hipSuccess;
+  hipLaunchKernelGGL(first_occurrence, dim3(grid_of(len)), dim3(kBlock), 0, stream, dofmap, len,
+                     start, num_nodes, first);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_select_first(int gdim, const uint32_t* dofmap, uint64_t len, uint32_t start,
+                               uint32_t k, const T* field, const uint32_t* first,
+                               uint64_t num_nodes, T* out, hipStream_t stream) {
+  if (len == 0) return hipSuccess;
+  switch (gdim) {
+    case 1:
+      hipLaunchKernelGGL((select_first<T, 1>), dim3(grid_of(len)), dim3(kBlock), 0, stream,
+                         dofmap, len, start, k, field, first, num_nodes, out);
+      break;
+    case 2:
+      hipLaunchKernelGGL((select_first<T, 2>), dim3(grid_of(len)), dim3(kBlock), 0, stream,
+                         dofmap, len, start, k, field, first, num_nodes, out);
+      break;
+    case 3:
+      hipLaunchKernelGGL((select_first<T, 3>), dim3(grid_of(len)), dim3(kBlock), 0, stream,
+                         dofmap, len, start, k, field, first, num_nodes, out);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+template hipError_t launch_select_first<float>(int, const uint32_t*, uint64_t, uint32_t,
+                                               uint32_t, const float*, const uint32_t*, uint64_t,
+                                               float*, hipStream_t);
+template hipError_t launch_select_first<double>(int, const uint32_t*, uint64_t, uint32_t,
+                                                uint32_t, const double*, const uint32_t*,
+                                                uint64_t, double*, hipStream_t);
+
+}  // namespace tg

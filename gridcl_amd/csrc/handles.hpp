@@ -49,6 +49,17 @@ const char* cell_name(int c);
 
 }  // namespace tg
 
+struct tg_pointset {
+  tg::Cell cell;
+  int tdim;
+  uint64_t nq;
+  std::vector<double> points, weights, normals;
+  bool has_normals = false;
+  std::map<std::tuple<unsigned, int, int>, std::unique_ptr<tg::DeviceTables>> cache;
+  double* weights_device = nullptr;  // uploaded on first use
+  ~tg_pointset();
+};
+
 struct tg_geometry {
   int dtype;
   int gdim;
@@ -59,20 +70,11 @@ struct tg_geometry {
   double tau = 1e-2;
   // The pull-back's coefficient matrices per (cell, degree, family), uploaded on first use.
   std::map<std::tuple<int, unsigned, int>, void*> pull_back_coef;
+  // The equispaced lattices per (cell, degree) of the GLL conversion, made on first use.
+  std::map<std::pair<int, unsigned>, std::unique_ptr<tg_pointset>> lattices;
   ~tg_geometry() {
     for (auto& kv : pull_back_coef) (void)hipFree(kv.second);
   }
-};
-
-struct tg_pointset {
-  tg::Cell cell;
-  int tdim;
-  uint64_t nq;
-  std::vector<double> points, weights, normals;
-  bool has_normals = false;
-  std::map<std::tuple<unsigned, int, int>, std::unique_ptr<tg::DeviceTables>> cache;
-  double* weights_device = nullptr;  // uploaded on first use
-  ~tg_pointset();
 };
 
 namespace tg {

@@ -77,7 +77,9 @@ tg::FactorParams<T> params(const tg_geometry* g, const tg::Block& b, const tg_po
   p.tau = g->tau;
   p.mapping = tg::choose_mapping(g->dtype, b.tdim, g->gdim, b.k, p.nq, tile, options);
   p.xcd_remap = tg::choose_xcd_remap();
-  for (int i = 0; i < 3; ++i) p.origin[i] = static_cast<T>(i < g->gdim ? g->origin[i] : 0.0);
+  const bool relative = (options & TG_OPT_POINTS_RELATIVE) != 0;
+  for (int i = 0; i < 3; ++i)
+    p.origin[i] = static_cast<T>(i < g->gdim && !relative ? g->origin[i] : 0.0);
   return p;
 }
 

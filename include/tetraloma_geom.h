@@ -105,7 +105,10 @@ enum {
   /* Hexahedra at tensor-product point sets: run the sum-factorised kernel wherever it applies,
    * also for the (degree, points) pairs where the dense kernels measure faster and run by
    * default (FP32: degree 1 below 4 points per direction, degree 2 below 3; FP64: degree 1). */
-  TG_OPT_SUMFACT_ALWAYS = 1u << 9
+  TG_OPT_SUMFACT_ALWAYS = 1u << 9,
+  /* Physical points relative to the geometry's origin (the origin is not added back), like the
+   * node array; the reference evaluates its equispaced conversion this way (equispaced.rs:9-10). */
+  TG_OPT_POINTS_RELATIVE = 1u << 10
 };
 
 typedef struct tg_geometry tg_geometry;
@@ -269,6 +272,18 @@ int tg_measure_block(const tg_geometry* g, int32_t block, tg_pointset* pts, uint
 int tg_pull_back_block(const tg_geometry* g, int32_t block, const void* x, uint64_t num_points,
                        uint32_t tile, uint32_t max_iterations, double tolerance,
                        uint32_t options, void* xi, void* step, void* stream);
+
+/* ---- GLL -> equispaced nodes ------------------------------------------------------------------
+ *
+ * equispaced_nodes (tetraloma-geometry/src/equispaced.rs:158): `out` ([num_nodes][gdim], the
+ * geometry's type, relative to the origin like the node array) receives a copy of the nodes in
+ * which every node of a GLL block holds its cell's geometry evaluated at the equispaced node of
+ * the same degree (the interpolant is the same polynomial map, so the conversion is exact up to
+ * rounding). A node shared by several cells takes the value of its first occurrence in (block,
+ * cell, local node) order, so the result is the same on every run; nodes of equispaced blocks
+ * and nodes that no GLL cell uses are copied unchanged, and dofmap entries past the node array
+ * are skipped. Work arrays are allocated and freed on `stream`. */
+int tg_equispaced_nodes(const tg_geometry* g, void* out, void* stream);
 
 /* ---- The mesh functional and its adjoint (no reference counterpart) --------------------------
  *
