@@ -35,7 +35,7 @@ def stats(path: str, top: int) -> None:
               f"{float(r['MaxNs']) / 1e3:>9.1f} {float(r['Percentage']):>6.2f}")
 
 
-def pmc(path: str, match: str | None) -> None:
+def pmc(path: str, match: str | None, drop: int = 0) -> None:
     acc = defaultdict(list)  # (kernel, counter, grid size in work-items) -> values
     for r in csv.DictReader(open(path)):
         if match and match not in r["Kernel_Name"]:
@@ -45,11 +45,13 @@ def pmc(path: str, match: str | None) -> None:
     print(f"{'kernel':<72} {'counter':<11} {'n':>5} {'grid':>9} {'mean':>14} {'min':>14} "
           f"{'max':>14}")
     for (name, counter, grid), v in sorted(acc.items(), key=lambda kv: -max(kv[1])):
+        if drop:
+            v = v[:-drop] or v
         print(f"{name:<72} {counter:<11} {len(v):>5} {grid:>9} "
               f"{sum(v) / len(v):>14.1f} {min(v):>14.1f} {max(v):>14.1f}")
 
 
-def trace(path: str, match: str | None, last: int) -> None:
+def trace(path: str, match: str | None, last: int, drop: int = 0) -> None:
     acc = defaultdict(list)  # (kernel, grid size in work-items) -> durations in ns
     for r in csv.DictReader(open(path)):
         if match and match not in r["Kernel_Name"]:
@@ -59,6 +61,8 @@ def trace(path: str, match: str | None, last: int) -> None:
             int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
     print(f"{'kernel':<72} {'grid':>10} {'n':>6} {'avg us':>9} {'min us':>9} {'max us':>9}")
     for (name, grid), v in sorted(acc.items(), key=lambda kv: -sum(kv[1])):
+        if drop:
+            v = v[:-drop] or v
         if last:
             v = v[-last:]
         print(f"{name:<72} {grid:>10} {len(v):>6} {sum(v) / len(v) / 1e3:>9.1f} "
@@ -74,17 +78,21 @@ def main(argv=None) -> int:
     b = sub.add_parser("pmc")
     b.add_argument("csv")
     b.add_argument("--match")
+    b.add_argument("--drop", type=int, default=0,
+                   help="leave out the last DROP dispatches of each (kernel, counter, grid)")
     c = sub.add_parser("trace")
     c.add_argument("csv")
     c.add_argument("--match")
     c.add_argument("--last", type=int, default=0)
+    c.add_argument("--drop", type=int, default=0,
+                   help="leave out the last DROP dispatches of each (kernel, grid) first")
     args = ap.parse_args(argv)
     if args.cmd == "stats":
         stats(args.csv, args.top)
     elif args.cmd == "pmc":
-        pmc(args.csv, args.match)
+        pmc(args.csv, args.match, args.drop)
     else:
-        trace(args.csv, args.match, args.last)
+        trace(args.csv, args.match, args.last, args.drop)
     return 0
 
 
