@@ -264,6 +264,77 @@ def bench_pull_back(api, M, steps, warmup, n=64):
     return out
 
 
+def gmsh_sections(M, mesh):
+    """The sections of an MSH 4.1 file holding `mesh`'s one cell block: node tags 1..N in one
+    block, element tags 1..C, each cell's nodes in Gmsh's order."""
+    from gridcl_amd import gmsh
+
+    blk = mesh.blocks[0]
+    order = gmsh.gmsh_to_internal(blk.cell, blk.degree)
+    cells = blk.cells.reshape(blk.num_cells, -1)
+    coords = np.zeros((mesh.num_nodes, 3))
+    coords[:, :mesh.gdim] = mesh.nodes
+    ty = {(4, 2): 11, (4, 1): 4}[(blk.cell, blk.degree)]
+    return gmsh.Sections(
+        node_blocks=[(3, 1, np.arange(1, mesh.num_nodes + 1, dtype=np.int64), coords)],
+        element_blocks=[(3, 1, ty, np.arange(1, blk.num_cells + 1, dtype=np.int64),
+                         cells[:, order].astype(np.int64) + 1)])
+
+
+def bench_gmsh(api, M, mesh, runs=5):
+    """Reading the headline mesh (curved P2 tetrahedra 64^3 x 6) from a binary MSH 4.1 file:
+    the host reader (parse, unique-tag checks, Gmsh-to-A.5 permutation, u32 dofmap) from memory
+    and from a file in the page cache, then the upload into a device geometry; the P2 mesh
+    32^3 x 6 as an ASCII file. The dofmap read back must equal the generator's bit for bit."""
+    import ctypes
+    import tempfile
+
+    import torch
+
+    from gridcl_amd import _lib, gmsh
+
+    lib = _lib.load()
+    out = {}
+    for name, m, binary in (("binary_p2tet_64", mesh, True),
+                            ("ascii_p2tet_32", M.config5_mesh(), False)):
+        data = gmsh.encode(gmsh_sections(M, m), binary)
+        times = []
+        for _ in range(runs):
+            h = ctypes.c_void_p()
+            t0 = time.perf_counter()
+            rc = lib.tg_gmsh_read_bytes(data, len(data), 0, ctypes.byref(h))
+            times.append(time.perf_counter() - t0)
+            assert rc == 0, _lib.last_error()
+            lib.tg_gmsh_destroy(h)
+        mem = float(np.median(times))
+        with tempfile.NamedTemporaryFile(suffix=".msh") as f:
+            f.write(data)
+            f.flush()
+            times = []
+            for _ in range(runs):
+                h = ctypes.c_void_p()
+                t0 = time.perf_counter()
+                rc = lib.tg_gmsh_read_file(f.name.encode(), 0, ctypes.byref(h))
+                times.append(time.perf_counter() - t0)
+                assert rc == 0, _lib.last_error()
+                lib.tg_gmsh_destroy(h)
+            disk = float(np.median(times))
+            read = gmsh.read(f.name)
+        cells = m.blocks[0].cells.reshape(m.blocks[0].num_cells, -1)
+        assert np.array_equal(read.blocks[0].nodes, cells)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g = read.geometry(torch.float32)
+        torch.cuda.synchronize()
+        upload = time.perf_counter() - t0
+        del g
+        out[name] = {"file_mb": len(data) / 1e6, "cells": int(cells.shape[0]),
+                     "nodes": int(m.num_nodes), "read_bytes_s": mem, "read_file_s": disk,
+                     "mb_per_s": len(data) / 1e6 / mem, "cells_per_s": cells.shape[0] / mem,
+                     "upload_f32_s": upload, "dofmap_identical": True}
+    return out
+
+
 def bench_q3(api, M, steps, warmup, n=96):
     """Q3 hexahedra (96^3 cells, 64 nodes and 64 Gauss points per cell), X + J + det J, FP32."""
     import torch
@@ -387,6 +458,8 @@ def main() -> int:
         torch.cuda.empty_cache()
         extras["pull_back_p2tet_64"] = bench_pull_back(api, M, args.steps, args.warmup)
         log(f"pull-back: {extras['pull_back_p2tet_64']}")
+        extras["gmsh_read"] = bench_gmsh(api, M, mesh)
+        log(f"gmsh: {extras['gmsh_read']}")
 
     r32, r64 = results["f32"], results["f64"]
     line = {

@@ -8,7 +8,8 @@ from __future__ import annotations
 
 import ctypes
 import os
-from ctypes import POINTER, c_char_p, c_double, c_int32, c_size_t, c_uint32, c_uint64, c_void_p
+from ctypes import (POINTER, c_char_p, c_double, c_int32, c_int64, c_size_t, c_uint32, c_uint64,
+                    c_void_p)
 
 LIB_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")
 LIB_PATH = os.path.join(LIB_DIR, "libtetraloma_geom.so")
@@ -75,6 +76,22 @@ SIGNATURES = {
     "tg_functional_vjp": (
         c_int32, [c_void_p, c_void_p, c_double, c_void_p, c_void_p, c_void_p]
     ),
+    "tg_gmsh_read_file": (c_int32, [c_char_p, c_int32, POINTER(c_void_p)]),
+    "tg_gmsh_read_bytes": (c_int32, [c_char_p, c_uint64, c_int32, POINTER(c_void_p)]),
+    "tg_gmsh_destroy": (c_int32, [c_void_p]),
+    "tg_gmsh_last_error_detail": (c_int64, []),
+    "tg_gmsh_get_info": (c_int32, [c_void_p, c_void_p]),
+    "tg_gmsh_nodes": (c_void_p, [c_void_p]),
+    "tg_gmsh_node_tags": (c_void_p, [c_void_p]),
+    "tg_gmsh_get_block": (c_int32, [c_void_p, c_int32, c_void_p]),
+    "tg_gmsh_get_tag_group": (c_int32, [c_void_p, c_int32, c_void_p]),
+    "tg_gmsh_get_name": (
+        c_int32, [c_void_p, c_int32, POINTER(c_int32), POINTER(c_int32), POINTER(c_char_p)]
+    ),
+    "tg_gmsh_get_element_block": (c_int32, [c_void_p, c_int32, c_void_p]),
+    "tg_gmsh_to_internal": (
+        c_int32, [c_int32, c_uint32, POINTER(c_int32), c_size_t, POINTER(c_size_t)]
+    ),
 }
 
 
@@ -102,6 +119,68 @@ class Outputs(ctypes.Structure):
         ("pinv", c_void_p),
         ("facet_normals", c_void_p),
         ("normals", c_void_p),
+    ]
+
+
+class GmshInfo(ctypes.Structure):
+    """tg_gmsh_info."""
+
+    _fields_ = [
+        ("tdim", c_int32),
+        ("gdim", c_int32),
+        ("num_nodes", c_uint64),
+        ("num_blocks", c_int32),
+        ("num_tag_groups", c_int32),
+        ("num_names", c_int32),
+        ("binary", c_int32),
+        ("file_nodes", c_uint64),
+        ("num_element_blocks", c_int32),
+        ("partitioned", c_int32),
+        ("num_partitions", c_uint64),
+        ("num_periodic", c_uint64),
+    ]
+
+
+class GmshBlock(ctypes.Structure):
+    """tg_gmsh_block."""
+
+    _fields_ = [
+        ("cell", c_int32),
+        ("degree", c_uint32),
+        ("num_cells", c_uint64),
+        ("nodes_per_cell", c_uint32),
+        ("nodes", c_void_p),
+        ("tags", c_void_p),
+        ("elementary", c_void_p),
+        ("num_physical", c_uint64),
+        ("physical", c_void_p),
+    ]
+
+
+class GmshTagGroup(ctypes.Structure):
+    """tg_gmsh_tag_group."""
+
+    _fields_ = [
+        ("dim", c_int32),
+        ("cell", c_int32),
+        ("num_elements", c_uint64),
+        ("corners_per_element", c_uint32),
+        ("corners", c_void_p),
+        ("tags", c_void_p),
+        ("elementary", c_void_p),
+        ("num_physical", c_uint64),
+        ("physical", c_void_p),
+    ]
+
+
+class GmshElementBlock(ctypes.Structure):
+    """tg_gmsh_element_block."""
+
+    _fields_ = [
+        ("dim", c_int32),
+        ("entity_tag", c_int32),
+        ("element_type", c_int32),
+        ("count", c_uint64),
     ]
 
 

@@ -103,9 +103,41 @@ class ServerError(GridError):
     code = 9
 
 
+# The Gmsh reader's variants (tetraloma-device/src/error.rs:129-190). `detail` is the number the
+# variant carries: the repeated id, the element tag, the byte offset or the element type.
+class DimensionMismatch(GridError):
+    code = 10
+
+
+class DuplicateOriginId(GridError):
+    code = 11
+
+
+class DuplicateCellBlock(GridError):
+    code = 12
+
+
+class UnmatchedTagElement(GridError):
+    code = 13
+
+
+class Io(GridError):
+    code = 14
+
+
+class Parse(GridError):
+    code = 15
+
+
+class UnsupportedElementType(GridError):
+    code = 16
+
+
 _ERRORS = {c.code: c for c in (LengthMismatch, TooLarge, AllocationTooLarge, UnsupportedDegree,
                                UnsupportedNodeFamily, CellNotInRule, UnsupportedPrecision,
-                               InvalidArgument, ServerError)}
+                               InvalidArgument, ServerError, DimensionMismatch, DuplicateOriginId,
+                               DuplicateCellBlock, UnmatchedTagElement, Io, Parse,
+                               UnsupportedElementType)}
 
 
 def _check(rc: int) -> None:

@@ -28,7 +28,7 @@ HIP_SOURCES = [
 ]
 HOST_HIP_SOURCES = ["capi.cpp", "adjoint.cpp", "plan.cpp", "tuning.cpp", "measure.cpp", "pullback.cpp",
                     "equispaced.cpp"]  # host code that calls the HIP runtime
-HOST_SOURCES = ["refcell.cpp"]
+HOST_SOURCES = ["refcell.cpp", "gmsh_order.cpp", "gmsh_parse.cpp", "gmsh_mesh.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [

@@ -41,7 +41,16 @@ typedef enum tg_status {
   TG_ERR_CELL_NOT_IN_RULE = 6,     /* GridError::CellNotInRule(cell) */
   TG_ERR_UNSUPPORTED_PRECISION = 7, /* GridError::UnsupportedPrecision {scalar, device} */
   TG_ERR_INVALID_ARGUMENT = 8,     /* null handle or pointer, bad tile, block index out of range */
-  TG_ERR_HIP = 9                   /* GridError::Server(..): a HIP runtime call failed */
+  TG_ERR_HIP = 9,                  /* GridError::Server(..): a HIP runtime call failed */
+  /* The Gmsh reader's variants (tetraloma-device/src/error.rs:129-190); tg_gmsh_last_error_detail
+   * gives the number each carries. */
+  TG_ERR_DIMENSION_MISMATCH = 10,  /* GridError::DimensionMismatch {expected, found} */
+  TG_ERR_DUPLICATE_ORIGIN_ID = 11, /* GridError::DuplicateOriginId(id): detail = id */
+  TG_ERR_DUPLICATE_CELL_BLOCK = 12, /* GridError::DuplicateCellBlock(cell) */
+  TG_ERR_UNMATCHED_TAG_ELEMENT = 13, /* GridError::UnmatchedTagElement {dim, tag}: detail = tag */
+  TG_ERR_IO = 14,                  /* GridError::Io(message) */
+  TG_ERR_PARSE = 15,               /* GridError::Parse {format, offset, ..}: detail = offset */
+  TG_ERR_UNSUPPORTED_ELEMENT_TYPE = 16 /* GridError::UnsupportedElementType: detail = type id */
 } tg_status;
 
 /* tetraloma_dims::CellType, in its order (tetraloma-dims/src/lib.rs). */
@@ -311,6 +320,88 @@ int tg_functional_forward(tg_functional* f, const void* nodes, double beta, doub
 /* grad ([num_nodes][3] FP64, device) = seed * dF/dX, with `seed` a device double (NULL: 1). */
 int tg_functional_vjp(tg_functional* f, const void* nodes, double beta, const double* seed,
                       double* grad, void* stream);
+
+/* ---- Gmsh MSH 4.1 files (host) ----------------------------------------------------------------
+ *
+ * Replaces tetraloma_io::gmsh::{read_file, parse} followed by to_gmsh_mesh
+ * (tetraloma-io/src/gmsh/parse.rs:571, 621; convert.rs:229): ASCII and binary files, either byte
+ * order, 4- or 8-byte size_t, into the mesh the file describes. Cell blocks hold the elements of
+ * the highest dimension, one block per cell type in tg_cell order, cells in file order, nodes as
+ * u32 rows into the node array in A.5 order (Gmsh's node order permuted,
+ * tetraloma-refcell/src/node_maps/gmsh.rs); serendipity cells (Gmsh types 16-19) are completed to
+ * P2. Lower-dimensional elements become tag groups with their corner node tags. Node coordinates
+ * are [node][gdim] f64 with gdim inferred from the coordinates (or the caller's, gdim > 0). A
+ * cell block's `nodes` is the dofmap tg_geometry_create takes once copied to the device, with
+ * family TG_EQUISPACED. Everything here runs on the host and touches no device. */
+typedef struct tg_gmsh_mesh tg_gmsh_mesh;
+
+typedef struct tg_gmsh_info {
+  int32_t tdim, gdim;         /* GmshMesh::{tdim, gdim} */
+  uint64_t num_nodes;         /* mesh nodes: the file's, then those the P2 completion adds */
+  int32_t num_blocks, num_tag_groups, num_names;
+  int32_t binary;             /* the file's encoding */
+  uint64_t file_nodes;        /* nodes in $Nodes */
+  int32_t num_element_blocks; /* element blocks in $Elements */
+  int32_t partitioned;        /* the file has $PartitionedEntities */
+  uint64_t num_partitions;
+  uint64_t num_periodic;      /* links in $Periodic */
+} tg_gmsh_info;
+
+/* GmshCellBlock (convert.rs:36-52). Arrays stay owned by the mesh handle. */
+typedef struct tg_gmsh_block {
+  int32_t cell;              /* tg_cell */
+  uint32_t degree;
+  uint64_t num_cells;
+  uint32_t nodes_per_cell;   /* k */
+  const uint32_t* nodes;     /* [num_cells][k], A.5 order */
+  const int64_t* tags;       /* element tag per cell */
+  const int32_t* elementary; /* entity tag per cell */
+  uint64_t num_physical;
+  const int64_t* physical;   /* [num_physical][2]: (cell index, physical tag), sorted */
+} tg_gmsh_block;
+
+/* GmshTagGroup (convert.rs:82-96). */
+typedef struct tg_gmsh_tag_group {
+  int32_t dim;
+  int32_t cell;
+  uint64_t num_elements;
+  uint32_t corners_per_element;
+  const int64_t* corners;    /* [num_elements][vertices], node tags in internal vertex order */
+  const int64_t* tags;
+  const int32_t* elementary;
+  uint64_t num_physical;
+  const int64_t* physical;   /* [num_physical][2]: (element index, physical tag), sorted */
+} tg_gmsh_tag_group;
+
+/* One element block of $Elements as stored (model.rs, ElementBlock without its arrays). */
+typedef struct tg_gmsh_element_block {
+  int32_t dim, entity_tag, element_type;
+  uint64_t count;
+} tg_gmsh_element_block;
+
+/* Reads the file at `path` (Io if it cannot be read) or the `len` bytes at `bytes`; gdim 0
+ * infers the geometric dimension. Errors: Parse (malformed or truncated file, with the byte
+ * offset), UnsupportedElementType, DuplicateOriginId, DuplicateCellBlock, DimensionMismatch,
+ * UnmatchedTagElement, LengthMismatch, TooLarge, as to_gmsh_mesh (convert.rs:183-206). */
+int tg_gmsh_read_file(const char* path, int32_t gdim, tg_gmsh_mesh** out);
+int tg_gmsh_read_bytes(const void* bytes, uint64_t len, int32_t gdim, tg_gmsh_mesh** out);
+int tg_gmsh_destroy(tg_gmsh_mesh* m);
+/* The number the last reader error carries on this thread (byte offset, tag or type id). */
+int64_t tg_gmsh_last_error_detail(void);
+int tg_gmsh_get_info(const tg_gmsh_mesh* m, tg_gmsh_info* info);
+/* [num_nodes][gdim] coordinates and the node tags (origin ids), owned by the handle. */
+const double* tg_gmsh_nodes(const tg_gmsh_mesh* m);
+const int64_t* tg_gmsh_node_tags(const tg_gmsh_mesh* m);
+int tg_gmsh_get_block(const tg_gmsh_mesh* m, int32_t b, tg_gmsh_block* out);
+int tg_gmsh_get_tag_group(const tg_gmsh_mesh* m, int32_t g, tg_gmsh_tag_group* out);
+/* Physical name i ($PhysicalNames, model.rs:20-27); *name is owned by the handle. */
+int tg_gmsh_get_name(const tg_gmsh_mesh* m, int32_t i, int32_t* dim, int32_t* tag,
+                     const char** name);
+int tg_gmsh_get_element_block(const tg_gmsh_mesh* m, int32_t i, tg_gmsh_element_block* out);
+/* gmsh_to_internal (tetraloma-refcell/src/node_maps.rs:54): map[k] = the A.5 index of Gmsh's
+ * k-th node of the degree-`degree` Lagrange element on `cell` (internal[map[k]] = file[k]);
+ * *len = the node count (map NULL: only the count). The point cell gives {0}. */
+int tg_gmsh_to_internal(int32_t cell, uint32_t degree, int32_t* map, size_t cap, size_t* len);
 
 #ifdef __cplusplus
 }
