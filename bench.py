@@ -332,6 +332,22 @@ def bench_gmsh(api, M, mesh, runs=5):
                      "nodes": int(m.num_nodes), "read_bytes_s": mem, "read_file_s": disk,
                      "mb_per_s": len(data) / 1e6 / mem, "cells_per_s": cells.shape[0] / mem,
                      "upload_f32_s": upload, "dofmap_identical": True}
+    # VTU output of the headline geometry (FP32): download, VTK order, raw appended data.
+    from gridcl_amd import vtk
+
+    g = api.geometry_from_host(mesh, dtype=torch.float32)
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "mesh.vtu")
+        times = []
+        for _ in range(3):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            vtk.write_geometry(path, g)
+            times.append(time.perf_counter() - t0)
+        size = os.path.getsize(path)
+    del g
+    out["vtu_write_p2tet_64"] = {"file_mb": size / 1e6, "write_s": float(np.median(times)),
+                                 "mb_per_s": size / 1e6 / float(np.median(times))}
     return out
 
 

@@ -89,6 +89,18 @@ SIGNATURES = {
         c_int32, [c_void_p, c_int32, POINTER(c_int32), POINTER(c_int32), POINTER(c_char_p)]
     ),
     "tg_gmsh_get_element_block": (c_int32, [c_void_p, c_int32, c_void_p]),
+    "tg_vtk_to_internal": (
+        c_int32, [c_int32, c_uint32, POINTER(c_int32), c_size_t, POINTER(c_size_t)]
+    ),
+    "tg_vtk_cell_type": (c_int32, [c_int32, c_uint32, c_int32]),
+    "tg_vtu_write_mesh": (
+        c_int32, [c_char_p, c_int32, c_void_p, c_uint64, c_void_p, c_int32, c_uint32, c_void_p,
+                  c_int32, c_void_p, c_int32, POINTER(c_uint64)]
+    ),
+    "tg_vtu_write_geometry": (
+        c_int32, [c_void_p, c_char_p, c_uint32, c_void_p, c_int32, c_void_p, c_int32,
+                  POINTER(c_uint64), c_void_p]
+    ),
     "tg_gmsh_to_internal": (
         c_int32, [c_int32, c_uint32, POINTER(c_int32), c_size_t, POINTER(c_size_t)]
     ),
@@ -170,6 +182,29 @@ class GmshTagGroup(ctypes.Structure):
         ("elementary", c_void_p),
         ("num_physical", c_uint64),
         ("physical", c_void_p),
+    ]
+
+
+class VtuArray(ctypes.Structure):
+    """tg_vtu_array."""
+
+    _fields_ = [
+        ("name", c_char_p),
+        ("type", c_int32),
+        ("components", c_uint32),
+        ("values", c_void_p),
+        ("count", c_uint64),
+    ]
+
+
+class VtuBlock(ctypes.Structure):
+    """tg_vtu_block."""
+
+    _fields_ = [
+        ("cell", c_int32),
+        ("degree", c_uint32),
+        ("nodes", c_void_p),
+        ("len", c_uint64),
     ]
 
 

@@ -133,11 +133,15 @@ class UnsupportedElementType(GridError):
     code = 16
 
 
+class IndexOutOfRange(GridError):
+    code = 17
+
+
 _ERRORS = {c.code: c for c in (LengthMismatch, TooLarge, AllocationTooLarge, UnsupportedDegree,
                                UnsupportedNodeFamily, CellNotInRule, UnsupportedPrecision,
                                InvalidArgument, ServerError, DimensionMismatch, DuplicateOriginId,
                                DuplicateCellBlock, UnmatchedTagElement, Io, Parse,
-                               UnsupportedElementType)}
+                               UnsupportedElementType, IndexOutOfRange)}
 
 
 def _check(rc: int) -> None:

@@ -27,8 +27,9 @@ HIP_SOURCES = [
     ("equispaced_kernels.hip", []),
 ]
 HOST_HIP_SOURCES = ["capi.cpp", "adjoint.cpp", "plan.cpp", "tuning.cpp", "measure.cpp", "pullback.cpp",
-                    "equispaced.cpp"]  # host code that calls the HIP runtime
-HOST_SOURCES = ["refcell.cpp", "gmsh_order.cpp", "gmsh_parse.cpp", "gmsh_mesh.cpp"]
+                    "equispaced.cpp", "vtk_geometry.cpp"]  # host code that calls the HIP runtime
+HOST_SOURCES = ["refcell.cpp", "gmsh_order.cpp", "gmsh_parse.cpp", "gmsh_mesh.cpp", "vtk_order.cpp",
+                "vtk_write.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [
@@ -73,7 +74,7 @@ def build_library(verbose: bool = False) -> str:
     objs = [obj for obj, _ in jobs]
     out = os.path.join(OUT_DIR, "libtetraloma_geom.so")
     tmp = out + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-lz", "-o", tmp])
     os.replace(tmp, out)
     if verbose:
         print(f"built {out}")

@@ -50,7 +50,8 @@ typedef enum tg_status {
   TG_ERR_UNMATCHED_TAG_ELEMENT = 13, /* GridError::UnmatchedTagElement {dim, tag}: detail = tag */
   TG_ERR_IO = 14,                  /* GridError::Io(message) */
   TG_ERR_PARSE = 15,               /* GridError::Parse {format, offset, ..}: detail = offset */
-  TG_ERR_UNSUPPORTED_ELEMENT_TYPE = 16 /* GridError::UnsupportedElementType: detail = type id */
+  TG_ERR_UNSUPPORTED_ELEMENT_TYPE = 16, /* GridError::UnsupportedElementType: detail = type id */
+  TG_ERR_INDEX_OUT_OF_RANGE = 17   /* GridError::IndexOutOfRange {index, len} (the VTU writer) */
 } tg_status;
 
 /* tetraloma_dims::CellType, in its order (tetraloma-dims/src/lib.rs). */
@@ -402,6 +403,62 @@ int tg_gmsh_get_element_block(const tg_gmsh_mesh* m, int32_t i, tg_gmsh_element_
  * k-th node of the degree-`degree` Lagrange element on `cell` (internal[map[k]] = file[k]);
  * *len = the node count (map NULL: only the count). The point cell gives {0}. */
 int tg_gmsh_to_internal(int32_t cell, uint32_t degree, int32_t* map, size_t cap, size_t* len);
+
+/* ---- VTU output (host) ------------------------------------------------------------------------
+ *
+ * Replaces tetraloma_io::vtk::{to_vtu, VtuFile::write_to} (tetraloma-io/src/vtk/convert.rs:116,
+ * write.rs:378) and, for a device geometry on one rank, tetraloma::VtkWriter::vtu(..).write
+ * (tetraloma/src/vtk.rs:736, 869): an UnstructuredGrid piece, version 2.2, little-endian,
+ * UInt64 headers. Points are every node padded to three coordinates; cells of degree >= 2 are
+ * VTK Lagrange cells (68-73) with their nodes in VTK's order, linear cells list their corners,
+ * pyramids of degree >= 2 are written from their vertices (type 14) and counted in *pyramids. */
+typedef enum tg_vtu_option {
+  TG_VTU_LINEAR = 1u << 0,     /* every cell as its linear type (VtkWriter::lagrange(false)) */
+  TG_VTU_BASE64 = 1u << 1,     /* base64 inside each array instead of raw appended data */
+  TG_VTU_ZLIB = 1u << 2,       /* zlib-compressed data in blocks of 32 KiB */
+  TG_VTU_F32_POINTS = 1u << 3  /* Float32 points (tg_vtu_write_mesh; FP32 geometries always) */
+} tg_vtu_option;
+
+typedef enum tg_vtu_type {
+  TG_VTU_F32 = 0, TG_VTU_F64 = 1, TG_VTU_I32 = 2, TG_VTU_U32 = 3, TG_VTU_I64 = 4, TG_VTU_U8 = 5
+} tg_vtu_type;
+
+/* A named data array (DataArray, tetraloma-io/src/vtk/model.rs): count = entries x components. */
+typedef struct tg_vtu_array {
+  const char* name;
+  int32_t type;         /* tg_vtu_type */
+  uint32_t components;
+  const void* values;   /* host for tg_vtu_write_mesh, device for tg_vtu_write_geometry */
+  uint64_t count;
+} tg_vtu_array;
+
+/* One cell block (VtkCellBlock): host u32 node rows in A.5 order, len = cells x k. */
+typedef struct tg_vtu_block {
+  int32_t cell;
+  uint32_t degree;
+  const uint32_t* nodes;
+  uint64_t len;
+} tg_vtu_block;
+
+/* vtk_to_internal (tetraloma-refcell/src/node_maps.rs:107): map[k] = the A.5 index of VTK's k-th
+ * node (degree 1 only for the pyramid, UnsupportedDegree otherwise); map NULL: only *len. */
+int tg_vtk_to_internal(int32_t cell, uint32_t degree, int32_t* map, size_t cap, size_t* len);
+/* vtk_cell_type (tetraloma-io/src/vtk/cell_type.rs:41); -1 for a bad cell or degree 0. */
+int32_t tg_vtk_cell_type(int32_t cell, uint32_t degree, int32_t lagrange);
+/* to_vtu + write_to on host arrays: nodes [num_nodes][gdim] f64. Errors as to_vtu
+ * (DimensionMismatch, LengthMismatch, UnsupportedDegree, IndexOutOfRange, TooLarge) and Io. */
+int tg_vtu_write_mesh(const char* path, int32_t gdim, const double* nodes, uint64_t num_nodes,
+                      const tg_vtu_block* blocks, int32_t num_blocks, uint32_t options,
+                      const tg_vtu_array* point_data, int32_t num_point_data,
+                      const tg_vtu_array* cell_data, int32_t num_cell_data, uint64_t* pyramids);
+/* The geometry's mesh: its nodes converted to equispaced nodes on the device when a block is
+ * GLL (tg_equispaced_nodes), the origin added in f64 (tetraloma/src/mesh.rs:227-241), point data
+ * indexed by node and cell data by flat cell index in block order (device arrays). Synchronises
+ * the stream (the file is written on return). */
+int tg_vtu_write_geometry(const tg_geometry* g, const char* path, uint32_t options,
+                          const tg_vtu_array* point_data, int32_t num_point_data,
+                          const tg_vtu_array* cell_data, int32_t num_cell_data,
+                          uint64_t* pyramids, void* stream);
 
 #ifdef __cplusplus
 }
