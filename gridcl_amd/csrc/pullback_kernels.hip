@@ -155,6 +155,8 @@ template <typename T>
 using PullBackFn = void (*)(const T*, const uint32_t*, const T*, const T*, T*, T*,
                             PullBackParams<T>);
 
+// Degrees 1-4; pyramids degree 1, hexahedra 1-3 (the degree-4 hexahedron's 125 nodes and
+// coefficients per cell would take 96 KiB of LDS per 16 FP64 cells, above a workgroup's 64 KiB).
 template <typename T, int CELL, int GD>
 PullBackFn<T> by_degree(int degree) {
   if constexpr (CELL == kPyramid) {
@@ -164,6 +166,9 @@ PullBackFn<T> by_degree(int degree) {
       case 1: return pull_back_tile<T, CELL, GD, 1>;
       case 2: return pull_back_tile<T, CELL, GD, 2>;
       case 3: return pull_back_tile<T, CELL, GD, 3>;
+      case 4:
+        if constexpr (CELL == kHexahedron) return nullptr;
+        else return pull_back_tile<T, CELL, GD, 4>;
       default: return nullptr;
     }
   }

@@ -276,9 +276,10 @@ int tg_measure_block(const tg_geometry* g, int32_t block, tg_pointset* pts, uint
  * step for every point (frozen points do not move; a wavefront stops once all of its points are
  * frozen), so a point's result does not depend on the others. A point with step above the
  * tolerance has not converged (its xi is an iterate). FP32 geometries work relative to each
- * cell's node 0 unless TG_OPT_NO_CELL_SHIFT. Elements: degrees 1-3 of intervals (in 1D and 2D),
- * triangles and quadrilaterals (2D and 3D), tetrahedra, hexahedra and prisms (3D), and degree-1
- * pyramids; others are UnsupportedDegree. An empty block or point set launches nothing. */
+ * cell's node 0 unless TG_OPT_NO_CELL_SHIFT. Elements: degrees 1-4 of intervals (in 1D and 2D),
+ * triangles and quadrilaterals (2D and 3D), tetrahedra and prisms (3D), degrees 1-3 of
+ * hexahedra and degree-1 pyramids; others are UnsupportedDegree. An empty block or point set
+ * launches nothing. */
 int tg_pull_back_block(const tg_geometry* g, int32_t block, const void* x, uint64_t num_points,
                        uint32_t tile, uint32_t max_iterations, double tolerance,
                        uint32_t options, void* xi, void* step, void* stream);

@@ -70,11 +70,11 @@ def torch_dtype(dtype):
     return torch.float64 if dtype == "f64" else torch.float32
 
 
-CASES = ([(IV, p, g) for p in (1, 2, 3) for g in (1, 2)]
-         + [(TET, p, 3) for p in (1, 2, 3)] + [(HEX, p, 3) for p in (1, 2, 3)]
-         + [(PRISM, p, 3) for p in (1, 2, 3)] + [(PYR, 1, 3)]
-         + [(TRI, p, g) for p in (1, 2, 3) for g in (2, 3)]
-         + [(QUAD, p, g) for p in (1, 2, 3) for g in (2, 3)])
+CASES = ([(IV, p, g) for p in (1, 2, 3, 4) for g in (1, 2)]
+         + [(TET, p, 3) for p in (1, 2, 3, 4)] + [(HEX, p, 3) for p in (1, 2, 3)]
+         + [(PRISM, p, 3) for p in (1, 2, 3, 4)] + [(PYR, 1, 3)]
+         + [(TRI, p, g) for p in (1, 2, 3, 4) for g in (2, 3)]
+         + [(QUAD, p, g) for p in (1, 2, 3, 4) for g in (2, 3)])
 
 
 @pytest.mark.parametrize("cell,p,gdim", CASES)
@@ -103,7 +103,8 @@ def oracle_pull_back(g, mesh, b, x_cell_major, max_it=16, tol=1e-13):
                        x_cell_major, max_it, tol)
 
 
-@pytest.mark.parametrize("cell,p", [(TET, 2), (HEX, 2), (PRISM, 2), (PYR, 1), (TET, 3)])
+@pytest.mark.parametrize("cell,p", [(TET, 2), (HEX, 2), (PRISM, 2), (PYR, 1), (TET, 3), (TET, 4),
+                                    (PRISM, 4)])
 def test_matches_oracle_inside_and_outside(cuda, cell, p):
     """FP64, against the oracle's iteration: points inside the cells, points up to 0.05 outside
     in reference coordinates (their preimages lie in the box [-0.1, 1.1], so they converge to
@@ -138,7 +139,7 @@ def test_matches_oracle_inside_and_outside(cuda, cell, p):
     assert fin.min() >= float(np.float32(-0.1)) and fin.max() <= float(np.float32(1.1))
 
 
-@pytest.mark.parametrize("cell,p", [(TRI, 2), (QUAD, 2), (TRI, 3)])
+@pytest.mark.parametrize("cell,p", [(TRI, 2), (QUAD, 2), (TRI, 3), (QUAD, 4)])
 def test_surface_points_pull_back_to_closest_point(cuda, cell, p):
     """Points moved off a curved surface by 0.01 along its normal pull back (least squares) to
     the reference point they came from (the closest point of the surface), as in the oracle."""
@@ -180,7 +181,8 @@ def test_affine_cells_take_one_step(cuda, dtype):
 def test_tiles_and_mesh_level_and_errors(cuda):
     """The result does not depend on the fields' tile (bitwise); Mesh.pull_back covers every
     block of the mixed mesh; a field of another size is LengthMismatch, an element without a
-    kernel UnsupportedDegree, and an empty point set launches nothing."""
+    kernel (Q4 hexahedra, P5 tetrahedra) UnsupportedDegree, and an empty point set launches
+    nothing."""
     import torch
 
     mesh = volume_mesh(TET, 2)
@@ -214,10 +216,14 @@ def test_tiles_and_mesh_level_and_errors(cuda):
     x2 = api.physical_points_block(g2, 0, ps)
     with pytest.raises(api.LengthMismatch):
         api.pull_back_block(g, 0, x2)
-    g4 = api.geometry_from_host(volume_mesh(TET, 4, n=1), dtype=torch.float32)
-    x4 = api.physical_points_block(g4, 0, ps)
+    g4 = api.geometry_from_host(volume_mesh(HEX, 4, n=1), dtype=torch.float32)
+    x4 = api.physical_points_block(g4, 0, api.PointSet.gauss(HEX, 4))
     with pytest.raises(api.UnsupportedDegree):
         api.pull_back_block(g4, 0, x4)
+    g5 = api.geometry_from_host(volume_mesh(TET, 5, n=1), dtype=torch.float32)
+    x5 = api.physical_points_block(g5, 0, ps)
+    with pytest.raises(api.UnsupportedDegree):
+        api.pull_back_block(g5, 0, x5)
     empty = api.PointSet.custom(TET, np.zeros((0, 3)), np.zeros(0))
     x0 = api.physical_points_block(g, 0, empty)
     xi0, _ = api.pull_back_block(g, 0, x0)

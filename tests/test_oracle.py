@@ -102,7 +102,8 @@ def test_functional_single_affine_tet():
     assert np.abs(g0 - want_g).max() <= 1e-15
 
 
-@pytest.mark.parametrize("cell,p", [(TET, 1), (TET, 2), (TET, 3), (HEX, 2), (PRISM, 2), (7, 1)])
+@pytest.mark.parametrize("cell,p", [(TET, 1), (TET, 2), (TET, 3), (TET, 4), (HEX, 2), (PRISM, 2),
+                                    (PRISM, 4), (7, 1)])
 def test_pull_back_round_trip(cell, p):
     """or_pull_back (pull_back.rs:26-180) inverts the oracle's own X: the physical points of
     reference points come back within 1e-13, converged (last step <= 1e-13)."""
