@@ -146,7 +146,15 @@ def cpu_baseline(mesh, ps, sample_cells: int):
         times.append((time.perf_counter() - t0) / calls)
     med = float(np.median(times))
     evals = sample_cells * len(ps)
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")),
+                         model)
+    except OSError:
+        pass
     return {"value": evals / med, "unit": "evals/s", "cores": threads, "kind": "port",
+            "cpu_model": model, "host_cpus": os.cpu_count(),
             "sample": f"first {sample_cells} cells of the benchmark mesh x {len(ps)} points "
                       f"(FP64, E-vector + X + J + det J, tiles of 4), median of 10 runs of "
                       f"{calls} calls"}, out
