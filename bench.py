@@ -359,6 +359,47 @@ def bench_gmsh(api, M, mesh, runs=5):
     return out
 
 
+def bench_measure(api, M, steps, warmup, n=48):
+    """Block measure (sum_c sum_q w_q det J, the reference's fixed-order sums widened to FP64) of
+    the curved P2 tetrahedra n^3 x 6 (48: another grid than the headline's, so that profiles keep
+    the headline's dispatches apart) at the 14-point rule, FP32 and FP64, on preallocated
+    outputs."""
+    import torch
+
+    mesh = M.config2_mesh(n=n)
+    ps = api.PointSet.tet_14()
+    lib = api._lib.load()
+    out = {"cells": mesh.blocks[0].num_cells, "points_per_cell": len(ps)}
+    for dtype, tdt in (("f32", torch.float32), ("f64", torch.float64)):
+        g = api.geometry_from_host(mesh, dtype=tdt)
+        val = torch.empty((), dtype=torch.float64, device="cuda")
+        call = lambda: lib.tg_measure_block(g.handle, 0, ps._h, 0, val.data_ptr(), api._stream())
+        ms = time_steps(call, steps, warmup)
+        torch.cuda.synchronize()
+        out[dtype] = {"ms": ms, "evals_per_s": mesh.blocks[0].num_cells * len(ps) / (ms * 1e-3),
+                      "volume": float(val.item())}
+        del g
+        torch.cuda.empty_cache()
+    return out
+
+
+def bench_equispaced(api, M, steps, warmup, n=64):
+    """GLL -> equispaced node conversion of Q3 hexahedra n^3 read as GLL nodes, FP32: X at each
+    cell's 64 equispaced lattice points, then each node's first occurrence."""
+    import torch
+
+    h = M.config3_mesh(n=n)
+    h = M.HostMesh(h.gdim, h.nodes, [M.HostBlock(b.cell, b.degree, b.cells, family=1)
+                                     for b in h.blocks])
+    g = api.geometry_from_host(h, dtype=torch.float32)
+    lib = api._lib.load()
+    out_t = torch.empty_like(g.nodes)
+    call = lambda: lib.tg_equispaced_nodes(g.handle, out_t.data_ptr(), api._stream())
+    ms = time_steps(call, max(2, steps // 5), 2)
+    return {"cells": h.blocks[0].num_cells, "nodes": h.num_nodes, "ms": ms,
+            "nodes_per_s": h.num_nodes / (ms * 1e-3)}
+
+
 def bench_q3(api, M, steps, warmup, n=96):
     """Q3 hexahedra (96^3 cells, 64 nodes and 64 Gauss points per cell), X + J + det J, FP32."""
     import torch
@@ -482,6 +523,11 @@ def main() -> int:
         torch.cuda.empty_cache()
         extras["pull_back_p2tet_64"] = bench_pull_back(api, M, args.steps, args.warmup)
         log(f"pull-back: {extras['pull_back_p2tet_64']}")
+        extras["measure_p2tet_48"] = bench_measure(api, M, args.steps, args.warmup)
+        log(f"measure: {extras['measure_p2tet_48']}")
+        extras["equispaced_q3hex_64"] = bench_equispaced(api, M, args.steps, args.warmup)
+        log(f"equispaced: {extras['equispaced_q3hex_64']}")
+        torch.cuda.empty_cache()
         extras["gmsh_read"] = bench_gmsh(api, M, mesh)
         log(f"gmsh: {extras['gmsh_read']}")
 

@@ -279,7 +279,7 @@ int enqueue_block(const tg_geometry* g, const Block& b, tg_pointset* pts, Device
   p.fixup = (f32 && !(options & TG_OPT_NO_FP64_FIXUP) && !(options & TG_OPT_NO_CELL_SHIFT)) ? 1u
                                                                                             : 0u;
   p.tau = g->tau;
-  p.mapping = tg::choose_mapping(g->dtype, b.tdim, g->gdim, b.k, p.nq, tile, options);
+  p.mapping = tg::choose_mapping(g->dtype, b.tdim, g->gdim, b.k, p.nq, tile, options, p.mask);
   p.xcd_remap = tg::choose_xcd_remap();
   const bool relative = (options & TG_OPT_POINTS_RELATIVE) != 0;
   for (int i = 0; i < 3; ++i)

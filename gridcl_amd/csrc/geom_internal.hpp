@@ -29,9 +29,10 @@ enum : uint32_t { kMapCell = 0, kMapCellStaged = 1, kMapTile = 2 };
 // Cells one 256-lane workgroup covers under a mapping.
 inline uint32_t cells_per_workgroup(uint32_t mapping) { return mapping == kMapTile ? 64u : 256u; }
 
-// The mapping for (dtype, tdim, gdim, k, nq, tile) and the call's options (tuning.cpp).
+// The mapping for (dtype, tdim, gdim, k, nq, tile), the call's options and the factors it
+// writes (tuning.cpp).
 uint32_t choose_mapping(int dtype, int tdim, int gdim, uint32_t k, uint32_t nq, uint32_t tile,
-                        uint32_t options);
+                        uint32_t options, uint32_t factors);
 // Whether per-block launches remap workgroups to XCD-contiguous ranges (default 1;
 // TETRALOMA_GEOM_XCD_REMAP=0 turns it off).
 uint32_t choose_xcd_remap();

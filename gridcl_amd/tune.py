@@ -1,9 +1,11 @@
 """Autotuner of the factor kernels' lane mapping: ``python -m gridcl_amd.tune [--out FILE]``.
 
-For every workload of TUNE_SET (precision, element, point count, mesh size) it evaluates X, J
-and det J with each mapping (one lane per cell storing from registers; one lane per cell with
-LDS-staged 16-byte stores; one wavefront per 16-cell tile), timing the mappings interleaved over
-several rounds with HIP events, keeps the fastest median, and writes the table that
+For every workload of TUNE_SET (precision, element, point count, mesh size) and output class it
+evaluates, with each mapping (one lane per cell storing from registers; one lane per cell with
+LDS-staged 16-byte stores; one wavefront per 16-cell tile), X, J and det J in one call (class 0:
+more than three components per point) or det J alone plus X alone (class 1: at most three),
+timing the mappings interleaved over several rounds with HIP events, keeps the fastest median,
+and writes the table that
 csrc/tuning.cpp compiles in (csrc/tuning_table.inc), together with a JSON record of every
 timing. The reference derives its strategy policy offline from measurements the same way
 (tetraloma-bench decide*.rs -> degree-policy.toml).
@@ -82,29 +84,33 @@ def tune(rounds: int = 5, reps: int = 10, quick: bool = False, log=print):
         ps = (api.PointSet.gauss(cell, rule[1]) if rule[0] == "gauss"
               else (api.PointSet.tet_keast4() if rule[1] == 0 else api.PointSet.tet_14()))
         td = M.TDIM[cell]
-        names = ["points", "jacobians", "detj"]
+        classes = ((0, [["points", "jacobians", "detj"]]), (1, [["detj"], ["points"]]))
         k = mesh.blocks[0].nodes_per_cell
         for dtype, tdt, code in (("f32", torch.float32, 0), ("f64", torch.float64, 1)):
             g = api.geometry_from_host(mesh, dtype=tdt)
             C = mesh.blocks[0].num_cells
             out = {nm: torch.empty(int(api._lib.load().tg_field_elems(
                 C, len(ps), api.factor_len(nm, td, gdim), 16)), dtype=tdt, device="cuda")
-                for nm in names}
-            times = {m: [] for m in MAPPINGS}
-            for _ in range(rounds):
-                for m, opt in OPTS.items():
+                for nm in ("points", "jacobians", "detj")}
+            for cls, calls in classes:
+                def run(opt):
                     # The dense kernels' mapping is what is tuned: the sum-factorised
                     # hexahedron kernel (which ignores it) is switched off.
-                    times[m].append(_time(lambda: api.geometric_factors_block(
-                        g, 0, ps, names, options=opt | api.OPT_NO_SUMFACT, out=out), reps, 2))
-            med = {m: float(np.median(v)) for m, v in times.items()}
-            best = min(med, key=med.get)
-            rec = {"cell": cell, "degree": p, "gdim": gdim, "tdim": td, "k": k, "nq": len(ps),
-                   "dtype": dtype, "cells": C, "ms": {MAPPINGS[m]: med[m] for m in med},
-                   "best": MAPPINGS[best]}
-            log(json.dumps(rec))
-            records.append(rec)
-            table.append((code, td, gdim, k, len(ps), best))
+                    for names in calls:
+                        api.geometric_factors_block(g, 0, ps, names, options=opt | api.OPT_NO_SUMFACT,
+                                                    out={nm: out[nm] for nm in names})
+                times = {m: [] for m in MAPPINGS}
+                for _ in range(rounds):
+                    for m, opt in OPTS.items():
+                        times[m].append(_time(lambda: run(opt), reps, 2))
+                med = {m: float(np.median(v)) for m, v in times.items()}
+                best = min(med, key=med.get)
+                rec = {"cell": cell, "degree": p, "gdim": gdim, "tdim": td, "k": k, "nq": len(ps),
+                       "dtype": dtype, "class": cls, "cells": C,
+                       "ms": {MAPPINGS[m]: med[m] for m in med}, "best": MAPPINGS[best]}
+                log(json.dumps(rec))
+                records.append(rec)
+                table.append((code, td, gdim, k, len(ps), cls, best))
             del g, out
     return records, table
 
@@ -113,12 +119,12 @@ def write_table(table, path: str) -> None:
     """Entries cover each tuned point count exactly; untuned shapes fall back to kMapTile."""
     lines = [
         "// Generated by gridcl_amd/tune.py; edit by re-running it. Entries: {dtype (0 FP32, 1 FP64),",
-        "// tdim, gdim, nodes per cell, points low, points high, mapping}.",
+        "// tdim, gdim, nodes per cell, points low, points high, output class, mapping}.",
         "constexpr uint32_t kDefaultMapping = kMapTile;",
         "const TuneEntry kTuned[] = {",
     ]
-    for code, td, gd, k, nq, best in table:
-        lines.append(f"    {{{code}, {td}, {gd}, {k}, {nq}, {nq}, {MAPPINGS[best]}}},")
+    for code, td, gd, k, nq, cls, best in table:
+        lines.append(f"    {{{code}, {td}, {gd}, {k}, {nq}, {nq}, {cls}, {MAPPINGS[best]}}},")
     lines.append("};")
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
