@@ -2,6 +2,7 @@
 // (tetraloma-geometry/src/equispaced.rs:158, equispaced_nodes).
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <memory>
 #include <string>
 #include <vector>
@@ -12,24 +13,56 @@
 
 namespace {
 
-// The equispaced lattice of (cell, degree) as a point set (point j = local node j in A.5 order),
-// kept with the geometry so that its device tables outlive the launches that read them.
-int lattice(tg_geometry* g, const tg::Block& b, tg_pointset** out) {
+// The equispaced lattice of (cell, degree) as a point set, kept with the geometry so that its
+// device tables outlive the launches that read them, and the lattice point of each local node.
+// Tensor cells list the lattice lexicographically (first coordinate fastest), so that hexahedra
+// take the sum-factorised kernel; simplices list it in A.5 order.
+int lattice(tg_geometry* g, const tg::Block& b, tg_pointset** out, const uint32_t** point_of) {
   const auto key = std::make_pair(static_cast<int>(b.cell), b.degree);
   auto hit = g->lattices.find(key);
   if (hit != g->lattices.end()) {
     *out = hit->second.get();
+    *point_of = g->lattice_point_of.at(key);
     return TG_OK;
   }
   auto p = std::make_unique<tg_pointset>();
   p->cell = b.cell;
   p->tdim = tg::tdim(b.cell);
-  if (!tg::lagrange_points(b.cell, b.degree, tg::kEquispaced, p->points))
+  std::vector<double> a5;
+  if (!tg::lagrange_points(b.cell, b.degree, tg::kEquispaced, a5))
     return tg::fail(TG_ERR_UNSUPPORTED_DEGREE, "UnsupportedDegree: no equispaced lattice");
-  p->nq = p->points.size() / p->tdim;
+  const int td = p->tdim;
+  const size_t k = a5.size() / td;
+  std::vector<uint32_t> at(k);
+  const bool tensor = b.cell == tg::kInterval || b.cell == tg::kQuadrilateral ||
+                      b.cell == tg::kHexahedron;
+  if (tensor) {
+    const unsigned n = b.degree + 1;
+    p->points.assign(k * td, 0.0);
+    for (size_t j = 0; j < k; ++j) {
+      uint32_t lex = 0;
+      for (int d = td; d-- > 0;)
+        lex = lex * n + static_cast<uint32_t>(std::lround(a5[j * td + d] * b.degree));
+      at[j] = lex;
+      for (int d = 0; d < td; ++d) p->points[lex * td + d] = a5[j * td + d];
+    }
+  } else {
+    p->points = a5;
+    for (size_t j = 0; j < k; ++j) at[j] = static_cast<uint32_t>(j);
+  }
+  p->nq = k;
   p->weights.assign(p->nq, 1.0);
+  uint32_t* d = nullptr;
+  hipError_t e = hipMalloc(&d, k * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMemcpy(d, at.data(), k * sizeof(uint32_t), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    if (d) (void)hipFree(d);
+    return tg::hip_fail(e, "equispaced lattice order");
+  }
   *out = p.get();
+  *point_of = d;
   g->lattices.emplace(key, std::move(p));
+  g->lattice_point_of.emplace(key, d);
   return TG_OK;
 }
 
@@ -44,7 +77,8 @@ int convert(tg_geometry* g, const std::vector<Occurrences>& occ, uint32_t* first
   for (const Occurrences& o : occ) {
     const tg::Block& b = g->blocks[o.block];
     tg_pointset* pts = nullptr;
-    int rc = lattice(g, b, &pts);
+    const uint32_t* point_of = nullptr;
+    int rc = lattice(g, b, &pts, &point_of);
     if (rc) return rc;
     tg::DeviceTables* tabs = nullptr;
     rc = tg::device_tables(pts, b.degree, b.family, g->dtype, &tabs);
@@ -59,8 +93,8 @@ int convert(tg_geometry* g, const std::vector<Occurrences>& occ, uint32_t* first
     rc = tg::enqueue_block<T>(g, b, pts, tabs, TG_POINTS, 16, TG_OPT_POINTS_RELATIVE, &outs, s);
     if (rc == TG_OK) {
       e = tg::launch_select_first<T>(g->gdim, b.dofmap, o.len, static_cast<uint32_t>(o.start),
-                                     b.k, static_cast<const T*>(field), first, g->num_nodes,
-                                     static_cast<T*>(out), s);
+                                     b.k, point_of, static_cast<const T*>(field), first,
+                                     g->num_nodes, static_cast<T*>(out), s);
       if (e != hipSuccess) rc = tg::hip_fail(e, "equispaced conversion kernel launch");
     }
     const hipError_t ef = hipFreeAsync(field, s);

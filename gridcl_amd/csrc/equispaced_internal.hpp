@@ -12,11 +12,12 @@ hipError_t launch_first_init(uint32_t* first, uint64_t num_nodes, hipStream_t st
 // Records the block's occurrences start + i (i < len, node dofmap[i]) in first.
 hipError_t launch_first_occurrence(const uint32_t* dofmap, uint64_t len, uint32_t start,
                                    uint64_t num_nodes, uint32_t* first, hipStream_t stream);
-// Writes out[node] from the field (X at the block's lattice, tiles of 16, gdim components) for
-// the nodes whose first occurrence is in this block.
+// Writes out[node] from the field (X at the block's lattice, tiles of 16, gdim components; local
+// node j at lattice point point_of[j]) for the nodes whose first occurrence is in this block.
 template <typename T>
 hipError_t launch_select_first(int gdim, const uint32_t* dofmap, uint64_t len, uint32_t start,
-                               uint32_t k, const T* field, const uint32_t* first,
-                               uint64_t num_nodes, T* out, hipStream_t stream);
+                               uint32_t k, const uint32_t* point_of, const T* field,
+                               const uint32_t* first, uint64_t num_nodes, T* out,
+                               hipStream_t stream);
 
 }  // namespace tg

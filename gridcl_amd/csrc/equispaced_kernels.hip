@@ -34,13 +34,13 @@ __global__ void __launch_bounds__(kBlock)
 template <typename T, int GD>
 __global__ void __launch_bounds__(kBlock)
     select_first(const uint32_t* __restrict__ dofmap, uint64_t len, uint32_t start, uint32_t k,
-                 const T* __restrict__ field, const uint32_t* __restrict__ first,
-                 uint64_t num_nodes, T* __restrict__ out) {
+                 const uint32_t* __restrict__ point_of, const T* __restrict__ field,
+                 const uint32_t* __restrict__ first, uint64_t num_nodes, T* __restrict__ out) {
   const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
   if (i >= len) return;
   const uint32_t node = dofmap[i];
   if (node >= num_nodes || first[node] != start + static_cast<uint32_t>(i)) return;
-  const uint64_t c = i / k, q = i - c * k;
+  const uint64_t c = i / k, q = point_of[i - c * k];
 #pragma unroll
   for (int g = 0; g < GD; ++g)
     out[static_cast<uint64_t>(node) * GD + g] = field[(((c >> 4) * k + q) * GD + g) * 16 + (c & 15u)];
@@ -67,21 +67,22 @@ hipError_t launch_first_occurrence(const uint32_t* dofmap, uint64_t len, uint32_
 
 template <typename T>
 hipError_t launch_select_first(int gdim, const uint32_t* dofmap, uint64_t len, uint32_t start,
-                               uint32_t k, const T* field, const uint32_t* first,
-                               uint64_t num_nodes, T* out, hipStream_t stream) {
+                               uint32_t k, const uint32_t* point_of, const T* field,
+                               const uint32_t* first, uint64_t num_nodes, T* out,
+                               hipStream_t stream) {
   if (len == 0) return hipSuccess;
   switch (gdim) {
     case 1:
       hipLaunchKernelGGL((select_first<T, 1>), dim3(grid_of(len)), dim3(kBlock), 0, stream,
-                         dofmap, len, start, k, field, first, num_nodes, out);
+                         dofmap, len, start, k, point_of, field, first, num_nodes, out);
       break;
     case 2:
       hipLaunchKernelGGL((select_first<T, 2>), dim3(grid_of(len)), dim3(kBlock), 0, stream,
-                         dofmap, len, start, k, field, first, num_nodes, out);
+                         dofmap, len, start, k, point_of, field, first, num_nodes, out);
       break;
     case 3:
       hipLaunchKernelGGL((select_first<T, 3>), dim3(grid_of(len)), dim3(kBlock), 0, stream,
-                         dofmap, len, start, k, field, first, num_nodes, out);
+                         dofmap, len, start, k, point_of, field, first, num_nodes, out);
       break;
     default:
       return hipErrorInvalidValue;
@@ -90,10 +91,10 @@ hipError_t launch_select_first(int gdim, const uint32_t* dofmap, uint64_t len, u
 }
 
 template hipError_t launch_select_first<float>(int, const uint32_t*, uint64_t, uint32_t,
-                                               uint32_t, const float*, const uint32_t*, uint64_t,
-                                               float*, hipStream_t);
+                                               uint32_t, const uint32_t*, const float*,
+                                               const uint32_t*, uint64_t, float*, hipStream_t);
 template hipError_t launch_select_first<double>(int, const uint32_t*, uint64_t, uint32_t,
-                                                uint32_t, const double*, const uint32_t*,
-                                                uint64_t, double*, hipStream_t);
+                                                uint32_t, const uint32_t*, const double*,
+                                                const uint32_t*, uint64_t, double*, hipStream_t);
 
 }  // namespace tg

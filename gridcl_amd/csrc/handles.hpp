@@ -70,10 +70,13 @@ struct tg_geometry {
   double tau = 1e-2;
   // The pull-back's coefficient matrices per (cell, degree, family), uploaded on first use.
   std::map<std::tuple<int, unsigned, int>, void*> pull_back_coef;
-  // The equispaced lattices per (cell, degree) of the GLL conversion, made on first use.
+  // The equispaced lattices per (cell, degree) of the GLL conversion, made on first use, and
+  // the lattice point of each local node (device, k entries).
   std::map<std::pair<int, unsigned>, std::unique_ptr<tg_pointset>> lattices;
+  std::map<std::pair<int, unsigned>, uint32_t*> lattice_point_of;
   ~tg_geometry() {
     for (auto& kv : pull_back_coef) (void)hipFree(kv.second);
+    for (auto& kv : lattice_point_of) (void)hipFree(kv.second);
   }
 };
 

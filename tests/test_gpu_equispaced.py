@@ -3,7 +3,7 @@
 Pinned by: the same geometry (GLL and equispaced nodes of one degree interpolate the same
 polynomial map, so the converted nodes read as an equispaced block give the GLL block's X at any
 point, FP64 1e-13); the first-occurrence rule, repeated on the host from the GPU's own X at the
-lattice (bitwise); the FP64 oracle's GLL basis at the equispaced lattice (1e-13); equispaced
+lattice, listed lexicographically like the conversion does (bitwise); the FP64 oracle's GLL basis at the equispaced lattice (1e-13); equispaced
 blocks and unused nodes copied unchanged (bitwise); the same result on every run."""
 import numpy as np
 import pytest
@@ -61,12 +61,17 @@ def test_equispaced_conversion(cuda, cell, p, gdim, dtype):
     again = api.equispaced_nodes(g)
     torch.cuda.synchronize()
     assert torch.equal(out, again)
-    # The first-occurrence rule from the GPU's own X at the lattice, relative to the origin.
-    lat = api.PointSet.custom(cell, M.reference_nodes(cell, p, EQ),
-                              np.ones(len(M.reference_nodes(cell, p, EQ))))
+    # The first-occurrence rule from the GPU's own X at the lattice, relative to the origin. The
+    # conversion lists a tensor cell's lattice lexicographically (first coordinate fastest, so
+    # that hexahedra take the sum-factorised kernel): X at that point set, read in A.5 order.
+    a5 = M.reference_nodes(cell, p, EQ)
+    lex = np.rint(a5 * p).astype(int) @ (p + 1) ** np.arange(a5.shape[1])
+    pts = np.empty_like(a5)
+    pts[lex] = a5
+    lat = api.PointSet.custom(cell, pts, np.ones(len(pts)))
     f = api.geometric_factors_block(g, 0, lat, ["points"], options=api.OPT_POINTS_RELATIVE)
     torch.cuda.synchronize()
-    xl = f["points"].to_host_cell_major()  # [c][k][gdim]
+    xl = f["points"].to_host_cell_major()[:, lex]  # [c][k][gdim], A.5 order
     want = g.nodes.cpu().numpy().copy()
     seen = set()
     for c, row in enumerate(mesh.blocks[0].cells):
