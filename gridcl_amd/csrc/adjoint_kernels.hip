@@ -138,7 +138,7 @@ __global__ void __launch_bounds__(kBlock)
                          const T* __restrict__ tab, const double* __restrict__ wts, uint32_t nq,
                          uint64_t num_cells, uint32_t last_node, double beta,
                          const uint32_t* __restrict__ slot, T* __restrict__ contrib) {
-  __shared__ T lds[kBlock / 64][64 * 3];  // per wavefront: one local node of its 64 cells
+  __shared__ T lds[kBlock / 128][K * 3 * 64];  // per odd wavefront: all K x 3 sums of its 64 cells
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), half = wave & 1u,
                  w = threadIdx.x & 63u;
   const uint64_t c0 = static_cast<uint64_t>(blockIdx.x) * kCells + (wave >> 1) * 64;
@@ -192,22 +192,23 @@ __global__ void __launch_bounds__(kBlock)
     }
   }
   // The cell's contributions (even points + odd points) go to their CSR positions: the odd
-  // wavefront stages its part in LDS, the even one adds and stores each pair's three values.
+  // wavefront stages all of its sums in LDS, the even one adds and stores each triple.
   const uint32_t* __restrict__ slots = slot + c * K;
+  T* __restrict__ pair = lds[wave >> 1];
+  if (half) {
 #pragma unroll
-  for (int k = 0; k < K; ++k) {
-    if (half) {
+    for (int k = 0; k < K; ++k)
 #pragma unroll
-      for (int g = 0; g < 3; ++g) lds[wave][w * 3 + g] = ct[k][g];
-    }
-    __syncthreads();
-    if (!half && cl < num_cells) {
-      const T* odd = lds[wave | 1u];
+      for (int g = 0; g < 3; ++g) pair[(k * 3 + g) * 64 + w] = ct[k][g];
+  }
+  __syncthreads();
+  if (!half && cl < num_cells) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
       T* dst = contrib + static_cast<uint64_t>(slots[k]) * 3;
 #pragma unroll
-      for (int g = 0; g < 3; ++g) dst[g] = ct[k][g] + odd[w * 3 + g];
+      for (int g = 0; g < 3; ++g) dst[g] = ct[k][g] + pair[(k * 3 + g) * 64 + w];
     }
-    __syncthreads();
   }
 }
 
