@@ -263,10 +263,11 @@ static bool sumfact_pays(bool f32, int n1, int m) {
   return f32 ? (m >= 4 || (n1 == 3 && m == 3)) : n1 >= 3;
 }
 
+// The scalars of one factor launch over block b.
 template <typename T>
-int enqueue_block(const tg_geometry* g, const Block& b, tg_pointset* pts, DeviceTables* tabs,
-                  uint32_t factors, uint32_t tile, uint32_t options, const tg_outputs* out,
-                  hipStream_t stream) {
+static tg::FactorParams<T> block_params(const tg_geometry* g, const Block& b,
+                                        const tg_pointset* pts, uint32_t factors, uint32_t tile,
+                                        uint32_t options) {
   tg::FactorParams<T> p{};
   p.num_cells = b.num_cells;
   p.padded_cells = (b.num_cells + tile - 1) / tile * tile;
@@ -284,13 +285,29 @@ int enqueue_block(const tg_geometry* g, const Block& b, tg_pointset* pts, Device
   const bool relative = (options & TG_OPT_POINTS_RELATIVE) != 0;
   for (int i = 0; i < 3; ++i)
     p.origin[i] = static_cast<T>(i < g->gdim && !relative ? g->origin[i] : 0.0);
+  return p;
+}
+
+// Whether the factors of a block go to the sum-factorised kernel (hexahedra of gdim 3 whose
+// factors it computes, tiles of 16, where it pays or is forced).
+static bool takes_sumfact(const tg_geometry* g, const DeviceTables* tabs, uint32_t factors,
+                          uint32_t tile, uint32_t options, bool f32) {
+  const uint32_t sf_factors = TG_POINTS | TG_JACOBIAN | TG_DETJ;
+  return tabs->sf_m > 0 && tile == 16 && g->gdim == 3 && !(factors & ~sf_factors) &&
+         !(options & (TG_OPT_FORCE_GENERIC | TG_OPT_NO_SUMFACT)) &&
+         ((options & TG_OPT_SUMFACT_ALWAYS) || sumfact_pays(f32, tabs->sf_n1, tabs->sf_m));
+}
+
+template <typename T>
+int enqueue_block(const tg_geometry* g, const Block& b, tg_pointset* pts, DeviceTables* tabs,
+                  uint32_t factors, uint32_t tile, uint32_t options, const tg_outputs* out,
+                  hipStream_t stream) {
+  const tg::FactorParams<T> p = block_params<T>(g, b, pts, factors, tile, options);
+  const bool f32 = sizeof(T) == 4;
   tg::FactorOutputs o{out->points, out->jacobians, out->detj, out->pinv, out->facet_normals,
                       out->normals};
   const bool shift = f32 && !(options & TG_OPT_NO_CELL_SHIFT);
-  const uint32_t sf_factors = TG_POINTS | TG_JACOBIAN | TG_DETJ;
-  if (tabs->sf_m > 0 && tile == 16 && g->gdim == 3 && !(factors & ~sf_factors) &&
-      !(options & (TG_OPT_FORCE_GENERIC | TG_OPT_NO_SUMFACT)) &&
-      ((options & TG_OPT_SUMFACT_ALWAYS) || sumfact_pays(f32, tabs->sf_n1, tabs->sf_m))) {
+  if (takes_sumfact(g, tabs, factors, tile, options, f32)) {
     hipError_t e = tg::launch_sumfact<T>(
         tabs->sf_n1, tabs->sf_m, shift, static_cast<const T*>(g->nodes), b.dofmap,
         static_cast<const T*>(tabs->sf_tab), tabs->sf_lex, tabs->tab64, p, o, stream);
@@ -307,10 +324,45 @@ int enqueue_block(const tg_geometry* g, const Block& b, tg_pointset* pts, Device
   return TG_OK;
 }
 
+template <typename T>
+bool fused_cell_measures(const tg_geometry* g, const Block& b, const DeviceTables* tabs,
+                         uint32_t options) {
+  const bool f32 = sizeof(T) == 4;
+  if ((options & TG_OPT_FORCE_GENERIC) || takes_sumfact(g, tabs, TG_DETJ, 16, options, f32))
+    return false;
+  return tg::has_measure_cells<T>(b.tdim, g->gdim, b.k, f32 && !(options & TG_OPT_NO_CELL_SHIFT));
+}
+
+template <typename T>
+int enqueue_cell_measures(const tg_geometry* g, const Block& b, tg_pointset* pts,
+                          DeviceTables* tabs, const double* weights, uint32_t options, T* sums,
+                          hipStream_t stream) {
+  const tg::FactorParams<T> p = block_params<T>(g, b, pts, TG_DETJ, 16, options);
+  const bool shift = sizeof(T) == 4 && !(options & TG_OPT_NO_CELL_SHIFT);
+  hipError_t e = tg::launch_measure_cells<T>(b.tdim, g->gdim, shift,
+                                             static_cast<const T*>(g->nodes), b.dofmap,
+                                             static_cast<const T*>(tabs->tab), tabs->tab64,
+                                             weights, p, sums, stream);
+  if (e == hipErrorNotSupported)
+    return fail(TG_ERR_UNSUPPORTED_DEGREE, "no measure kernel for this (tdim, gdim)");
+  if (e != hipSuccess) return hip_fail(e, "measure kernel launch");
+  return TG_OK;
+}
+
 template int enqueue_block<float>(const tg_geometry*, const Block&, tg_pointset*, DeviceTables*,
                                   uint32_t, uint32_t, uint32_t, const tg_outputs*, hipStream_t);
 template int enqueue_block<double>(const tg_geometry*, const Block&, tg_pointset*, DeviceTables*,
                                    uint32_t, uint32_t, uint32_t, const tg_outputs*, hipStream_t);
+template bool fused_cell_measures<float>(const tg_geometry*, const Block&, const DeviceTables*,
+                                         uint32_t);
+template bool fused_cell_measures<double>(const tg_geometry*, const Block&, const DeviceTables*,
+                                          uint32_t);
+template int enqueue_cell_measures<float>(const tg_geometry*, const Block&, tg_pointset*,
+                                          DeviceTables*, const double*, uint32_t, float*,
+                                          hipStream_t);
+template int enqueue_cell_measures<double>(const tg_geometry*, const Block&, tg_pointset*,
+                                           DeviceTables*, const double*, uint32_t, double*,
+                                           hipStream_t);
 
 }  // namespace tg
 

@@ -699,6 +699,59 @@ FactorFn<T> direct_kernel(int td, int gd, uint32_t k, bool shift) {
   return nullptr;
 }
 
+// A product rounded before it is added to the running sum, as the measure's cell sums read
+// (measure.rs:17-41): the first point starts the sum.
+template <typename T>
+__device__ __forceinline__ T weighted_sum(T acc, T d, T w, bool first) {
+#pragma clang fp contract(off)
+  const T t = d * w;
+  return first ? t : acc + t;
+}
+
+// The measure's cell sums straight from each point's det J (tg_measure_block): one lane per
+// cell, det J as the factor kernels compute it (detj_checked after contract), summed over the
+// points in order with the weights narrowed to T, so the sums equal those of the det J field
+// without the field being written.
+template <typename T, int TD, int GD, int K, bool SHIFT>
+__global__ void __launch_bounds__(kBlock)
+    measure_cells(const T* __restrict__ nodes, const uint32_t* __restrict__ dofmap,
+                  const T* __restrict__ tab, const double* __restrict__ tab64,
+                  const double* __restrict__ weights, FactorParams<T> p, T* __restrict__ sums) {
+  // Each XCD takes one contiguous range of cells, so nodes shared by neighbouring cells are
+  // fetched into one L2.
+  const uint32_t b = p.xcd_remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
+  const uint64_t c = static_cast<uint64_t>(b) * kBlock + threadIdx.x;
+  if (c >= p.num_cells) return;
+  const uint32_t* __restrict__ row = dofmap + c * K;
+  T x[K][GD];
+  gather_cell<T, GD, K, SHIFT>(nodes, row, p.last_node, x);
+  T acc = T(0);
+  for (uint32_t q = 0; q < p.nq; ++q) {
+    T xv[GD], j[TD * GD];
+    contract<T, TD, GD, K, SHIFT>(x, tab + static_cast<uint64_t>(q) * K * 4, false, true, xv, j);
+    const T d = detj_checked<T, TD, GD, K>(j, p, nodes, row, tab64, q);
+    acc = weighted_sum<T>(acc, d, static_cast<T>(weights[q]), q == 0);
+  }
+  sums[c] = acc;
+}
+
+template <typename T>
+using MeasureFn = void (*)(const T*, const uint32_t*, const T*, const double*, const double*,
+                           FactorParams<T>, T*);
+
+template <typename T>
+MeasureFn<T> measure_kernel(int td, int gd, uint32_t k, bool shift) {
+  if (sizeof(T) == 8 && k > 27) return nullptr;
+  if (sizeof(T) == 4 && !shift) return nullptr;  // as direct_kernel: F32 without the shift is generic
+#define TG_CASE(TD_, GD_, K_)                                                      \
+  if (td == TD_ && gd == GD_ && k == K_)                                           \
+    return shift ? static_cast<MeasureFn<T>>(measure_cells<T, TD_, GD_, K_, true>) \
+                 : static_cast<MeasureFn<T>>(measure_cells<T, TD_, GD_, K_, false>);
+  TG_DIRECT_SHAPES(TG_CASE)
+#undef TG_CASE
+  return nullptr;
+}
+
 template <typename T>
 FactorFn<T> generic_kernel(int td, int gd, bool shift) {
 #define TG_G(TD_, GD_)                                                              \
@@ -737,6 +790,26 @@ hipError_t launch_factor(int tdim, int gdim, bool shift, bool generic, const T* 
   hipLaunchKernelGGL(fn, dim3(static_cast<uint32_t>(blocks)), dim3(kBlock),
                      static_cast<uint32_t>(lds), stream, nodes, dofmap, flags, tab, tab64,
                      ref_normals, p, out);
+  return hipGetLastError();
+}
+
+template <typename T>
+bool has_measure_cells(int tdim, int gdim, uint32_t k, bool shift) {
+  return measure_kernel<T>(tdim, gdim, k, shift) != nullptr;
+}
+
+template <typename T>
+hipError_t launch_measure_cells(int tdim, int gdim, bool shift, const T* nodes,
+                                const uint32_t* dofmap, const T* tab, const double* tab64,
+                                const double* weights, const FactorParams<T>& p, T* sums,
+                                hipStream_t stream) {
+  MeasureFn<T> fn = measure_kernel<T>(tdim, gdim, p.k, shift);
+  if (fn == nullptr) return hipErrorNotSupported;
+  const uint64_t blocks = (p.num_cells + kBlock - 1) / kBlock;
+  if (blocks == 0) return hipSuccess;
+  if (blocks > 0x7fffffffu) return hipErrorInvalidConfiguration;
+  hipLaunchKernelGGL(fn, dim3(static_cast<uint32_t>(blocks)), dim3(kBlock), 0, stream, nodes,
+                     dofmap, tab, tab64, weights, p, sums);
   return hipGetLastError();
 }
 
@@ -812,5 +885,9 @@ template hipError_t launch_factor<FT>(int, int, bool, bool, const FT*, const uin
                                       const FactorParams<FT>&, const FactorOutputs&, hipStream_t);
 template hipError_t launch_evector<FT>(const FT*, const uint32_t*, uint64_t, uint64_t, uint32_t,
                                        int, uint32_t, FT*, hipStream_t);
+template bool has_measure_cells<FT>(int, int, uint32_t, bool);
+template hipError_t launch_measure_cells<FT>(int, int, bool, const FT*, const uint32_t*, const FT*,
+                                             const double*, const double*,
+                                             const FactorParams<FT>&, FT*, hipStream_t);
 
 }  // namespace tg

@@ -126,4 +126,16 @@ hipError_t launch_evector(const T* nodes, const uint32_t* dofmap, uint64_t num_c
                           uint64_t padded, uint32_t k, int gdim, uint32_t last_node, T* out,
                           hipStream_t stream);
 
+// Whether launch_measure_cells has a kernel for (tdim, gdim, k, shift).
+template <typename T>
+bool has_measure_cells(int tdim, int gdim, uint32_t k, bool shift);
+
+// The measure's per-cell sums sum_q det J(c, q) w_q of cells [0, p.num_cells) straight from the
+// register kernel's det J (no field); hipErrorNotSupported for shapes without one.
+template <typename T>
+hipError_t launch_measure_cells(int tdim, int gdim, bool shift, const T* nodes,
+                                const uint32_t* dofmap, const T* tab, const double* tab64,
+                                const double* weights, const FactorParams<T>& p, T* sums,
+                                hipStream_t stream);
+
 }  // namespace tg

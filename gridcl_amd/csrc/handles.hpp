@@ -92,4 +92,16 @@ template <typename T>
 int enqueue_block(const tg_geometry* g, const Block& b, tg_pointset* pts, DeviceTables* tabs,
                   uint32_t factors, uint32_t tile, uint32_t options, const struct tg_outputs* out,
                   hipStream_t stream);
+// Whether enqueue_cell_measures applies to block b: enqueue_block would compute its det J alone
+// with a register-resident kernel (not the sum-factorised or runtime-K one).
+template <typename T>
+bool fused_cell_measures(const tg_geometry* g, const Block& b, const DeviceTables* tabs,
+                         uint32_t options);
+// Enqueues sums[c] = sum_q det J(c, q) w_q (w narrowed to T, each product rounded, q ascending)
+// for every cell of block b straight from the register-resident kernel's det J, without a det J
+// field. Returns a TG status.
+template <typename T>
+int enqueue_cell_measures(const tg_geometry* g, const Block& b, tg_pointset* pts,
+                          DeviceTables* tabs, const double* weights, uint32_t options, T* sums,
+                          hipStream_t stream);
 }  // namespace tg

@@ -12,9 +12,26 @@ namespace {
 template <typename T>
 int measure(const tg_geometry* g, const tg::Block& b, tg_pointset* pts, tg::DeviceTables* tabs,
             const double* weights, uint32_t options, double* value, hipStream_t s) {
-  // Work arrays, stream-ordered: the det J field (tiles of 16) and the sums behind it.
+  // Where the register-resident kernel computes det J, one kernel sums each cell's points
+  // straight from it (no det J field); the block sums follow.
+  const uint64_t sums = tg::measure_scratch_elems(b.num_cells);
+  if (tg::fused_cell_measures<T>(g, b, tabs, options)) {
+    void* work = nullptr;
+    hipError_t e = hipMallocAsync(&work, sums * sizeof(T), s);
+    if (e != hipSuccess) return tg::hip_fail(e, "measure work arrays");
+    int rc =
+        tg::enqueue_cell_measures<T>(g, b, pts, tabs, weights, options, static_cast<T*>(work), s);
+    if (rc == TG_OK) {
+      e = tg::launch_measure_sums<T>(b.num_cells, static_cast<T*>(work), value, s);
+      if (e != hipSuccess) rc = tg::hip_fail(e, "measure kernel launch");
+    }
+    const hipError_t ef = hipFreeAsync(work, s);
+    if (rc == TG_OK && ef != hipSuccess) rc = tg::hip_fail(ef, "measure work arrays");
+    return rc;
+  }
+  // Elsewhere: the det J field (tiles of 16) and the sums behind it, stream-ordered.
   const uint64_t field = (b.num_cells + 15) / 16 * 16 * pts->nq;
-  const uint64_t elems = field + tg::measure_scratch_elems(b.num_cells);
+  const uint64_t elems = field + sums;
   void* work = nullptr;
   hipError_t e = hipMallocAsync(&work, elems * sizeof(T), s);
   if (e != hipSuccess) return tg::hip_fail(e, "measure work arrays");

@@ -18,4 +18,10 @@ template <typename T>
 hipError_t launch_measure(const T* detj, const double* weights, uint32_t nq, uint64_t num_cells,
                           T* scratch, double* value, hipStream_t stream);
 
+// *value (device) = the FP64 widening of the fixed-order sum of the per-cell sums in
+// scratch[0, num_cells) (num_cells > 0); scratch holds measure_scratch_elems(num_cells) elements
+// and is overwritten.
+template <typename T>
+hipError_t launch_measure_sums(uint64_t num_cells, T* scratch, double* value, hipStream_t stream);
+
 }  // namespace tg

@@ -67,12 +67,21 @@ hipError_t launch_measure(const T* detj, const double* weights, uint32_t nq, uin
     return hipGetLastError();
   }
   if ((num_cells + kBlock - 1) / kBlock > 0x7fffffffu) return hipErrorInvalidConfiguration;
+  hipLaunchKernelGGL(cell_sums<T>, dim3(grid_of(num_cells)), dim3(kBlock), 0, stream, detj,
+                     weights, nq, num_cells, scratch);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  return launch_measure_sums<T>(num_cells, scratch, value, stream);
+}
+
+template <typename T>
+hipError_t launch_measure_sums(uint64_t num_cells, T* scratch, double* value, hipStream_t stream) {
+  if (num_cells == 0 || (num_cells + kBlock - 1) / kBlock > 0x7fffffffu)
+    return hipErrorInvalidConfiguration;
   // Ping-pong between the cell sums [num_cells] and the first pass's outputs behind them.
   T* a = scratch;
   T* b = scratch + num_cells;
-  hipLaunchKernelGGL(cell_sums<T>, dim3(grid_of(num_cells)), dim3(kBlock), 0, stream, detj,
-                     weights, nq, num_cells, a);
-  hipError_t e = hipGetLastError();
+  hipError_t e = hipSuccess;
   uint64_t len = num_cells;
   while (e == hipSuccess && len > 1) {
     const uint64_t num_out = (len + kSumBlock - 1) / kSumBlock;
@@ -93,5 +102,7 @@ template hipError_t launch_measure<float>(const float*, const double*, uint32_t,
                                           double*, hipStream_t);
 template hipError_t launch_measure<double>(const double*, const double*, uint32_t, uint64_t,
                                            double*, double*, hipStream_t);
+template hipError_t launch_measure_sums<float>(uint64_t, float*, double*, hipStream_t);
+template hipError_t launch_measure_sums<double>(uint64_t, double*, double*, hipStream_t);
 
 }  // namespace tg

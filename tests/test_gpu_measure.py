@@ -110,6 +110,40 @@ def test_measure_matches_oracle_and_host_reduction(cuda, cell, p, n, dtype):
     assert got == host_reduction(field.to_host_cell_major(), ps.weights, npdt)
 
 
+# (cell, degree, rule degree, options): the one-kernel cell sums (tetrahedra, prisms, pyramids,
+# triangles, Q1 hexahedra at a low rule), the det J field summed afterwards (sum-factorised Q2
+# hexahedra, the runtime-K kernel under FORCE_GENERIC, FP32 without the cell shift).
+PATHS = [(TET, 2, 4, 0), (TET, 3, 6, 0), (PRISM, 2, 5, 0), (PYR, 2, 4, 0), (TRI, 2, 6, 0),
+         (HEX, 1, 3, 0), (HEX, 2, 5, 0), (TET, 2, 4, api.OPT_FORCE_GENERIC),
+         (TET, 2, 4, api.OPT_NO_CELL_SHIFT), (TET, 2, 4, api.OPT_NO_FP64_FIXUP)]
+
+
+@pytest.mark.parametrize("cell,p,deg,options", PATHS)
+@pytest.mark.parametrize("dtype", ["f32", "f64"])
+def test_measure_equals_reduction_of_detj_field_on_every_path(cuda, cell, p, deg, options, dtype):
+    """Whichever kernels compute it, the measure is the reduction of the det J field that
+    tg_eval_block writes with the same options, repeated on the host in the geometry's type,
+    bit for bit (3,000-6,000 cells: two blocked passes), and near the FP64 oracle's (rel. 1e-12
+    FP64, 2e-6 FP32)."""
+    import torch
+
+    tdt = torch.float64 if dtype == "f64" else torch.float32
+    if M.TDIM[cell] == 2:
+        mesh = lifted_surface(cell, p, n=40)
+    else:
+        mesh = box_mesh(cell, p, n=16 if cell == HEX else 8)
+    g = api.geometry_from_host(mesh, dtype=tdt)
+    ps = api.PointSet.gauss(cell, deg)
+    got = api.measure_block(g, 0, ps, options)
+    field = api.geometric_factors_block(g, 0, ps, ["detj"], 16, options)["detj"]
+    torch.cuda.synchronize()
+    npdt = np.float64 if dtype == "f64" else np.float32
+    assert got == host_reduction(field.to_host_cell_major(), ps.weights, npdt)
+    want = oracle_measure(g, mesh, 0, ps)
+    tol = 1e-12 if dtype == "f64" else 2e-6
+    assert abs(got - want) <= tol * abs(want), (got, want)
+
+
 @pytest.mark.parametrize("dtype", ["f32", "f64"])
 def test_mesh_measure_mixed_mesh(cuda, dtype):
     """Mesh.measure over the five blocks of a small mixed mesh (tetrahedra, hexahedra, prisms,
