@@ -144,6 +144,34 @@ def test_measure_equals_reduction_of_detj_field_on_every_path(cuda, cell, p, deg
     assert abs(got - want) <= tol * abs(want), (got, want)
 
 
+@pytest.mark.parametrize("n,bad", [(7, False), (1, False), (3, True)])
+@pytest.mark.parametrize("dtype", ["f32", "f64"])
+def test_cell_measures_ragged_single_and_clamped(cuda, n, bad, dtype):
+    """The one-kernel cell sums of P2 tetrahedra on a ragged block (2,058 cells: the last
+    workgroup part-full), a box of six cells and a block of one, and with dofmap entries past the
+    last node (read as the last node, evector.rs:187-191): bit for bit the reduction of the det J
+    field, which clamps the same way."""
+    import torch
+
+    mesh = box_mesh(TET, 2, n)
+    if bad:
+        cells = mesh.blocks[0].cells.copy()
+        cells[5, 4] = 10_000_000
+        cells[17, 0] = 0xFFFFFFFF
+        mesh.blocks[0].cells = cells
+    tdt = torch.float64 if dtype == "f64" else torch.float32
+    npdt = np.float64 if dtype == "f64" else np.float32
+    ps = api.PointSet.tet_14()
+    b0 = mesh.blocks[0]
+    one = M.HostMesh(mesh.gdim, mesh.nodes, [M.HostBlock(b0.cell, b0.degree, b0.cells[:1])])
+    for m in ([mesh, one] if n == 1 else [mesh]):
+        g = api.geometry_from_host(m, dtype=tdt)
+        got = api.measure_block(g, 0, ps)
+        field = api.detj_block(g, 0, ps)
+        torch.cuda.synchronize()
+        assert got == host_reduction(field.to_host_cell_major(), ps.weights, npdt)
+
+
 @pytest.mark.parametrize("dtype", ["f32", "f64"])
 def test_mesh_measure_mixed_mesh(cuda, dtype):
     """Mesh.measure over the five blocks of a small mixed mesh (tetrahedra, hexahedra, prisms,
