@@ -708,31 +708,47 @@ __device__ __forceinline__ T weighted_sum(T acc, T d, T w, bool first) {
   return first ? t : acc + t;
 }
 
-// The measure's cell sums straight from each point's det J (tg_measure_block): one lane per
-// cell, det J as the factor kernels compute it (detj_checked after contract), summed over the
-// points in order with the weights narrowed to T, so the sums equal those of the det J field
-// without the field being written.
+// The measure's first blocked sums straight from each point's det J (tg_measure_block): one lane
+// per cell computes det J at its points as the factor kernels do (detj_checked after contract)
+// and sums them in point order with the weights narrowed to T (measure.rs:17-41); each 64 cells
+// of the workgroup are one block of the first pass of 64-value sums (measure.rs:46-72), which
+// one lane adds in cell order from LDS into partials[c / 64]. No det J field is written.
 template <typename T, int TD, int GD, int K, bool SHIFT>
 __global__ void __launch_bounds__(kBlock)
     measure_cells(const T* __restrict__ nodes, const uint32_t* __restrict__ dofmap,
                   const T* __restrict__ tab, const double* __restrict__ tab64,
-                  const double* __restrict__ weights, FactorParams<T> p, T* __restrict__ sums) {
+                  const double* __restrict__ weights, FactorParams<T> p, T* __restrict__ partials) {
   // Each XCD takes one contiguous range of cells, so nodes shared by neighbouring cells are
   // fetched into one L2.
+  constexpr uint32_t kRow = 65;  // LDS row of one block of 64 sums, padded against conflicts
+  __shared__ T cell_sum[(kBlock / 64) * kRow];
   const uint32_t b = p.xcd_remap ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x;
   const uint64_t c = static_cast<uint64_t>(b) * kBlock + threadIdx.x;
-  if (c >= p.num_cells) return;
-  const uint32_t* __restrict__ row = dofmap + c * K;
-  T x[K][GD];
-  gather_cell<T, GD, K, SHIFT>(nodes, row, p.last_node, x);
   T acc = T(0);
-  for (uint32_t q = 0; q < p.nq; ++q) {
-    T xv[GD], j[TD * GD];
-    contract<T, TD, GD, K, SHIFT>(x, tab + static_cast<uint64_t>(q) * K * 4, false, true, xv, j);
-    const T d = detj_checked<T, TD, GD, K>(j, p, nodes, row, tab64, q);
-    acc = weighted_sum<T>(acc, d, static_cast<T>(weights[q]), q == 0);
+  if (c < p.num_cells) {
+    const uint32_t* __restrict__ row = dofmap + c * K;
+    T x[K][GD];
+    gather_cell<T, GD, K, SHIFT>(nodes, row, p.last_node, x);
+    for (uint32_t q = 0; q < p.nq; ++q) {
+      T xv[GD], j[TD * GD];
+      contract<T, TD, GD, K, SHIFT>(x, tab + static_cast<uint64_t>(q) * K * 4, false, true, xv, j);
+      const T d = detj_checked<T, TD, GD, K>(j, p, nodes, row, tab64, q);
+      acc = weighted_sum<T>(acc, d, static_cast<T>(weights[q]), q == 0);
+    }
   }
-  sums[c] = acc;
+  cell_sum[(threadIdx.x >> 6) * kRow + (threadIdx.x & 63u)] = acc;
+  __syncthreads();
+  if (threadIdx.x < kBlock / 64) {
+    const uint64_t first = static_cast<uint64_t>(b) * kBlock + threadIdx.x * 64u;
+    if (first < p.num_cells) {
+      const uint64_t left = p.num_cells - first;
+      const uint32_t count = left < 64 ? static_cast<uint32_t>(left) : 64u;
+      const T* __restrict__ row = cell_sum + threadIdx.x * kRow;
+      T sum = row[0];
+      for (uint32_t l = 1; l < count; ++l) sum += row[l];
+      partials[first >> 6] = sum;
+    }
+  }
 }
 
 template <typename T>
@@ -801,7 +817,7 @@ bool has_measure_cells(int tdim, int gdim, uint32_t k, bool shift) {
 template <typename T>
 hipError_t launch_measure_cells(int tdim, int gdim, bool shift, const T* nodes,
                                 const uint32_t* dofmap, const T* tab, const double* tab64,
-                                const double* weights, const FactorParams<T>& p, T* sums,
+                                const double* weights, const FactorParams<T>& p, T* partials,
                                 hipStream_t stream) {
   MeasureFn<T> fn = measure_kernel<T>(tdim, gdim, p.k, shift);
   if (fn == nullptr) return hipErrorNotSupported;
@@ -809,7 +825,7 @@ hipError_t launch_measure_cells(int tdim, int gdim, bool shift, const T* nodes,
   if (blocks == 0) return hipSuccess;
   if (blocks > 0x7fffffffu) return hipErrorInvalidConfiguration;
   hipLaunchKernelGGL(fn, dim3(static_cast<uint32_t>(blocks)), dim3(kBlock), 0, stream, nodes,
-                     dofmap, tab, tab64, weights, p, sums);
+                     dofmap, tab, tab64, weights, p, partials);
   return hipGetLastError();
 }
 

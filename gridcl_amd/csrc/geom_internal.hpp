@@ -130,12 +130,13 @@ hipError_t launch_evector(const T* nodes, const uint32_t* dofmap, uint64_t num_c
 template <typename T>
 bool has_measure_cells(int tdim, int gdim, uint32_t k, bool shift);
 
-// The measure's per-cell sums sum_q det J(c, q) w_q of cells [0, p.num_cells) straight from the
+// The first pass of the measure's 64-value sums, partials[i] = sum of the cell sums
+// sum_q det J(c, q) w_q of cells 64 i .. 64 i + 63 (below p.num_cells), straight from the
 // register kernel's det J (no field); hipErrorNotSupported for shapes without one.
 template <typename T>
 hipError_t launch_measure_cells(int tdim, int gdim, bool shift, const T* nodes,
                                 const uint32_t* dofmap, const T* tab, const double* tab64,
-                                const double* weights, const FactorParams<T>& p, T* sums,
+                                const double* weights, const FactorParams<T>& p, T* partials,
                                 hipStream_t stream);
 
 }  // namespace tg

@@ -97,9 +97,10 @@ int enqueue_block(const tg_geometry* g, const Block& b, tg_pointset* pts, Device
 template <typename T>
 bool fused_cell_measures(const tg_geometry* g, const Block& b, const DeviceTables* tabs,
                          uint32_t options);
-// Enqueues sums[c] = sum_q det J(c, q) w_q (w narrowed to T, each product rounded, q ascending)
-// for every cell of block b straight from the register-resident kernel's det J, without a det J
-// field. Returns a TG status.
+// Enqueues sums[i] = the sum over cells c = 64 i .. 64 i + 63 of block b, in order, of
+// sum_q det J(c, q) w_q (w narrowed to T, each product rounded, q ascending): the cell sums and
+// the first pass of 64-value sums, straight from the register-resident kernel's det J, without a
+// det J field. Returns a TG status.
 template <typename T>
 int enqueue_cell_measures(const tg_geometry* g, const Block& b, tg_pointset* pts,
                           DeviceTables* tabs, const double* weights, uint32_t options, T* sums,

@@ -13,7 +13,8 @@ template <typename T>
 int measure(const tg_geometry* g, const tg::Block& b, tg_pointset* pts, tg::DeviceTables* tabs,
             const double* weights, uint32_t options, double* value, hipStream_t s) {
   // Where the register-resident kernel computes det J, one kernel sums each cell's points
-  // straight from it (no det J field); the block sums follow.
+  // straight from it and adds each 64 cells' sums (the first blocked pass; no det J field); the
+  // later passes follow.
   const uint64_t sums = tg::measure_scratch_elems(b.num_cells);
   if (tg::fused_cell_measures<T>(g, b, tabs, options)) {
     void* work = nullptr;
@@ -22,7 +23,7 @@ int measure(const tg_geometry* g, const tg::Block& b, tg_pointset* pts, tg::Devi
     int rc =
         tg::enqueue_cell_measures<T>(g, b, pts, tabs, weights, options, static_cast<T*>(work), s);
     if (rc == TG_OK) {
-      e = tg::launch_measure_sums<T>(b.num_cells, static_cast<T*>(work), value, s);
+      e = tg::launch_measure_sums<T>((b.num_cells + 63) / 64, static_cast<T*>(work), value, s);
       if (e != hipSuccess) rc = tg::hip_fail(e, "measure kernel launch");
     }
     const hipError_t ef = hipFreeAsync(work, s);

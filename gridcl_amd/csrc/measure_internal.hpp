@@ -18,10 +18,10 @@ template <typename T>
 hipError_t launch_measure(const T* detj, const double* weights, uint32_t nq, uint64_t num_cells,
                           T* scratch, double* value, hipStream_t stream);
 
-// *value (device) = the FP64 widening of the fixed-order sum of the per-cell sums in
-// scratch[0, num_cells) (num_cells > 0); scratch holds measure_scratch_elems(num_cells) elements
-// and is overwritten.
+// *value (device) = the FP64 widening of the passes of 64-value sums over the values
+// scratch[0, len) (len > 0: the cell sums, or the first pass's outputs); scratch holds
+// len + ceil(len / 64) elements and is overwritten.
 template <typename T>
-hipError_t launch_measure_sums(uint64_t num_cells, T* scratch, double* value, hipStream_t stream);
+hipError_t launch_measure_sums(uint64_t len, T* scratch, double* value, hipStream_t stream);
 
 }  // namespace tg

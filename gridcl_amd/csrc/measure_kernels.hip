@@ -29,6 +29,28 @@ __global__ void __launch_bounds__(kBlock)
   out[c] = acc;
 }
 
+// x[0] + x[1] + ... + x[count - 1] in order (0 < count <= 64), the values loaded a chunk at a
+// time into registers before they are added, so the loads overlap.
+template <typename T>
+__device__ __forceinline__ T ordered_sum(const T* __restrict__ x, uint32_t count) {
+  constexpr uint32_t kChunk = sizeof(T) == 4 ? 64 : 32;
+  T acc = T(0);
+#pragma unroll 1
+  for (uint32_t c0 = 0; c0 < kSumBlock; c0 += kChunk) {
+    T v[kChunk];
+#pragma unroll
+    for (uint32_t j = 0; j < kChunk; ++j) v[j] = c0 + j < count ? x[c0 + j] : T(0);
+#pragma unroll
+    for (uint32_t j = 0; j < kChunk; ++j) {
+      if (c0 + j == 0)
+        acc = v[0];
+      else if (c0 + j < count)
+        acc += v[j];
+    }
+  }
+  return acc;
+}
+
 // out[i] = x[64 i] + x[64 i + 1] + ... over the entries below len, in order (measure.rs:46-72).
 template <typename T>
 __global__ void __launch_bounds__(kBlock)
@@ -36,15 +58,39 @@ __global__ void __launch_bounds__(kBlock)
   const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
   if (i >= num_out) return;
   const uint64_t start = i * kSumBlock;
-  const uint64_t end = start + kSumBlock < len ? start + kSumBlock : len;
-  T acc = x[start];
-  for (uint64_t j = start + 1; j < end; ++j) acc += x[j];
-  out[i] = acc;
+  const uint64_t count = start + kSumBlock < len ? kSumBlock : len - start;
+  out[i] = ordered_sum<T>(x + start, static_cast<uint32_t>(count));
 }
 
+constexpr uint32_t kFinish = 1024;  // threads of the finishing workgroup
+
+// The passes of 64-value sums from len <= 64 kFinish values x down to one (measure.rs:46-72),
+// in one workgroup with the levels after the first in LDS, and the total widened to FP64
+// (measure.rs:75-102). Every thread runs the same number of passes.
 template <typename T>
-__global__ void widen(const T* __restrict__ x, double* __restrict__ value) {
-  if (threadIdx.x == 0) *value = static_cast<double>(x[0]);
+__global__ void __launch_bounds__(kFinish)
+    finish_sums(const T* __restrict__ x, uint64_t len, double* __restrict__ value) {
+  __shared__ T level[2][kFinish];
+  uint32_t n = static_cast<uint32_t>((len + kSumBlock - 1) / kSumBlock);
+  for (uint32_t i = threadIdx.x; i < n; i += kFinish) {
+    const uint64_t start = static_cast<uint64_t>(i) * kSumBlock;
+    const uint64_t count = start + kSumBlock < len ? kSumBlock : len - start;
+    level[0][i] = ordered_sum<T>(x + start, static_cast<uint32_t>(count));
+  }
+  __syncthreads();
+  uint32_t cur = 0;
+  while (n > 1) {
+    const uint32_t m = (n + kSumBlock - 1) / kSumBlock;
+    for (uint32_t i = threadIdx.x; i < m; i += kFinish) {
+      const uint32_t start = i * kSumBlock;
+      const uint32_t count = start + kSumBlock < n ? kSumBlock : n - start;
+      level[cur ^ 1u][i] = ordered_sum<T>(&level[cur][start], count);
+    }
+    __syncthreads();
+    cur ^= 1u;
+    n = m;
+  }
+  if (threadIdx.x == 0) *value = static_cast<double>(level[cur][0]);
 }
 
 __global__ void store_zero(double* __restrict__ value) {
@@ -75,15 +121,14 @@ hipError_t launch_measure(const T* detj, const double* weights, uint32_t nq, uin
 }
 
 template <typename T>
-hipError_t launch_measure_sums(uint64_t num_cells, T* scratch, double* value, hipStream_t stream) {
-  if (num_cells == 0 || (num_cells + kBlock - 1) / kBlock > 0x7fffffffu)
-    return hipErrorInvalidConfiguration;
-  // Ping-pong between the cell sums [num_cells] and the first pass's outputs behind them.
+hipError_t launch_measure_sums(uint64_t len, T* scratch, double* value, hipStream_t stream) {
+  if (len == 0 || (len + kBlock - 1) / kBlock > 0x7fffffffu) return hipErrorInvalidConfiguration;
+  // Passes over the whole grid while one workgroup cannot finish; they ping-pong between the
+  // values [len] and the space behind them.
   T* a = scratch;
-  T* b = scratch + num_cells;
+  T* b = scratch + len;
   hipError_t e = hipSuccess;
-  uint64_t len = num_cells;
-  while (e == hipSuccess && len > 1) {
+  while (e == hipSuccess && len > static_cast<uint64_t>(kSumBlock) * kFinish) {
     const uint64_t num_out = (len + kSumBlock - 1) / kSumBlock;
     hipLaunchKernelGGL(block_sums<T>, dim3(grid_of(num_out)), dim3(kBlock), 0, stream, a, len,
                        num_out, b);
@@ -94,7 +139,7 @@ hipError_t launch_measure_sums(uint64_t num_cells, T* scratch, double* value, hi
     len = num_out;
   }
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(widen<T>, dim3(1), dim3(64), 0, stream, a, value);
+  hipLaunchKernelGGL(finish_sums<T>, dim3(1), dim3(kFinish), 0, stream, a, len, value);
   return hipGetLastError();
 }
 

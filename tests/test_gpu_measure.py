@@ -172,6 +172,38 @@ def test_cell_measures_ragged_single_and_clamped(cuda, n, bad, dtype):
         assert got == host_reduction(field.to_host_cell_major(), ps.weights, npdt)
 
 
+def unit_square_triangles(n):
+    """The unit square in n x n squares of two P1 triangles each, counter-clockwise."""
+    t = np.linspace(0.0, 1.0, n + 1)
+    x, y = np.meshgrid(t, t)
+    nodes = np.stack([x.ravel(), y.ravel()], axis=1)
+    v0 = (np.arange(n)[None, :] + (n + 1) * np.arange(n)[:, None]).ravel()
+    v1, v2 = v0 + 1, v0 + n + 1
+    cells = np.concatenate([np.stack([v0, v1, v2], 1), np.stack([v1, v2 + 1, v2], 1)])
+    return M.HostMesh(2, nodes, [M.HostBlock(TRI, 1, cells.astype(np.uint32))])
+
+
+@pytest.mark.parametrize("options", [0, api.OPT_FORCE_GENERIC])
+@pytest.mark.parametrize("dtype", ["f32", "f64"])
+def test_measure_passes_beyond_one_workgroup(cuda, options, dtype):
+    """4,500,000 triangles: more first-pass sums (70,313) than the finishing workgroup takes, so
+    a grid-wide pass of 64-value sums runs first, on the one-kernel path and on the det J field
+    (FORCE_GENERIC); bit for bit the host's reduction of the field, and the square's area 1
+    (FP64 1e-12; FP32 1e-4, the rounding of 4.5 million sums in FP32)."""
+    import torch
+
+    mesh = unit_square_triangles(1500)
+    tdt = torch.float64 if dtype == "f64" else torch.float32
+    npdt = np.float64 if dtype == "f64" else np.float32
+    g = api.geometry_from_host(mesh, dtype=tdt)
+    ps = api.PointSet.gauss(TRI, 2)
+    got = api.measure_block(g, 0, ps, options)
+    field = api.geometric_factors_block(g, 0, ps, ["detj"], 16, options)["detj"]
+    torch.cuda.synchronize()
+    assert got == host_reduction(field.to_host_cell_major(), ps.weights, npdt)
+    assert abs(got - 1.0) <= (1e-12 if dtype == "f64" else 1e-4), got
+
+
 @pytest.mark.parametrize("dtype", ["f32", "f64"])
 def test_mesh_measure_mixed_mesh(cuda, dtype):
     """Mesh.measure over the five blocks of a small mixed mesh (tetrahedra, hexahedra, prisms,
