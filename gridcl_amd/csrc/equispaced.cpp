@@ -128,22 +128,36 @@ int tg_equispaced_nodes(const tg_geometry* g, void* out, void* stream) {
   }
   if (total == 0) return TG_OK;
   if (total > 0xffffffffull) return fail(TG_ERR_TOO_LARGE, "TooLarge: too many node occurrences");
-  tg_geometry* gm = const_cast<tg_geometry*>(g);  // only its lattice cache changes
-  void* first = nullptr;
-  e = hipMallocAsync(&first, g->num_nodes * sizeof(uint32_t), s);
-  if (e != hipSuccess) return tg::hip_fail(e, "equispaced conversion occurrences");
-  uint32_t* f = static_cast<uint32_t*>(first);
-  e = tg::launch_first_init(f, g->num_nodes, s);
-  for (const Occurrences& o : occ)
-    if (e == hipSuccess)
-      e = tg::launch_first_occurrence(g->blocks[o.block].dofmap, o.len,
-                                      static_cast<uint32_t>(o.start), g->num_nodes, f, s);
-  int rc = e == hipSuccess ? TG_OK : tg::hip_fail(e, "equispaced conversion kernel launch");
-  if (rc == TG_OK)
-    rc = g->dtype == TG_F32 ? convert<float>(gm, occ, f, out, s) : convert<double>(gm, occ, f, out, s);
-  const hipError_t ef = hipFreeAsync(first, s);
-  if (rc == TG_OK && ef != hipSuccess) rc = tg::hip_fail(ef, "equispaced conversion occurrences");
-  return rc;
+  tg_geometry* gm = const_cast<tg_geometry*>(g);  // only its caches change
+  if (!gm->first_occurrence) {
+    // Built once: the first occurrences depend on the dofmaps only, which stay unchanged while
+    // the geometry lives. Later calls, on any stream, wait for the event recorded after it.
+    void* first = nullptr;
+    e = hipMalloc(&first, g->num_nodes * sizeof(uint32_t));
+    if (e != hipSuccess) return tg::hip_fail(e, "equispaced conversion occurrences");
+    uint32_t* f = static_cast<uint32_t*>(first);
+    hipEvent_t ready = nullptr;
+    e = hipEventCreateWithFlags(&ready, hipEventDisableTiming);
+    if (e == hipSuccess) e = tg::launch_first_init(f, g->num_nodes, s);
+    for (const Occurrences& o : occ)
+      if (e == hipSuccess)
+        e = tg::launch_first_occurrence(g->blocks[o.block].dofmap, o.len,
+                                        static_cast<uint32_t>(o.start), g->num_nodes, f, s);
+    if (e == hipSuccess) e = hipEventRecord(ready, s);
+    if (e != hipSuccess) {
+      (void)hipStreamSynchronize(s);
+      if (ready) (void)hipEventDestroy(ready);
+      (void)hipFree(first);
+      return tg::hip_fail(e, "equispaced conversion kernel launch");
+    }
+    gm->first_occurrence = f;
+    gm->first_ready = ready;
+  } else if (hipEventQuery(gm->first_ready) != hipSuccess) {  // still being built
+    e = hipStreamWaitEvent(s, gm->first_ready, 0);
+    if (e != hipSuccess) return tg::hip_fail(e, "equispaced conversion occurrences");
+  }
+  return g->dtype == TG_F32 ? convert<float>(gm, occ, gm->first_occurrence, out, s)
+                            : convert<double>(gm, occ, gm->first_occurrence, out, s);
 }
 
 }  // extern "C"

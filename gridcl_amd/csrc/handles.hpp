@@ -74,9 +74,15 @@ struct tg_geometry {
   // the lattice point of each local node (device, k entries).
   std::map<std::pair<int, unsigned>, std::unique_ptr<tg_pointset>> lattices;
   std::map<std::pair<int, unsigned>, uint32_t*> lattice_point_of;
+  // The equispaced conversion's table of each node's first GLL occurrence (a function of the
+  // dofmaps alone), built by the first conversion; `first_ready` is recorded on its stream.
+  uint32_t* first_occurrence = nullptr;
+  hipEvent_t first_ready = nullptr;
   ~tg_geometry() {
     for (auto& kv : pull_back_coef) (void)hipFree(kv.second);
     for (auto& kv : lattice_point_of) (void)hipFree(kv.second);
+    if (first_ready) (void)hipEventDestroy(first_ready);
+    if (first_occurrence) (void)hipFree(first_occurrence);
   }
 };
 

@@ -293,7 +293,10 @@ int tg_pull_back_block(const tg_geometry* g, int32_t block, const void* x, uint6
  * rounding). A node shared by several cells takes the value of its first occurrence in (block,
  * cell, local node) order, so the result is the same on every run; nodes of equispaced blocks
  * and nodes that no GLL cell uses are copied unchanged, and dofmap entries past the node array
- * are skipped. Work arrays are allocated and freed on `stream`. */
+ * are skipped. Work arrays are allocated and freed on `stream`. The first call also builds each
+ * node's first occurrence (num_nodes u32, a function of the dofmaps, which must therefore stay
+ * unchanged while the geometry lives; the nodes may change) and keeps it with the geometry;
+ * later calls, on any stream, reuse it. */
 int tg_equispaced_nodes(const tg_geometry* g, void* out, void* stream);
 
 /* ---- The mesh functional and its adjoint (no reference counterpart) --------------------------

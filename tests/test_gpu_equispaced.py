@@ -113,3 +113,22 @@ def test_equispaced_blocks_and_unused_nodes_unchanged(cuda):
     before = g.nodes.cpu().numpy()
     assert np.array_equal(out[n0:], before[n0:])
     assert not np.array_equal(out[:n0], before[:n0])
+
+
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_equispaced_second_call_after_nodes_move(cuda, dtype):
+    """The first call's table of first occurrences (kept with the geometry) serves a later call
+    on another stream after the nodes moved: doubling every node (exact in floating point)
+    doubles every converted node bit for bit, as the conversion is linear in the nodes."""
+    import torch
+
+    tdt = torch.float64 if dtype == "f64" else torch.float32
+    g = api.geometry_from_host(gll_box(HEX, 3, 3, 3), dtype=tdt)
+    first = api.equispaced_nodes(g).clone()
+    g.nodes.mul_(2.0)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        second = api.equispaced_nodes(g)
+    side.synchronize()
+    assert torch.equal(second, 2.0 * first)
