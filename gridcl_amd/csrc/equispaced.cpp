@@ -92,9 +92,9 @@ int convert(tg_geometry* g, const std::vector<Occurrences>& occ, uint32_t* first
     // X at the lattice relative to the origin, like the node array (equispaced.rs:7-10).
     rc = tg::enqueue_block<T>(g, b, pts, tabs, TG_POINTS, 16, TG_OPT_POINTS_RELATIVE, &outs, s);
     if (rc == TG_OK) {
-      e = tg::launch_select_first<T>(g->gdim, b.dofmap, o.len, static_cast<uint32_t>(o.start),
-                                     b.k, point_of, static_cast<const T*>(field), first,
-                                     g->num_nodes, static_cast<T*>(out), s);
+      e = tg::launch_select_first<T>(g->gdim, o.len, static_cast<uint32_t>(o.start), b.k,
+                                     point_of, static_cast<const T*>(field), first, g->num_nodes,
+                                     static_cast<T*>(out), s);
       if (e != hipSuccess) rc = tg::hip_fail(e, "equispaced conversion kernel launch");
     }
     const hipError_t ef = hipFreeAsync(field, s);

@@ -13,11 +13,11 @@ hipError_t launch_first_init(uint32_t* first, uint64_t num_nodes, hipStream_t st
 hipError_t launch_first_occurrence(const uint32_t* dofmap, uint64_t len, uint32_t start,
                                    uint64_t num_nodes, uint32_t* first, hipStream_t stream);
 // Writes out[node] from the field (X at the block's lattice, tiles of 16, gdim components; local
-// node j at lattice point point_of[j]) for the nodes whose first occurrence is in this block.
+// node j at lattice point point_of[j]) for the nodes whose first occurrence is one of this
+// block's, start .. start + len - 1.
 template <typename T>
-hipError_t launch_select_first(int gdim, const uint32_t* dofmap, uint64_t len, uint32_t start,
-                               uint32_t k, const uint32_t* point_of, const T* field,
-                               const uint32_t* first, uint64_t num_nodes, T* out,
-                               hipStream_t stream);
+hipError_t launch_select_first(int gdim, uint64_t len, uint32_t start, uint32_t k,
+                               const uint32_t* point_of, const T* field, const uint32_t* first,
+                               uint64_t num_nodes, T* out, hipStream_t stream);
 
 }  // namespace tg

@@ -3,7 +3,7 @@
 // (block, cell, local node) order from the block's physical points at the equispaced lattice.
 // The first occurrence of every node is the minimum of its flat occurrence indices, taken with
 // atomicMin, which gives the same node-by-node answer in any execution order; each node is then
-// written once, by that occurrence.
+// written once, from that occurrence.
 #include <hip/hip_runtime.h>
 
 #include "equispaced_internal.hpp"
@@ -29,21 +29,24 @@ __global__ void __launch_bounds__(kBlock)
   if (node < num_nodes) atomicMin(&first[node], start + static_cast<uint32_t>(i));
 }
 
-// out[node] = the field's value at the node's first occurrence, for the occurrences of one
-// block: occurrence i is point q = i % k of cell c = i / k of the field (tiles of 16).
+// out[node] = the field's value at the node's first occurrence, for the nodes whose first
+// occurrence start + i (i < len) is in this block: occurrence i is local node i % k of cell
+// c = i / k, read at lattice point point_of[i % k] of the field (tiles of 16). One thread per
+// node, so the table is read in order and each node written once.
 template <typename T, int GD>
 __global__ void __launch_bounds__(kBlock)
-    select_first(const uint32_t* __restrict__ dofmap, uint64_t len, uint32_t start, uint32_t k,
-                 const uint32_t* __restrict__ point_of, const T* __restrict__ field,
-                 const uint32_t* __restrict__ first, uint64_t num_nodes, T* __restrict__ out) {
-  const uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
-  if (i >= len) return;
-  const uint32_t node = dofmap[i];
-  if (node >= num_nodes || first[node] != start + static_cast<uint32_t>(i)) return;
+    select_first(uint64_t len, uint32_t start, uint32_t k, const uint32_t* __restrict__ point_of,
+                 const T* __restrict__ field, const uint32_t* __restrict__ first,
+                 uint64_t num_nodes, T* __restrict__ out) {
+  const uint64_t node = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x;
+  if (node >= num_nodes) return;
+  const uint32_t o = first[node];  // 0xffffffff: no occurrence
+  if (o < start || static_cast<uint64_t>(o - start) >= len) return;
+  const uint64_t i = o - start;
   const uint64_t c = i / k, q = point_of[i - c * k];
 #pragma unroll
   for (int g = 0; g < GD; ++g)
-    out[static_cast<uint64_t>(node) * GD + g] = field[(((c >> 4) * k + q) * GD + g) * 16 + (c & 15u)];
+    out[node * GD + g] = field[(((c >> 4) * k + q) * GD + g) * 16 + (c & 15u)];
 }
 
 uint32_t grid_of(uint64_t n) { return static_cast<uint32_t>((n + kBlock - 1) / kBlock); }
@@ -66,23 +69,24 @@ hipError_t launch_first_occurrence(const uint32_t* dofmap, uint64_t len, uint32_
 }
 
 template <typename T>
-hipError_t launch_select_first(int gdim, const uint32_t* dofmap, uint64_t len, uint32_t start,
-                               uint32_t k, const uint32_t* point_of, const T* field,
-                               const uint32_t* first, uint64_t num_nodes, T* out,
-                               hipStream_t stream) {
-  if (len == 0) return hipSuccess;
+hipError_t launch_select_first(int gdim, uint64_t len, uint32_t start, uint32_t k,
+                               const uint32_t* point_of, const T* field, const uint32_t* first,
+                               uint64_t num_nodes, T* out, hipStream_t stream) {
+  if (len == 0 || num_nodes == 0) return hipSuccess;
+  if ((num_nodes + kBlock - 1) / kBlock > 0x7fffffffu) return hipErrorInvalidConfiguration;
+  const dim3 grid(grid_of(num_nodes));
   switch (gdim) {
     case 1:
-      hipLaunchKernelGGL((select_first<T, 1>), dim3(grid_of(len)), dim3(kBlock), 0, stream,
-                         dofmap, len, start, k, point_of, field, first, num_nodes, out);
+      hipLaunchKernelGGL((select_first<T, 1>), grid, dim3(kBlock), 0, stream, len, start, k,
+                         point_of, field, first, num_nodes, out);
       break;
     case 2:
-      hipLaunchKernelGGL((select_first<T, 2>), dim3(grid_of(len)), dim3(kBlock), 0, stream,
-                         dofmap, len, start, k, point_of, field, first, num_nodes, out);
+      hipLaunchKernelGGL((select_first<T, 2>), grid, dim3(kBlock), 0, stream, len, start, k,
+                         point_of, field, first, num_nodes, out);
       break;
     case 3:
-      hipLaunchKernelGGL((select_first<T, 3>), dim3(grid_of(len)), dim3(kBlock), 0, stream,
-                         dofmap, len, start, k, point_of, field, first, num_nodes, out);
+      hipLaunchKernelGGL((select_first<T, 3>), grid, dim3(kBlock), 0, stream, len, start, k,
+                         point_of, field, first, num_nodes, out);
       break;
     default:
       return hipErrorInvalidValue;
@@ -90,11 +94,11 @@ hipError_t launch_select_first(int gdim, const uint32_t* dofmap, uint64_t len, u
   return hipGetLastError();
 }
 
-template hipError_t launch_select_first<float>(int, const uint32_t*, uint64_t, uint32_t,
-                                               uint32_t, const uint32_t*, const float*,
-                                               const uint32_t*, uint64_t, float*, hipStream_t);
-template hipError_t launch_select_first<double>(int, const uint32_t*, uint64_t, uint32_t,
-                                                uint32_t, const uint32_t*, const double*,
-                                                const uint32_t*, uint64_t, double*, hipStream_t);
+template hipError_t launch_select_first<float>(int, uint64_t, uint32_t, uint32_t,
+                                               const uint32_t*, const float*, const uint32_t*,
+                                               uint64_t, float*, hipStream_t);
+template hipError_t launch_select_first<double>(int, uint64_t, uint32_t, uint32_t,
+                                                const uint32_t*, const double*, const uint32_t*,
+                                                uint64_t, double*, hipStream_t);
 
 }  // namespace tg
