@@ -77,6 +77,28 @@ static const Ent* ents(int cell, int dim, int* n) {
 
 int or_tdim(int cell) { return (cell >= 0 && cell < 8) ? TDIM[cell] : -1; }
 
+int or_sub_entity(int cell, int dim, int index, int* type, int* verts) {
+  if (cell < 0 || cell > 7 || dim < 0 || dim > TDIM[cell] || index < 0) return -1;
+  if (dim == 0) {
+    if (index >= NV[cell]) return -1;
+    *type = PT;
+    verts[0] = index;
+    return 1;
+  }
+  if (dim == TDIM[cell]) {
+    if (index > 0) return -1;
+    *type = cell;
+    for (int v = 0; v < NV[cell]; ++v) verts[v] = v;
+    return NV[cell];
+  }
+  int ne;
+  const Ent* es = ents(cell, dim, &ne);
+  if (index >= ne) return -1;
+  *type = es[index].type;
+  for (int v = 0; v < es[index].n; ++v) verts[v] = es[index].v[v];
+  return es[index].n;
+}
+
 size_t or_num_nodes(int cell, int degree) {
   size_t p = (size_t)degree;
   switch (cell) {

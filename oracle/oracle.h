@@ -23,10 +23,13 @@
  *     cell's centroid, with the reference's step, box clamp and freezing rule
  *     (tetraloma-geometry/src/pull_back.rs:26-180), evaluated with this oracle's own bases.
  *
- * Pinning: node layouts against the reference's lagrange_points.json / tables.json fixtures;
- * bases by the Kronecker property, partition of unity and polynomial reproduction; factors by
+ * Pinning: node layouts against the reference's lagrange_points.json fixture, sub-entity tables
+ * against tables.json, and the A.5 order of every cell to degree 3 (prisms and pyramids
+ * included) through the reference's A.9 rows and DOLFINx P3 arrays (node_maps.rs:11-170) applied
+ * to the reference's Gmsh corpus; bases by the Kronecker property, partition of unity and
+ * reproduction of the element's space (for pyramids the whole rational space
+ * x^a y^b z^c/(1-z)^min(a,b), a + c <= p, b + c <= p, to degree 4, with gradients); factors by
  * affine closed forms, unit-volume sums, K J = I and the reference's doc-test examples; the
- * pyramid of degree >= 2 (Vandermonde path) by Kronecker/partition/reproduction only; the
  * pull-back by round trips (the physical points of known reference points come back), by
  * affine cells (exact after one step) and by the closest point of a surface for points off
  * it. */
@@ -41,6 +44,9 @@ extern "C" {
 #endif
 
 int or_tdim(int cell);
+/* Sub-entity `index` of dimension `dim` of the reference cell (cell.rs:53-249): its cell type
+ * and its vertices in the reference's order. Returns the vertex count, or -1 past the last. */
+int or_sub_entity(int cell, int dim, int index, int* type, int* verts);
 size_t or_num_nodes(int cell, int degree);
 /* A.5 nodes [k][tdim]; returns the node count or -1. family 0 = equispaced, 1 = GLL. */
 long or_lagrange_points(int cell, int degree, int family, double* out);

@@ -28,6 +28,8 @@ def load() -> ctypes.CDLL:
         lib = ctypes.CDLL(LIB_PATH)
         lib.or_tdim.restype = c_int
         lib.or_tdim.argtypes = [c_int]
+        lib.or_sub_entity.restype = c_int
+        lib.or_sub_entity.argtypes = [c_int, c_int, c_int, POINTER(c_int), POINTER(c_int)]
         lib.or_num_nodes.restype = c_size_t
         lib.or_num_nodes.argtypes = [c_int, c_int]
         lib.or_lagrange_points.restype = c_long
@@ -68,6 +70,19 @@ def _p(a):
 
 
 TDIM = {1: 1, 2: 2, 3: 2, 4: 3, 5: 3, 6: 3, 7: 3}
+
+
+def sub_entities(cell: int, dim: int):
+    """[(type, [vertices])] of the reference cell's sub-entities of dimension dim."""
+    lib = load()
+    out = []
+    ty = c_int()
+    verts = (c_int * 8)()
+    while True:
+        n = lib.or_sub_entity(cell, dim, len(out), ctypes.byref(ty), verts)
+        if n < 0:
+            return out
+        out.append((ty.value, list(verts[:n])))
 
 
 def lagrange_points(cell: int, degree: int, family: int = 0) -> np.ndarray:
