@@ -170,7 +170,7 @@ def bench_hybrid(api, M, steps, warmup, scale=1.0):
     g = api.geometry_from_host(h, dtype=torch.float32)
     rule = api.QuadRule.for_grid(g, 4)
     factors = ["points", "jacobians", "detj", "normals"]
-    plan = api.MeshPlan(g, rule, factors)
+    plan = api.MeshPlan(g, rule, factors, options=api.OPT_PLAN_PERSISTENT)
     evals = sum(b.num_cells * len(rule.get(b.cell)) for b in h.blocks)
     nbytes = 0
     for b, blk in enumerate(h.blocks):

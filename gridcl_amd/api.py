@@ -59,6 +59,7 @@ OPT_PLAN_PER_BLOCK = 128
 OPT_PLAN_SERIAL = 256
 OPT_SUMFACT_ALWAYS = 512
 OPT_POINTS_RELATIVE = 1024
+OPT_PLAN_PERSISTENT = 2048
 
 
 # ---- Errors: one class per tetraloma_device::GridError variant of the path -------------------
@@ -550,8 +551,9 @@ class PullBack:
 
 
 class MeshPlan:
-    """Every block of a geometry evaluated in one persistent launch (or one launch per block
-    when an element has no register-resident kernel), optionally replayed from a hipGraph.
+    """Every block of a geometry evaluated by one plan: one launch per block (the default;
+    replayed from a hipGraph its blocks run as parallel branches) or, with OPT_PLAN_PERSISTENT,
+    one persistent launch over a work list of all blocks.
     ``fields[b][name]`` are the outputs of block b, filled by ``launch()``/``graph_launch()``."""
 
     def __init__(self, g: Geometry, rule: QuadRule, factors: Sequence[str], tile: int = 16,

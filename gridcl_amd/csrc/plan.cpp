@@ -255,7 +255,8 @@ int tg_plan_create(const tg_geometry* g, tg_pointset* const* pts, uint32_t facto
     if (tg::hybrid_shape(b.tdim, g->gdim, b.k) < 0) all_hybrid = false;
     plan->blocks.push_back(pb);
   }
-  plan->persistent = all_hybrid && !(options & TG_OPT_PLAN_PER_BLOCK);
+  plan->persistent =
+      all_hybrid && (options & TG_OPT_PLAN_PERSISTENT) && !(options & TG_OPT_PLAN_PER_BLOCK);
   if (!plan->persistent && plan->blocks.size() > 1 && !(options & TG_OPT_PLAN_SERIAL)) {
     // Largest block first, on the capturing stream; the others on their own streams.
     std::stable_sort(plan->blocks.begin(), plan->blocks.end(),
@@ -305,16 +306,24 @@ int tg_plan_graph_launch(tg_plan* plan, void* stream) {
   hipError_t e = hipSuccess;
   if (!plan->exec) {
     // Capture on a private stream of default priority, once; replays launch on `stream`.
-    e = hipStreamCreateWithFlags(&plan->capture, hipStreamNonBlocking);
+    if (!plan->capture) e = hipStreamCreateWithFlags(&plan->capture, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipStreamBeginCapture(plan->capture, hipStreamCaptureModeThreadLocal);
     int rc = TG_OK;
     if (e == hipSuccess) {
       rc = plan_enqueue(plan, plan->capture, true);
-      e = hipStreamEndCapture(plan->capture, &plan->graph);
+      // End the capture whatever the enqueue returned, so the stream leaves capture mode.
+      hipError_t end = hipStreamEndCapture(plan->capture, &plan->graph);
+      if (rc == TG_OK) e = end;
     }
-    if (rc) return rc;
-    if (e == hipSuccess) e = hipGraphInstantiate(&plan->exec, plan->graph, nullptr, nullptr, 0);
-    if (e != hipSuccess) return tg::hip_fail(e, "plan graph capture");
+    if (rc == TG_OK && e == hipSuccess)
+      e = hipGraphInstantiate(&plan->exec, plan->graph, nullptr, nullptr, 0);
+    if (rc != TG_OK || e != hipSuccess) {
+      // Release the partial graph (the capture stream is kept for the next attempt).
+      if (plan->graph) hipGraphDestroy(plan->graph);
+      plan->graph = nullptr;
+      plan->exec = nullptr;
+      return rc != TG_OK ? rc : tg::hip_fail(e, "plan graph capture");
+    }
   }
   e = hipGraphLaunch(plan->exec, static_cast<hipStream_t>(stream));
   return e == hipSuccess ? TG_OK : tg::hip_fail(e, "plan graph launch");

@@ -105,8 +105,8 @@ enum {
    * det J where it is the faster one (FP32: degree 3, degree 2 from 3 points per direction,
    * degree 1 at 4; FP64: degree >= 2); this option keeps them on the dense kernels. */
   TG_OPT_NO_SUMFACT = 1u << 6,
-  /* Plans only: one launch per block (each with the kernel tg_eval_block would run) instead of
-   * the persistent kernel over the work list; tg_plan_graph_launch captures them in one graph. */
+  /* Plans only: one launch per block (each with the kernel tg_eval_block would run), the default;
+   * it overrides TG_OPT_PLAN_PERSISTENT. tg_plan_graph_launch captures them in one graph. */
   TG_OPT_PLAN_PER_BLOCK = 1u << 7,
   /* Plans with one launch per block: capture the blocks one after another in the graph (by
    * default tg_plan_graph_launch's graph runs them as parallel branches, so they overlap;
@@ -118,7 +118,10 @@ enum {
   TG_OPT_SUMFACT_ALWAYS = 1u << 9,
   /* Physical points relative to the geometry's origin (the origin is not added back), like the
    * node array; the reference evaluates its equispaced conversion this way (equispaced.rs:9-10). */
-  TG_OPT_POINTS_RELATIVE = 1u << 10
+  TG_OPT_POINTS_RELATIVE = 1u << 10,
+  /* Plans only: run every block in the one persistent kernel over the work list, where every
+   * block's element has a kernel of that form (otherwise the plan stays per block). */
+  TG_OPT_PLAN_PERSISTENT = 1u << 11
 };
 
 typedef struct tg_geometry tg_geometry;
@@ -226,14 +229,14 @@ uint64_t tg_evector_elems(const tg_geometry* g, int32_t block);
  * and the outputs outs[b] of the factors in `factors` that apply to it (normals on blocks of
  * codimension 1, facet normals on blocks of codimension 0 with a facet point set; the others on
  * every block). Replaces the reference's per-block loop over launch_factor
- * (tetraloma/src/mesh.rs:249-257 per_block; tetraloma-geometry/src/layout.rs:488) with one
- * persistent kernel over a work list of the items (one workgroup's cells) of all blocks, split on
- * the host into one contiguous range of equal estimated work per resident workgroup (the ranges
- * of one XCD adjacent). Blocks whose element has no register-resident kernel (more than 27
- * nodes per cell, or forced generic), and TG_OPT_PLAN_PER_BLOCK, make the plan run one launch
- * per block instead: tg_plan_launch launches them in order on the caller's stream;
- * tg_plan_graph_launch's graph holds them as parallel branches (captured on streams of their own,
- * default priority, forked and joined by events; TG_OPT_PLAN_SERIAL captures them in order).
+ * (tetraloma/src/mesh.rs:249-257 per_block; tetraloma-geometry/src/layout.rs:488). By default
+ * the plan runs one launch per block with tg_eval_block's kernels: tg_plan_launch launches them
+ * in order on the caller's stream; tg_plan_graph_launch's graph holds them as parallel branches
+ * (captured on streams of their own, default priority, forked and joined by events;
+ * TG_OPT_PLAN_SERIAL captures them in order). TG_OPT_PLAN_PERSISTENT runs one persistent kernel
+ * over a work list of the items (one workgroup's cells) of all blocks instead, split on the host
+ * into one contiguous range of equal estimated work per resident workgroup (the ranges of one
+ * XCD adjacent), where every block's element has a kernel of that form.
  * Outputs are laid out exactly as tg_eval_block's. */
 typedef struct tg_plan tg_plan;
 int tg_plan_create(const tg_geometry* g, tg_pointset* const* pts, uint32_t factors,
@@ -244,7 +247,8 @@ int tg_plan_info(const tg_plan* plan, uint32_t* num_items, uint32_t* grid, int32
 /* Enqueues the plan on `stream`. */
 int tg_plan_launch(tg_plan* plan, void* stream);
 /* Captures the plan into a hipGraph on first use (on a private stream) and replays the graph on
- * `stream`. The plan's pointers are fixed at capture: later replays write the same outputs. */
+ * `stream`. The plan's pointers are fixed at capture: later replays write the same outputs. A
+ * capture that fails releases what it made, so a later call captures afresh. */
 int tg_plan_graph_launch(tg_plan* plan, void* stream);
 
 /* ---- Measure ----------------------------------------------------------------------------------

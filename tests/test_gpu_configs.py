@@ -95,7 +95,8 @@ def test_config4_hybrid(cuda, dtype):
     """Config 4: the mixed mesh (P2 tetrahedra, Q2 hexahedra, P2 prisms, P1 pyramids, P3 surface
     triangles with normals; 1.42 M cells, Gauss degree 4), each block's X, J, det J (and the
     triangles' normals) through the mesh plan's per-block launches and through its persistent
-    launch, against the oracle on every 97th cell of each block. FP32 1e-6 (block rel_err);
+    launch, each eager and replayed from its graph (outputs poisoned with NaN before each run),
+    against the oracle on every 97th cell of each block. FP32 1e-6 (block rel_err);
     FP64 5e-12: FP64 runs without a per-cell shift, and the P3 surface triangles' J sums ten
     products of coordinates about 40 cell widths from the origin with gradients up to ~10, so its
     rounding reaches ~1e-12 of the block's largest entry (1.03e-12 measured)."""
@@ -110,20 +111,36 @@ def test_config4_hybrid(cuda, dtype):
     factors = NAMES + ["normals"]
     tol = 1e-6 if dtype == "f32" else 5e-12
     worst = {}
-    for name, opt in (("per_block", api.OPT_PLAN_PER_BLOCK), ("persistent", 0)):
+    wants = {}
+    for name, opt in (("per_block", api.OPT_PLAN_PER_BLOCK), ("persistent", api.OPT_PLAN_PERSISTENT)):
         plan = api.MeshPlan(g, rule, factors, options=opt)
-        plan.launch()
-        torch.cuda.synchronize()
-        for b, blk in enumerate(mesh.blocks):
-            ps = rule.get(blk.cell)
-            cells = np.arange(0, blk.num_cells, 97)
-            got = {}
-            for n, f in plan.fields[b].items():
-                a = f.data.cpu().numpy().reshape(-1, len(ps), f.len, 16)
-                got[n] = a[cells // 16, :, :, cells % 16].astype(np.float64)
-            want = oracle_factors_sampled(g, mesh, b, ps, list(got), cells)
-            for n, e in rel_errs(got, want).items():
-                worst[(name, n)] = max(worst.get((name, n), 0.0), e)
-                assert e <= tol, (name, b, blk.cell, n, e)
+        for mode in ("eager", "graph"):
+            for fb in plan.fields:
+                for f in fb.values():
+                    f.data.fill_(float("nan"))
+            print(f"config 4 {dtype} {name} {mode}", flush=True)
+            if mode == "eager":
+                plan.launch()
+            else:
+                # two replays with eager work between them; the second one is checked
+                plan.graph_launch()
+                torch.cuda.synchronize()
+                for fb in plan.fields:
+                    for f in fb.values():
+                        f.data.fill_(float("nan"))
+                plan.graph_launch()
+            torch.cuda.synchronize()
+            for b, blk in enumerate(mesh.blocks):
+                ps = rule.get(blk.cell)
+                cells = np.arange(0, blk.num_cells, 97)
+                got = {}
+                for n, f in plan.fields[b].items():
+                    a = f.data.cpu().numpy().reshape(-1, len(ps), f.len, 16)
+                    got[n] = a[cells // 16, :, :, cells % 16].astype(np.float64)
+                if b not in wants:
+                    wants[b] = oracle_factors_sampled(g, mesh, b, ps, list(got), cells)
+                for n, e in rel_errs(got, wants[b]).items():
+                    worst[(name, n)] = max(worst.get((name, n), 0.0), e)
+                    assert e <= tol, (name, mode, b, blk.cell, n, e)
         del plan
     print(f"config 4 {dtype} rel_err {worst}")

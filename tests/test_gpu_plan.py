@@ -29,7 +29,7 @@ def test_plan_matches_per_block_and_oracle(cuda, hybrid_small, dtype):
     tdt = torch.float32 if dtype == "f32" else torch.float64
     g = api.geometry_from_host(h, dtype=tdt)
     rule = api.QuadRule.for_grid(g, 4)
-    plan = api.MeshPlan(g, rule, FACTORS)
+    plan = api.MeshPlan(g, rule, FACTORS, options=api.OPT_PLAN_PERSISTENT)
     # One workgroup per item at most; the small mesh has fewer items than resident workgroups.
     assert plan.persistent and 1 <= plan.grid <= plan.num_items
     plan.launch()
@@ -56,7 +56,7 @@ def test_plan_graph_replays(cuda, hybrid_small):
 
     g = api.geometry_from_host(hybrid_small, dtype=torch.float32)
     rule = api.QuadRule.for_grid(g, 4)
-    plan = api.MeshPlan(g, rule, FACTORS)
+    plan = api.MeshPlan(g, rule, FACTORS, options=api.OPT_PLAN_PERSISTENT)
     plan.launch()
     torch.cuda.synchronize()
     eager = [{n: f.data.clone() for n, f in fb.items()} for fb in plan.fields]
@@ -83,7 +83,7 @@ def test_plan_with_large_element_falls_back(cuda):
     h = M.jitter(M.cube_box(5, 2, 3), 0.01, 3, interior_only=True)
     g = api.geometry_from_host(h, dtype=torch.float32)
     rule = api.QuadRule.for_grid(g, 4)
-    plan = api.MeshPlan(g, rule, ["detj"])
+    plan = api.MeshPlan(g, rule, ["detj"], options=api.OPT_PLAN_PERSISTENT)
     assert not plan.persistent
     plan.launch()
     single = api.geometric_factors_block(g, 0, rule.get(5), ["detj"])
