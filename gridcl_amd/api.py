@@ -60,6 +60,7 @@ OPT_PLAN_SERIAL = 256
 OPT_SUMFACT_ALWAYS = 512
 OPT_POINTS_RELATIVE = 1024
 OPT_PLAN_PERSISTENT = 2048
+OPT_NO_MFMA = 4096
 
 
 # ---- Errors: one class per tetraloma_device::GridError variant of the path -------------------

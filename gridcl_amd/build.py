@@ -21,6 +21,10 @@ HIP_SOURCES = [
     ("factor_kernels.hip", ["-DTG_FACTOR_TYPE=float", "-DTG_FACTOR_COMMON"]),
     ("factor_kernels.hip", ["-DTG_FACTOR_TYPE=double"]),
     ("sumfact_kernels.hip", []),
+    # The matrix-core kernel's products go straight to VGPRs (not AGPRs): its J is read by the
+    # vector ALU at once, and the accumulation registers would otherwise count twice (4 waves
+    # per SIMD instead of 3 for Q3).
+    ("sumfact_mfma.hip", ["-mllvm", "-amdgpu-mfma-vgpr-form"]),
     ("adjoint_kernels.hip", []),
     ("measure_kernels.hip", []),
     ("pullback_kernels.hip", []),

@@ -43,6 +43,7 @@ DeviceTables::~DeviceTables() {
   if (ref_normals) (void)hipFree(ref_normals);
   if (sf_tab) (void)hipFree(sf_tab);
   if (sf_lex) (void)hipFree(sf_lex);
+  if (sf_mfma) (void)hipFree(sf_mfma);
 }
 
 }  // namespace tg
@@ -194,10 +195,33 @@ int device_tables(tg_pointset* p, unsigned degree, int family, int dtype, Device
   if (sumfact_tables(p, degree, family, tab1d, lex, &n1, &m)) {
     rc = dtype == TG_F32 ? upload<float>(tab1d, &t->sf_tab) : upload<double>(tab1d, &t->sf_tab);
     if (rc) return rc;
-    hipError_t e = hipMalloc(reinterpret_cast<void**>(&t->sf_lex), lex.size() * sizeof(uint32_t));
+    // lex, then its inverse (the A.5 index at each lexicographic position).
+    const size_t k = lex.size();
+    std::vector<uint32_t> both(2 * k);
+    for (size_t i = 0; i < k; ++i) {
+      both[i] = lex[i];
+      both[k + lex[i]] = static_cast<uint32_t>(i);
+    }
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&t->sf_lex), both.size() * sizeof(uint32_t));
     if (e == hipSuccess)
-      e = hipMemcpy(t->sf_lex, lex.data(), lex.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+      e = hipMemcpy(t->sf_lex, both.data(), both.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
     if (e != hipSuccess) return hip_fail(e, "sum-factorisation tables");
+    if (dtype == TG_F32 && tg::has_sumfact_mfma(n1, m)) {
+      // A operand of lane l at b for kind kd: the product of the 1D tables of (p, a) and (q, b),
+      // (p, q) = (l % 4, l % 16 / 4), a = l / 16, formed in f64 and rounded once.
+      std::vector<double> t2(3 * n1 * 64, 0.0);
+      for (int kd = 0; kd < 3; ++kd)
+        for (int b = 0; b < n1; ++b)
+          for (int l = 0; l < 64; ++l) {
+            const int pp = l & 3, qq = (l & 15) >> 2, a = l >> 4;
+            if (pp >= m || qq >= m || a >= n1) continue;
+            const double fa = tab1d[((kd == 1 ? m : 0) + pp) * n1 + a];
+            const double fb = tab1d[((kd == 2 ? m : 0) + qq) * n1 + b];
+            t2[(kd * n1 + b) * 64 + l] = fa * fb;
+          }
+      rc = upload<float>(t2, &t->sf_mfma);
+      if (rc) return rc;
+    }
     t->sf_n1 = n1;
     t->sf_m = m;
   }
@@ -308,6 +332,17 @@ int enqueue_block(const tg_geometry* g, const Block& b, tg_pointset* pts, Device
                       out->normals};
   const bool shift = f32 && !(options & TG_OPT_NO_CELL_SHIFT);
   if (takes_sumfact(g, tabs, factors, tile, options, f32)) {
+    if constexpr (sizeof(T) == 4) {
+      if (tabs->sf_mfma && !(options & TG_OPT_NO_MFMA)) {
+        const size_t k = static_cast<size_t>(b.k);
+        hipError_t e = tg::launch_sumfact_mfma(
+            tabs->sf_n1, tabs->sf_m, shift, static_cast<const float*>(g->nodes), b.dofmap,
+            static_cast<const float*>(tabs->sf_tab), tabs->sf_lex + k,
+            static_cast<const float*>(tabs->sf_mfma), tabs->tab64, p, o, stream);
+        if (e == hipSuccess) return TG_OK;
+        if (e != hipErrorNotSupported) return hip_fail(e, "sum-factorised matrix-core launch");
+      }
+    }
     hipError_t e = tg::launch_sumfact<T>(
         tabs->sf_n1, tabs->sf_m, shift, static_cast<const T*>(g->nodes), b.dofmap,
         static_cast<const T*>(tabs->sf_tab), tabs->sf_lex, tabs->tab64, p, o, stream);

@@ -121,6 +121,19 @@ hipError_t launch_sumfact(int n1, int m, bool shift, const T* nodes, const uint3
                           const T* tab1d, const uint32_t* lex, const double* tab64,
                           const FactorParams<T>& p, const FactorOutputs& out, hipStream_t stream);
 
+// Sum-factorised FP32 hexahedra with the second and third sums on the matrix cores
+// (sumfact_mfma.hip): n1 = 2..4, m = 2..4, tiles of 16; alex[l] is the A.5 index of the node at
+// lexicographic position l = (a n1 + b) n1 + c (the inverse of lex); tab2d is [3][n1][64], the
+// A operand of lane l for kind (phi x phi, phi' x phi, phi x phi') at b: the 2D table value
+// T[(p, q) = l % 16][a = l / 16] (0 outside p, q < m, a < n1). hipErrorNotSupported for other
+// (n1, m).
+bool has_sumfact_mfma(int n1, int m);
+hipError_t launch_sumfact_mfma(int n1, int m, bool shift, const float* nodes,
+                               const uint32_t* dofmap, const float* tab1d, const uint32_t* alex,
+                               const float* tab2d, const double* tab64,
+                               const FactorParams<float>& p, const FactorOutputs& out,
+                               hipStream_t stream);
+
 template <typename T>
 hipError_t launch_evector(const T* nodes, const uint32_t* dofmap, uint64_t num_cells,
                           uint64_t padded, uint32_t k, int gdim, uint32_t last_node, T* out,

@@ -30,14 +30,16 @@ struct Block {
 //
 // Hexahedra at a tensor-product point set also get the sum-factorisation tables: the 1D basis
 // values and derivatives at the 1D points [2][m][n1] in the geometry's type and the
-// lexicographic position of each A.5 node [k] (sf_m = 0 when the point set is not a tensor
-// product of one 1D set with the first coordinate fastest, or m > 4, or n1 > 4).
+// lexicographic position of each A.5 node [k] followed by its inverse [k] (sf_m = 0 when the
+// point set is not a tensor product of one 1D set with the first coordinate fastest, or m > 4,
+// or n1 > 4); FP32 tables also get the matrix-core kernel's A operands sf_mfma [3][n1][64].
 struct DeviceTables {
   void* tab = nullptr;
   double* tab64 = nullptr;
   void* ref_normals = nullptr;
   void* sf_tab = nullptr;
   uint32_t* sf_lex = nullptr;
+  void* sf_mfma = nullptr;
   int sf_n1 = 0, sf_m = 0;
   ~DeviceTables();
 };

@@ -121,7 +121,10 @@ enum {
   TG_OPT_POINTS_RELATIVE = 1u << 10,
   /* Plans only: run every block in the one persistent kernel over the work list, where every
    * block's element has a kernel of that form (otherwise the plan stays per block). */
-  TG_OPT_PLAN_PERSISTENT = 1u << 11
+  TG_OPT_PLAN_PERSISTENT = 1u << 11,
+  /* FP32 sum-factorised hexahedra: keep the second and third sums on the vector ALU instead of
+   * the matrix cores (v_mfma_f32_16x16x4f32), which run them by default. */
+  TG_OPT_NO_MFMA = 1u << 12
 };
 
 typedef struct tg_geometry tg_geometry;

@@ -62,7 +62,9 @@ def test_config2_p2_tets(cuda, config2, dtype):
 @pytest.mark.parametrize("dtype", ["f32", "f64"])
 def test_config3_q3_hexahedra(cuda, dtype):
     """Config 3: Q3 hexahedra 96^3 = 884,736 cells at 4^3 Gauss points (the sum-factorised
-    kernel), against the oracle on every 211th cell. FP32 1e-6, FP64 1e-12 (X, J, det J)."""
+    kernel; FP32 with its last two sums on the matrix cores, and on the vector ALU under
+    TG_OPT_NO_MFMA), against the oracle on every 211th cell. FP32 1e-6, FP64 1e-12 (X, J,
+    det J)."""
     import torch
 
     from tests.helpers import oracle_factors_sampled
@@ -73,21 +75,22 @@ def test_config3_q3_hexahedra(cuda, dtype):
     g = api.geometry_from_host(mesh, dtype=tdt)
     ps = api.PointSet.gauss(5, 7)
     assert len(ps) == 64
-    fields = api.geometric_factors_block(g, 0, ps, NAMES)
-    torch.cuda.synchronize()
     cells = np.arange(0, mesh.blocks[0].num_cells, 211)
-    got = {}
-    for n, f in fields.items():
-        a = f.data.cpu().numpy().reshape(-1, 64, f.len, 16)  # [tile][q][LEN][16]
-        got[n] = a[cells // 16, :, :, cells % 16].astype(np.float64)  # [cells][q][LEN]
-    del fields
     want = oracle_factors_sampled(g, mesh, 0, ps, NAMES, cells)
-    err = rel_errs(got, want)
-    pw = pointwise_rel(got["detj"], want["detj"])
-    print(f"config 3 {dtype} rel_err {err} det J pointwise {pw:.3e}")
     tol = 1e-6 if dtype == "f32" else 1e-12
-    for n, e in err.items():
-        assert e <= tol, (n, e)
+    for path, opt in (("default", 0), ("vector ALU", api.OPT_NO_MFMA))[: 2 if dtype == "f32" else 1]:
+        fields = api.geometric_factors_block(g, 0, ps, NAMES, options=opt)
+        torch.cuda.synchronize()
+        got = {}
+        for n, f in fields.items():
+            a = f.data.cpu().numpy().reshape(-1, 64, f.len, 16)  # [tile][q][LEN][16]
+            got[n] = a[cells // 16, :, :, cells % 16].astype(np.float64)  # [cells][q][LEN]
+        del fields
+        err = rel_errs(got, want)
+        pw = pointwise_rel(got["detj"], want["detj"])
+        print(f"config 3 {dtype} {path} rel_err {err} det J pointwise {pw:.3e}")
+        for n, e in err.items():
+            assert e <= tol, (path, n, e)
 
 
 @pytest.mark.parametrize("dtype", ["f32", "f64"])

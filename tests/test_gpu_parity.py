@@ -360,6 +360,33 @@ def test_fp64_fixup_on_near_degenerate_cells(cuda):
     assert e_plain > e_fixed
 
 
+@pytest.mark.parametrize("p", [2, 3])
+@pytest.mark.parametrize("opt", ["mfma", "valu"])
+def test_fp64_fixup_on_near_degenerate_hexahedra(cuda, p, opt):
+    """The sum-factorised FP32 kernels flag near-degenerate points too: on a hexahedral box
+    flattened in z by 1e-4 every point is flagged (all m^3 points of every cell, so the matrix-core
+    kernel's after-loop recomputation runs for every bit of its mask), and the pointwise det J
+    matches the oracle to 1e-6 where the FP32 value alone does not."""
+    import torch
+
+    mesh = volume_mesh(HEX, p, n=2)
+    x = mesh.nodes.copy()
+    x[:, 2] = 0.3 + 1e-4 * x[:, 2]
+    mesh.nodes = x
+    g = api.geometry_from_host(mesh, dtype=torch.float32)
+    ps = api.PointSet.gauss(HEX, 7)
+    base = api.OPT_SUMFACT_ALWAYS | (api.OPT_NO_MFMA if opt == "valu" else 0)
+    want = oracle_factors(g, mesh, 0, ps, ["detj"])["detj"]
+    g.set_fixup_threshold(1e-2)
+    fixed = gpu_factors(g, 0, ps, ["points", "jacobians", "detj"], options=base)["detj"]
+    plain = gpu_factors(g, 0, ps, ["points", "jacobians", "detj"],
+                        options=base | api.OPT_NO_FP64_FIXUP)["detj"]
+    e_fixed, e_plain = pointwise_rel(fixed, want), pointwise_rel(plain, want)
+    print(f"hex p={p} {opt} near-degenerate det J pointwise: fixed {e_fixed} fp32 only {e_plain}")
+    assert e_fixed <= 1e-6
+    assert e_plain > e_fixed
+
+
 def test_runs_are_bitwise_identical(cuda):
     """Two evaluations give bitwise equal fields (tetraloma/tests/determinism.rs)."""
     import torch
@@ -369,6 +396,14 @@ def test_runs_are_bitwise_identical(cuda):
     ps = api.PointSet.gauss(HEX, 4)
     a = gpu_factors(g, 0, ps, ["points", "jacobians", "detj", "pinv"])
     b = gpu_factors(g, 0, ps, ["points", "jacobians", "detj", "pinv"])
+    for n in a:
+        assert np.array_equal(a[n], b[n])
+    # the matrix-core sum-factorised kernel (Q3 at 4^3 points)
+    mesh = volume_mesh(HEX, 3)
+    g = api.geometry_from_host(mesh, dtype=torch.float32)
+    ps = api.PointSet.gauss(HEX, 7)
+    a = gpu_factors(g, 0, ps, ["points", "jacobians", "detj"])
+    b = gpu_factors(g, 0, ps, ["points", "jacobians", "detj"])
     for n in a:
         assert np.array_equal(a[n], b[n])
 
@@ -396,9 +431,11 @@ def test_mesh_level_loops(cuda):
 @pytest.mark.parametrize("dtype", ["f64", "f32"])
 def test_sumfact_hexahedra(cuda, p, m, dtype):
     """Hexahedra at m^3 Gauss points with tiles of 16: the sum-factorised kernel for X, J and
-    det J (forced by TG_OPT_SUMFACT_ALWAYS, for every (p, m)), the default choice and the dense
-    kernels (TG_OPT_NO_SUMFACT), each against the oracle. 27 cells: the second tile is padded.
-    Tolerances: FP64 1e-12; FP32 1e-6 (X, J, det J), 4e-6 for the dense kernels on Q3."""
+    det J (forced by TG_OPT_SUMFACT_ALWAYS, for every (p, m); in FP32 its second and third sums
+    run on the matrix cores for m >= 2, and on the vector ALU under TG_OPT_NO_MFMA), the default
+    choice and the dense kernels (TG_OPT_NO_SUMFACT), each against the oracle. 27 cells: the
+    second tile is padded. Tolerances: FP64 1e-12; FP32 1e-6 (X, J, det J), 4e-6 for the dense
+    kernels on Q3."""
     import torch
 
     mesh = volume_mesh(HEX, p, n=3)
@@ -408,24 +445,28 @@ def test_sumfact_hexahedra(cuda, p, m, dtype):
     assert len(ps) == m ** 3
     names = ["points", "jacobians", "detj"]
     got = gpu_factors(g, 0, ps, names, options=api.OPT_SUMFACT_ALWAYS)
+    valu = gpu_factors(g, 0, ps, names, options=api.OPT_SUMFACT_ALWAYS | api.OPT_NO_MFMA)
     default = gpu_factors(g, 0, ps, names)
     dense = gpu_factors(g, 0, ps, names, options=api.OPT_NO_SUMFACT)
     want = oracle_factors(g, mesh, 0, ps, names)
     err = rel_errs(got, want)
     err_dense = rel_errs(dense, want)
-    print(f"hex p={p} m={m} {dtype} sumfact {err} dense {err_dense}")
+    print(f"hex p={p} m={m} {dtype} sumfact {err} vector-ALU sumfact {rel_errs(valu, want)} "
+          f"dense {err_dense}")
     tol = 1e-12 if dtype == "f64" else 1e-6
     # The dense FP32 contraction over 64 nodes (shifted to node 0) rounds about twice as much
     # as the sum-factorised one; it runs for Q3 only under TG_OPT_NO_SUMFACT.
     tol_dense = 4e-6 if (dtype == "f32" and p == 3) else tol
-    for res, t in ((got, tol), (default, tol), (dense, tol_dense)):
+    for res, t in ((got, tol), (valu, tol), (default, tol), (dense, tol_dense)):
         for name, e in rel_errs(res, want).items():
             assert e <= t, (name, e)
-    # The padding of the last tile holds the last cell (field.rs:234-253).
-    fields = api.geometric_factors_block(g, 0, ps, names, options=api.OPT_SUMFACT_ALWAYS)
-    torch.cuda.synchronize()
-    raw = fields["detj"].data.cpu().numpy().astype(np.float64).reshape(2, m ** 3, 1, 16)
-    assert np.array_equal(raw[1, :, 0, 11:], np.repeat(raw[1, :, 0, 10:11], 5, axis=1))
+    # The padding of the last tile holds the last cell (field.rs:234-253), on both sum paths.
+    for opt in (api.OPT_SUMFACT_ALWAYS, api.OPT_SUMFACT_ALWAYS | api.OPT_NO_MFMA):
+        fields = api.geometric_factors_block(g, 0, ps, names, options=opt)
+        torch.cuda.synchronize()
+        for name, ln in (("points", 3), ("jacobians", 9), ("detj", 1)):
+            raw = fields[name].data.cpu().numpy().astype(np.float64).reshape(2, m ** 3, ln, 16)
+            assert np.array_equal(raw[1, :, :, 11:], np.repeat(raw[1, :, :, 10:11], 5, axis=2))
     # Not a tensor-product point set: the dense kernel, same results to rounding.
     pts = ps.points.copy()
     pts[[0, -1]] = pts[[-1, 0]]
