@@ -242,4 +242,4 @@ def load() -> ctypes.CDLL:
 
 
 def last_error() -> str:
-    return load().tg_last_error_message().decode()
+    return load().tg_last_error_message().decode(errors="replace")

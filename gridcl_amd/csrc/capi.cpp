@@ -287,6 +287,14 @@ static bool sumfact_pays(bool f32, int n1, int m) {
   return f32 ? (m >= 4 || (n1 == 3 && m == 3)) : n1 >= 3;
 }
 
+// Whether the FP32 sum-factorised hexahedra take their second and third sums on the matrix cores
+// (measured on MI355X, X + J + det J, us per launch, matrix cores / vector ALU):
+//   degree 2 at 3^3 points 139 / 170 (373,248 cells), degree 3 at 4^3 points 173 / 172
+//   (262,144 cells) and 690 / 700 (884,736 cells);
+//   elsewhere the vector ALU is as fast or faster (degree 3 at 2^3 and 3^3 points 106 / 73 and
+//   122 / 114, degree 2 at 4^3 points 255 / 237, degree 1 at 4^3 points 574 / 497).
+static bool mfma_pays(int n1, int m) { return (n1 == 3 && m == 3) || (n1 == 4 && m == 4); }
+
 // The scalars of one factor launch over block b.
 template <typename T>
 static tg::FactorParams<T> block_params(const tg_geometry* g, const Block& b,
@@ -333,7 +341,8 @@ int enqueue_block(const tg_geometry* g, const Block& b, tg_pointset* pts, Device
   const bool shift = f32 && !(options & TG_OPT_NO_CELL_SHIFT);
   if (takes_sumfact(g, tabs, factors, tile, options, f32)) {
     if constexpr (sizeof(T) == 4) {
-      if (tabs->sf_mfma && !(options & TG_OPT_NO_MFMA)) {
+      if (tabs->sf_mfma && !(options & TG_OPT_NO_MFMA) &&
+          ((options & TG_OPT_SUMFACT_ALWAYS) || mfma_pays(tabs->sf_n1, tabs->sf_m))) {
         const size_t k = static_cast<size_t>(b.k);
         hipError_t e = tg::launch_sumfact_mfma(
             tabs->sf_n1, tabs->sf_m, shift, static_cast<const float*>(g->nodes), b.dofmap,

@@ -1,9 +1,20 @@
-// The MSH 4.1 parser (tetraloma-io/src/gmsh/parse.rs:14-620): ASCII and binary files, either
-// byte order and either size_t width, into the file's sections as stored.
+// Reading the sections of an MSH 4.1 file (the format tetraloma-io/src/gmsh/parse.rs reads) into
+// the file model of gmsh_internal.hpp: text or binary data, either byte order, 4- or 8-byte
+// size_t.
 //
-// Counts are checked against the bytes left before anything is sized from them, so a damaged file
-// gives a Parse error (with the byte offset where it was found), never a crash or a huge
-// allocation. Unknown sections are skipped; $PhysicalNames is text in binary files too.
+// Design. The format writes every section with three kinds of numbers (int, size_t, double),
+// as text tokens or as fixed-width binary values. Two number decoders with the same interface
+// (TextNumbers, BinaryNumbers) are chosen once from $MeshFormat, and every section decoder is
+// written once as a template over the decoder, so the text and binary paths cannot drift apart.
+// Binary arrays of tags and coordinates are converted in bulk. Records shared between sections
+// are decoded by shared helpers: an entity's bounding box and tag lists ($Entities and
+// $PartitionedEntities), the four-number block header and the announced totals ($Nodes and
+// $Elements). The sections are dispatched through a table; names not in it are skipped to their
+// $End line.
+//
+// Safety. Every count is checked against the bytes left (each value takes at least one byte and a
+// separator in text, its width in binary) before anything is sized from it, so a damaged file is a
+// Parse error with the byte offset where it was found, never a crash or a large allocation.
 #include <charconv>
 #include <cstring>
 #include <limits>
@@ -16,476 +27,509 @@ namespace tg::gmsh {
 
 namespace {
 
-constexpr int kParse = 15;
-constexpr int kUnsupportedElementType = 16;
-// The fewest bytes a text value takes: one character and one separator (parse.rs:23).
-constexpr size_t kToken = 2;
+constexpr int kParse = 15;  // TG_ERR_PARSE
 
-bool is_space(unsigned char c) { return c == ' ' || c == '\t' || c == '\n' || c == '\r' || c == '\f'; }
+[[noreturn]] void parse_error(size_t offset, const std::string& what) {
+  throw Error{kParse, "Parse: byte " + std::to_string(offset) + " of the Gmsh file: " + what,
+              static_cast<int64_t>(offset)};
+}
 
-class Cursor {
- public:
-  Cursor(const unsigned char* b, size_t n) : b_(b), n_(n) {}
+bool blank(unsigned char c) { return c == ' ' || c == '\n' || c == '\r' || c == '\t' || c == '\f'; }
 
-  [[noreturn]] void error_at(size_t at, const std::string& m) const {
-    throw Error{kParse, "Parse: Gmsh, byte " + std::to_string(at) + ": " + m,
-                static_cast<int64_t>(at)};
+// A file's word quoted in a message: printable ASCII as is, other bytes as \xNN, at most 40
+// bytes of it (a damaged file may hold anything).
+std::string quoted(std::string_view w) {
+  static const char hex[] = "0123456789abcdef";
+  std::string out = "\"";
+  for (size_t i = 0; i < w.size() && i < 40; ++i) {
+    const unsigned char c = static_cast<unsigned char>(w[i]);
+    if (c >= 0x20 && c < 0x7f && c != '\\' && c != '"') {
+      out.push_back(static_cast<char>(c));
+    } else {
+      out += "\\x";
+      out.push_back(hex[c >> 4]);
+      out.push_back(hex[c & 15]);
+    }
   }
-  [[noreturn]] void error(const std::string& m) const { error_at(pos_, m); }
+  if (w.size() > 40) out += "...";
+  return out + "\"";
+}
 
-  size_t pos() const { return pos_; }
-  size_t remaining() const { return n_ - pos_; }
-  bool binary() const { return binary_; }
+// The file's bytes and the read position.
+struct Bytes {
+  const unsigned char* data;
+  size_t size;
+  size_t at = 0;
 
-  void skip_whitespace() {
-    while (pos_ < n_ && is_space(b_[pos_])) ++pos_;
+  size_t left() const { return size - at; }
+  void skip_blanks() {
+    while (at < size && blank(data[at])) ++at;
   }
-  bool at_end() {
-    skip_whitespace();
-    return pos_ == n_;
+  // The next blank-separated word, and where it starts.
+  std::string_view word(size_t* start) {
+    skip_blanks();
+    *start = at;
+    while (at < size && !blank(data[at])) ++at;
+    if (*start == at) parse_error(at, "the file ends before the next value");
+    return {reinterpret_cast<const char*>(data + *start), at - *start};
   }
-  std::string_view token(size_t* at) {
-    skip_whitespace();
-    const size_t start = pos_;
-    while (pos_ < n_ && !is_space(b_[pos_])) ++pos_;
-    if (start == pos_) error("unexpected end of file");
-    *at = start;
-    return std::string_view(reinterpret_cast<const char*>(b_ + start), pos_ - start);
-  }
-
-  template <typename T>
-  T parse_text(const char* what) {
-    size_t at;
-    std::string_view t = token(&at);
-    std::string_view digits = t;
-    if (!digits.empty() && digits[0] == '+') digits.remove_prefix(1);
-    T v{};
-    auto r = std::from_chars(digits.data(), digits.data() + digits.size(), v);
-    if (r.ec != std::errc() || r.ptr != digits.data() + digits.size() || digits.empty() ||
-        (t.size() > 1 && t[0] == '+' && t[1] == '-'))
-      error_at(at, std::string("expected ") + what + ", found \"" + std::string(t) + "\"");
-    return v;
-  }
-
-  const unsigned char* take(size_t n) {
-    if (remaining() < n) error("unexpected end of file");
-    const unsigned char* p = b_ + pos_;
-    pos_ += n;
+  const unsigned char* consume(size_t n) {
+    if (left() < n) parse_error(at, "the file ends " + std::to_string(n - left()) +
+                                        " bytes early");
+    const unsigned char* p = data + at;
+    at += n;
     return p;
   }
+};
 
-  uint64_t raw(size_t width) {
-    const unsigned char* p = take(width);
-    uint64_t v = 0;
-    if (big_)
-      for (size_t i = 0; i < width; ++i) v = (v << 8) | p[i];
-    else
-      for (size_t i = width; i-- > 0;) v = (v << 8) | p[i];
+// Tags (node, element, periodic node) are size_t values from 1 to 2^63 - 1.
+int64_t checked_tag(uint64_t v, size_t offset, const char* kind) {
+  if (v == 0 || v > static_cast<uint64_t>(std::numeric_limits<int64_t>::max()))
+    parse_error(offset, std::string(kind) + " tag " + std::to_string(v) + " is outside 1..2^63-1");
+  return static_cast<int64_t>(v);
+}
+
+// Numbers written as text tokens.
+class TextNumbers {
+ public:
+  explicit TextNumbers(Bytes& b) : b_(b) {}
+  static constexpr bool kBinary = false;
+  // The fewest bytes an int, size_t or double takes: one character and a separator.
+  size_t int_bytes() const { return 2; }
+  size_t size_bytes() const { return 2; }
+  size_t real_bytes() const { return 2; }
+  Bytes& bytes() { return b_; }
+
+  int32_t i32() { return number<int32_t>("an int"); }
+  uint64_t word() { return number<uint64_t>("a size_t"); }
+  double real() { return number<double>("a double"); }
+  void reals(double* out, size_t n) {
+    for (size_t i = 0; i < n; ++i) out[i] = real();
+  }
+  // n tags, written to out[i * stride + offset_of(i)] by `place`.
+  template <class Place>
+  void tags(size_t n, const char* kind, Place place) {
+    for (size_t i = 0; i < n; ++i) {
+      const size_t at = next_offset();
+      place(i, checked_tag(word(), at, kind));
+    }
+  }
+  size_t next_offset() {
+    b_.skip_blanks();
+    return b_.at;
+  }
+  void section_body() {}  // text data may follow the header on the same line
+
+ private:
+  template <typename T>
+  T number(const char* what) {
+    size_t start;
+    std::string_view w = b_.word(&start);
+    std::string_view digits = w;
+    if (digits.size() > 1 && digits[0] == '+' && digits[1] != '-' && digits[1] != '+')
+      digits.remove_prefix(1);
+    T v{};
+    const auto res = std::from_chars(digits.data(), digits.data() + digits.size(), v);
+    if (res.ec != std::errc() || res.ptr != digits.data() + digits.size())
+      parse_error(start, quoted(w) + " is not " + what);
     return v;
   }
+  Bytes& b_;
+};
 
-  int32_t int_() {
-    if (binary_) return static_cast<int32_t>(static_cast<uint32_t>(raw(4)));
-    return parse_text<int32_t>("an integer");
-  }
-  uint64_t size() {
-    if (binary_) return raw(size_t_);
-    return parse_text<uint64_t>("an unsigned integer");
-  }
-  double float_() {
-    if (binary_) {
-      const uint64_t bits = raw(8);
-      double d;
-      std::memcpy(&d, &bits, 8);
-      return d;
-    }
-    return parse_text<double>("a number");
-  }
+// Numbers written as binary values of fixed width (int 4 bytes, size_t 4 or 8, double 8).
+class BinaryNumbers {
+ public:
+  BinaryNumbers(Bytes& b, bool big_endian, size_t size_width)
+      : b_(b), big_(big_endian), sw_(size_width) {}
+  static constexpr bool kBinary = true;
+  size_t int_bytes() const { return 4; }
+  size_t size_bytes() const { return sw_; }
+  size_t real_bytes() const { return 8; }
+  Bytes& bytes() { return b_; }
 
-  // n tags (size_t values >= 1) into out, the i-th at out[i + i / run * skip]: binary arrays are
-  // converted in one pass (run = 0: contiguous).
-  void tags_into(int64_t* out, size_t n, const char* what) { tags_split(out, nullptr, n, 0, what); }
-
-  // Rows of (1 + k) tags: the first of each row into `first`, the other k into `rest`.
-  void tags_split(int64_t* rest, int64_t* first, size_t n, size_t k, const char* what) {
-    const size_t per = first ? k + 1 : 1;
-    auto put = [&](size_t i, int64_t v) {
-      if (!first) {
-        rest[i] = v;
-      } else {
-        const size_t r = i / per, j = i % per;
-        if (j == 0)
-          first[r] = v;
-        else
-          rest[r * k + j - 1] = v;
-      }
-    };
-    if (!binary_) {
-      for (size_t i = 0; i < n * per; ++i) put(i, tag(what));
-      return;
-    }
-    const size_t start = pos_;
-    const unsigned char* p = take(n * per * size_t_);
-    for (size_t i = 0; i < n * per; ++i) {
-      const unsigned char* q = p + i * size_t_;
-      uint64_t v = 0;
-      if (big_)
-        for (size_t j = 0; j < size_t_; ++j) v = (v << 8) | q[j];
-      else if (size_t_ == 8)
-        std::memcpy(&v, q, 8);
-      else {
-        uint32_t w;
-        std::memcpy(&w, q, 4);
-        v = w;
-      }
-      if (v < 1 || v > static_cast<uint64_t>(std::numeric_limits<int64_t>::max()))
-        error_at(start + i * size_t_, std::string(what) + " tag " + std::to_string(v) +
-                                          ": tags must be 1 to 2^63 - 1");
-      put(i, static_cast<int64_t>(v));
-    }
+  int32_t i32() { return static_cast<int32_t>(static_cast<uint32_t>(load(b_.consume(4), 4))); }
+  uint64_t word() { return load(b_.consume(sw_), sw_); }
+  double real() {
+    const uint64_t bits = load(b_.consume(8), 8);
+    double d;
+    std::memcpy(&d, &bits, 8);
+    return d;
   }
-
-  // n doubles into out.
-  void floats_into(double* out, size_t n) {
-    if (!binary_) {
-      for (size_t i = 0; i < n; ++i) out[i] = float_();
-      return;
-    }
-    const unsigned char* p = take(n * 8);
+  void reals(double* out, size_t n) {
+    if (n == 0) return;  // (out may be the null data of an empty vector)
+    const unsigned char* p = b_.consume(n * 8);
     if (!big_) {
       std::memcpy(out, p, n * 8);
       return;
     }
     for (size_t i = 0; i < n; ++i) {
-      uint64_t v = 0;
-      for (size_t j = 0; j < 8; ++j) v = (v << 8) | p[i * 8 + j];
-      std::memcpy(&out[i], &v, 8);
+      const uint64_t bits = load(p + 8 * i, 8);
+      std::memcpy(out + i, &bits, 8);
     }
   }
-
-  // The line break after a section header; binary data starts after it.
-  void header_end() {
-    if (!binary_) return;
-    if (pos_ < n_ && b_[pos_] == '\r') ++pos_;
-    if (pos_ >= n_ || b_[pos_] != '\n') error("expected a line break after the section header");
-    ++pos_;
+  template <class Place>
+  void tags(size_t n, const char* kind, Place place) {
+    const size_t base = b_.at;
+    const unsigned char* p = b_.consume(n * sw_);
+    for (size_t i = 0; i < n; ++i) place(i, checked_tag(load(p + i * sw_, sw_), base + i * sw_, kind));
   }
-
-  int dim() {
-    const size_t at = pos_;
-    const int32_t d = int_();
-    if (d < 0 || d > 3) error_at(at, "dimension " + std::to_string(d) + " is not 0-3");
-    return d;
+  size_t next_offset() { return b_.at; }
+  // Binary data starts right after the line break that ends the section's header line.
+  void section_body() {
+    if (b_.at < b_.size && b_.data[b_.at] == '\r') ++b_.at;
+    if (b_.at >= b_.size || b_.data[b_.at] != '\n')
+      parse_error(b_.at, "binary data must start on the line after the section name");
+    ++b_.at;
   }
-
-  // A count of items of at least `min_bytes` bytes each that the rest of the file can hold.
-  size_t count(size_t min_bytes) {
-    const size_t at = pos_;
-    const uint64_t n = size();
-    const bool fits = min_bytes == 0 ? true
-                                     : n <= remaining() / min_bytes;
-    if (!fits) error_at(at, "count " + std::to_string(n) + " exceeds the rest of the file");
-    return static_cast<size_t>(n);
-  }
-
-  int64_t tag(const char* what) {
-    const size_t at = pos_;
-    const uint64_t t = size();
-    if (t < 1 || t > static_cast<uint64_t>(std::numeric_limits<int64_t>::max()))
-      error_at(at, std::string(what) + " tag " + std::to_string(t) +
-                       ": tags must be 1 to 2^63 - 1");
-    return static_cast<int64_t>(t);
-  }
-
-  // Bytes one value takes at least: int, size_t, double.
-  size_t w_int() const { return binary_ ? 4 : kToken; }
-  size_t w_size() const { return binary_ ? size_t_ : kToken; }
-  size_t w_float() const { return binary_ ? 8 : kToken; }
-
-  void end(const std::string& section) {
-    size_t at;
-    std::string_view t = token(&at);
-    if (t != "$End" + section)
-      error_at(at, "expected $End" + section + ", found \"" + std::string(t) + "\"");
-  }
-
-  void skip_section(const std::string& section) {
-    const std::string end = "$End" + section;
-    const size_t at = pos_;
-    const void* hit = nullptr;
-    if (remaining() >= end.size())
-      hit = memmem(b_ + pos_, remaining(), end.data(), end.size());
-    if (!hit) error_at(at, "no " + end);
-    pos_ = static_cast<const unsigned char*>(hit) - b_ + end.size();
-  }
-
-  void set_binary(bool big, size_t st) {
-    binary_ = true;
-    big_ = big;
-    size_t_ = st;
-  }
-  void set_text(bool text) { binary_ = !text && size_t_ != 0; }
-  bool big() const { return big_; }
 
  private:
-  const unsigned char* b_;
-  size_t n_;
-  size_t pos_ = 0;
-  bool binary_ = false, big_ = false;
-  size_t size_t_ = 0;
+  uint64_t load(const unsigned char* p, size_t w) const {
+    uint64_t v = 0;
+    if (big_) {
+      for (size_t i = 0; i < w; ++i) v = (v << 8) | p[i];
+    } else if (w == 8) {
+      std::memcpy(&v, p, 8);
+    } else {
+      uint32_t u;
+      std::memcpy(&u, p, 4);
+      v = u;
+    }
+    return v;
+  }
+  Bytes& b_;
+  bool big_;
+  size_t sw_;
 };
 
-void mesh_format(Cursor& c) {
-  size_t at;
-  std::string_view version = c.token(&at);
-  if (version != "4.1")
-    c.error_at(at, "MSH version " + std::string(version) + ": only 4.1 is supported");
-  at = c.pos();
-  const int32_t file_type = c.int_();
-  const int32_t data_size = c.int_();
-  if (data_size != 4 && data_size != 8)
-    c.error_at(at, "data size " + std::to_string(data_size) + " is not 4 or 8");
-  if (file_type == 1) {
-    c.set_binary(false, static_cast<size_t>(data_size));
-    c.header_end();
-    const size_t at1 = c.pos();
-    const unsigned char* one = c.take(4);
-    if (one[0] == 1 && one[1] == 0 && one[2] == 0 && one[3] == 0)
-      c.set_binary(false, static_cast<size_t>(data_size));
-    else if (one[0] == 0 && one[1] == 0 && one[2] == 0 && one[3] == 1)
-      c.set_binary(true, static_cast<size_t>(data_size));
-    else
-      c.error_at(at1, "an endianness marker that is not 1");
-  } else if (file_type != 0) {
-    c.error_at(at, "file type " + std::to_string(file_type) + " is not 0 or 1");
-  }
-  c.end("MeshFormat");
+// A count that the rest of the file can hold at `per_item` bytes per item at least.
+template <class Num>
+size_t bounded_count(Num& in, size_t per_item) {
+  const size_t at = in.next_offset();
+  const uint64_t n = in.word();
+  const size_t left = in.bytes().left();
+  if (per_item > 0 && n > left / per_item)
+    parse_error(at, "a count of " + std::to_string(n) + " does not fit in the " +
+                        std::to_string(left) + " bytes left");
+  return static_cast<size_t>(n);
 }
 
-std::vector<PhysicalName> physical_names(Cursor& c) {
-  const bool binary = c.binary();
-  c.set_text(true);
-  std::vector<PhysicalName> names;
-  const size_t n = c.count(3 * kToken + 2);
-  for (size_t i = 0; i < n; ++i) {
-    PhysicalName p;
-    p.dim = c.dim();
-    p.tag = c.int_();
-    c.skip_whitespace();
-    const size_t at = c.pos();
-    if (c.remaining() == 0 || *c.take(1) != '"') c.error_at(at, "expected a quoted name");
-    std::string name;
-    bool closed = false;
-    while (c.remaining() > 0) {
-      const unsigned char ch = *c.take(1);
-      if (ch == '"') {
-        closed = true;
-        break;
-      }
-      name.push_back(static_cast<char>(ch));
-    }
-    if (!closed) c.error_at(at, "unterminated name");
-    if (name.find_first_of("\r\n") != std::string::npos) c.error_at(at, "a name with a line break");
-    p.name = std::move(name);
-    names.push_back(std::move(p));
-  }
-  c.end("PhysicalNames");
-  c.set_text(!binary);
-  return names;
+template <class Num>
+int dimension(Num& in) {
+  const size_t at = in.next_offset();
+  const int32_t d = in.i32();
+  if (d < 0 || d > 3) parse_error(at, "dimension " + std::to_string(d) + " (0 to 3 expected)");
+  return d;
 }
 
-std::vector<int32_t> int_list(Cursor& c) {
-  const size_t n = c.count(c.w_int());
-  std::vector<int32_t> v(n);
-  for (size_t i = 0; i < n; ++i) v[i] = c.int_();
-  return v;
+// A counted list of ints (physical tags, bounding entities, partitions).
+template <class Num>
+std::vector<int32_t> int_list(Num& in) {
+  std::vector<int32_t> out(bounded_count(in, in.int_bytes()));
+  for (int32_t& v : out) v = in.i32();
+  return out;
 }
 
-void entity_body(Cursor& c, Entity& e, int dim) {
-  for (int g = 0; g < 3; ++g) e.min[g] = c.float_();
-  for (int g = 0; g < 3; ++g) e.max[g] = dim == 0 ? e.min[g] : c.float_();
-  e.physical = int_list(c);
-  if (dim > 0) e.boundary = int_list(c);
+// The part of an entity record after its tag (and, partitioned, its parent and partitions):
+// the point (dimension 0) or bounding box, the physical tags and, above dimension 0, the
+// bounding entities.
+template <class Num>
+void entity_rest(Num& in, int dim, Entity& e) {
+  in.reals(e.min, 3);
+  if (dim == 0)
+    std::memcpy(e.max, e.min, sizeof e.min);
+  else
+    in.reals(e.max, 3);
+  e.physical = int_list(in);
+  if (dim > 0) e.boundary = int_list(in);
 }
 
-void entities(Cursor& c, File& f) {
+// The four per-dimension entity counts that open $Entities and $PartitionedEntities.
+template <class Num>
+void dimension_counts(Num& in, size_t min_record, size_t (&counts)[4]) {
+  for (size_t& c : counts) c = bounded_count(in, min_record);
+}
+
+template <class Num>
+void read_entities(Num& in, File& f) {
   size_t counts[4];
-  for (size_t& n : counts) n = c.count(c.w_int() + 3 * c.w_float());
+  dimension_counts(in, in.int_bytes() + 3 * in.real_bytes(), counts);
   for (int dim = 0; dim < 4; ++dim) {
-    f.entities[dim].clear();
-    for (size_t i = 0; i < counts[dim]; ++i) {
-      Entity e;
-      e.tag = c.int_();
-      entity_body(c, e, dim);
-      f.entities[dim].push_back(std::move(e));
+    std::vector<Entity>& list = f.entities[dim];
+    list.assign(counts[dim], Entity{});
+    for (Entity& e : list) {
+      e.tag = in.i32();
+      entity_rest(in, dim, e);
     }
   }
   f.has_entities = true;
-  c.end("Entities");
 }
 
-void partitioned_entities(Cursor& c, File& f) {
-  f.num_partitions = c.count(0);
-  const size_t ghosts = c.count(2 * c.w_int());
-  f.ghost_entities.clear();
-  for (size_t i = 0; i < ghosts; ++i) {
-    const int32_t tag = c.int_();
-    const int32_t part = c.int_();
-    f.ghost_entities.push_back({tag, part});
+template <class Num>
+void read_partitioned_entities(Num& in, File& f) {
+  f.num_partitions = in.word();
+  f.ghost_entities.assign(bounded_count(in, 2 * in.int_bytes()), {0, 0});
+  for (auto& g : f.ghost_entities) {
+    g[0] = in.i32();
+    g[1] = in.i32();
   }
   size_t counts[4];
-  for (size_t& n : counts) n = c.count(3 * c.w_int() + 3 * c.w_float());
+  dimension_counts(in, 3 * in.int_bytes() + 3 * in.real_bytes(), counts);
   for (int dim = 0; dim < 4; ++dim) {
-    f.partitioned[dim].clear();
-    for (size_t i = 0; i < counts[dim]; ++i) {
-      PartitionedEntity p;
-      p.entity.tag = c.int_();
-      p.parent_dim = c.dim();
-      p.parent_tag = c.int_();
-      p.partitions = int_list(c);
-      entity_body(c, p.entity, dim);
-      f.partitioned[dim].push_back(std::move(p));
+    std::vector<PartitionedEntity>& list = f.partitioned[dim];
+    list.assign(counts[dim], PartitionedEntity{});
+    for (PartitionedEntity& pe : list) {
+      pe.entity.tag = in.i32();
+      pe.parent_dim = dimension(in);
+      pe.parent_tag = in.i32();
+      pe.partitions = int_list(in);
+      entity_rest(in, dim, pe.entity);
     }
   }
   f.has_partitioned = true;
-  c.end("PartitionedEntities");
 }
 
-void periodic(Cursor& c, File& f) {
-  const size_t n = c.count(3 * c.w_int() + 2 * c.w_size());
+template <class Num>
+void read_periodic(Num& in, File& f) {
+  const size_t links = bounded_count(in, 3 * in.int_bytes() + 2 * in.size_bytes());
+  for (size_t l = 0; l < links; ++l) {
+    Periodic per;
+    per.dim = dimension(in);
+    per.tag = in.i32();
+    per.master_tag = in.i32();
+    per.affine.resize(bounded_count(in, in.real_bytes()));
+    in.reals(per.affine.data(), per.affine.size());
+    per.nodes.resize(bounded_count(in, 2 * in.size_bytes()));
+    in.tags(2 * per.nodes.size(), "periodic node",
+            [&](size_t i, int64_t t) { per.nodes[i / 2][i % 2] = t; });
+    f.periodic.push_back(std::move(per));
+  }
+}
+
+// The header of $Nodes and $Elements (blocks, total, smallest and largest tag), and the check
+// that the blocks hold the announced total.
+struct Totals {
+  size_t blocks = 0;
+  uint64_t announced = 0, found = 0;
+  size_t at = 0;
+};
+
+template <class Num>
+Totals section_totals(Num& in) {
+  Totals t;
+  t.at = in.next_offset();
+  t.blocks = bounded_count(in, 3 * in.int_bytes() + in.size_bytes());
+  t.announced = in.word();
+  (void)in.word();  // smallest tag
+  (void)in.word();  // largest tag
+  return t;
+}
+
+void check_totals(const Totals& t, const char* what) {
+  if (t.found != t.announced)
+    parse_error(t.at, "the header announces " + std::to_string(t.announced) + " " + what +
+                          " but the blocks hold " + std::to_string(t.found));
+}
+
+template <class Num>
+void read_nodes(Num& in, File& f) {
+  Totals tot = section_totals(in);
+  for (size_t blk = 0; blk < tot.blocks; ++blk) {
+    NodeBlock nb;
+    nb.dim = dimension(in);
+    nb.entity_tag = in.i32();
+    const size_t flag_at = in.next_offset();
+    const int32_t parametric = in.i32();
+    if (parametric != 0 && parametric != 1)
+      parse_error(flag_at, "parametric flag " + std::to_string(parametric) + " (0 or 1 expected)");
+    // Each node: x, y, z and, parametric, its dim parametric coordinates (dropped here).
+    const size_t per_node = 3 + (parametric ? static_cast<size_t>(nb.dim) : 0);
+    const size_t n = bounded_count(in, in.size_bytes() + per_node * in.real_bytes());
+    nb.tags.resize(n);
+    in.tags(n, "node", [&](size_t i, int64_t t) { nb.tags[i] = t; });
+    nb.coords.resize(3 * n);
+    if (per_node == 3) {
+      in.reals(nb.coords.data(), 3 * n);
+    } else {
+      double row[6];
+      for (size_t i = 0; i < n; ++i) {
+        in.reals(row, per_node);
+        std::memcpy(&nb.coords[3 * i], row, 3 * sizeof(double));
+      }
+    }
+    tot.found += n;
+    f.node_blocks.push_back(std::move(nb));
+  }
+  check_totals(tot, "nodes");
+}
+
+template <class Num>
+void read_elements(Num& in, File& f) {
+  Totals tot = section_totals(in);
+  for (size_t blk = 0; blk < tot.blocks; ++blk) {
+    ElementBlock eb;
+    eb.dim = dimension(in);
+    eb.entity_tag = in.i32();
+    const size_t type_at = in.next_offset();
+    eb.element_type = in.i32();
+    const ElementType type = element_type(eb.element_type);  // UnsupportedElementType
+    if (tdim(type.cell) != eb.dim)
+      parse_error(type_at, "element type " + std::to_string(eb.element_type) + " has dimension " +
+                               std::to_string(tdim(type.cell)) + ", its block " +
+                               std::to_string(eb.dim));
+    const size_t k = type.num_nodes;
+    const size_t n = bounded_count(in, (k + 1) * in.size_bytes());
+    eb.tags.resize(n);
+    eb.nodes.resize(n * k);
+    // Each row: the element's tag, then its k node tags.
+    in.tags(n * (k + 1), "element or node", [&](size_t i, int64_t t) {
+      const size_t row = i / (k + 1), col = i % (k + 1);
+      if (col == 0)
+        eb.tags[row] = t;
+      else
+        eb.nodes[row * k + col - 1] = t;
+    });
+    tot.found += n;
+    f.element_blocks.push_back(std::move(eb));
+  }
+  check_totals(tot, "elements");
+}
+
+// $PhysicalNames is text in binary files too: dim, tag and a quoted name per line.
+void read_physical_names(Bytes& b, File& f) {
+  TextNumbers text(b);
+  const size_t n = bounded_count(text, 3 * text.int_bytes() + 2);
   for (size_t i = 0; i < n; ++i) {
-    Periodic p;
-    p.dim = c.dim();
-    p.tag = c.int_();
-    p.master_tag = c.int_();
-    const size_t na = c.count(c.w_float());
-    for (size_t k = 0; k < na; ++k) p.affine.push_back(c.float_());
-    const size_t nn = c.count(2 * c.w_size());
-    for (size_t k = 0; k < nn; ++k) {
-      const int64_t a = c.tag("node");
-      const int64_t b = c.tag("node");
-      p.nodes.push_back({a, b});
-    }
-    f.periodic.push_back(std::move(p));
+    PhysicalName pn;
+    pn.dim = dimension(text);
+    pn.tag = text.i32();
+    b.skip_blanks();
+    const size_t open = b.at;
+    if (b.left() == 0 || b.data[b.at] != '"') parse_error(open, "a physical name must be quoted");
+    const unsigned char* first = b.data + open + 1;
+    const void* close = std::memchr(first, '"', b.size - open - 1);
+    if (!close) parse_error(open, "the quoted physical name is not closed");
+    const size_t len = static_cast<const unsigned char*>(close) - first;
+    pn.name.assign(reinterpret_cast<const char*>(first), len);
+    if (pn.name.find_first_of("\r\n") != std::string::npos)
+      parse_error(open, "a physical name spans lines");
+    b.at = open + len + 2;
+    f.names.push_back(std::move(pn));
   }
-  c.end("Periodic");
 }
 
-void nodes(Cursor& c, File& f) {
-  const size_t at = c.pos();
-  const size_t num_blocks = c.count(3 * c.w_int() + c.w_size());
-  const uint64_t num_nodes = c.size();
-  (void)c.size();  // minNodeTag
-  (void)c.size();  // maxNodeTag
-  uint64_t total = 0;
-  for (size_t b = 0; b < num_blocks; ++b) {
-    NodeBlock blk;
-    blk.dim = c.dim();
-    blk.entity_tag = c.int_();
-    const size_t at_param = c.pos();
-    const int32_t param = c.int_();
-    if (param != 0 && param != 1)
-      c.error_at(at_param, "parametric flag " + std::to_string(param) + " is not 0 or 1");
-    const int parametric = param == 1 ? blk.dim : 0;
-    const size_t n = c.count(c.w_size() + 3 * c.w_float());
-    blk.tags.resize(n);
-    c.tags_into(blk.tags.data(), n, "node");
-    blk.coords.resize(3 * n);
-    if (parametric == 0) {
-      c.floats_into(blk.coords.data(), 3 * n);
-    } else {
-      // Parametric coordinates follow each node's x, y, z; they are read and dropped.
-      std::vector<double> row(3 + parametric);
-      for (size_t i = 0; i < n; ++i) {
-        c.floats_into(row.data(), row.size());
-        for (int g = 0; g < 3; ++g) blk.coords[3 * i + g] = row[g];
-      }
-    }
-    total += n;
-    f.node_blocks.push_back(std::move(blk));
-  }
-  if (total != num_nodes)
-    c.error_at(at, std::to_string(num_nodes) + " nodes announced, " + std::to_string(total) +
-                       " found");
-  c.end("Nodes");
+// The closing line of a section.
+void expect_end(Bytes& b, std::string_view name) {
+  size_t start;
+  const std::string_view w = b.word(&start);
+  if (w.size() != name.size() + 4 || w.compare(0, 4, "$End") != 0 || w.substr(4) != name)
+    parse_error(start, quoted(w) + " where " + quoted("$End" + std::string(name)) +
+                           " closes the section");
 }
 
-void elements(Cursor& c, File& f) {
-  const size_t at = c.pos();
-  const size_t num_blocks = c.count(3 * c.w_int() + c.w_size());
-  const uint64_t num_elements = c.size();
-  (void)c.size();
-  (void)c.size();
-  uint64_t total = 0;
-  for (size_t b = 0; b < num_blocks; ++b) {
-    ElementBlock blk;
-    blk.dim = c.dim();
-    blk.entity_tag = c.int_();
-    const size_t at_type = c.pos();
-    blk.element_type = c.int_();
-    const ElementType ty = element_type(blk.element_type);
-    if (tdim(ty.cell) != blk.dim)
-      c.error_at(at_type, "element type " + std::to_string(blk.element_type) +
-                              " in a block of dimension " + std::to_string(blk.dim));
-    const size_t k = ty.num_nodes;
-    const size_t n = c.count((1 + k) * c.w_size());
-    blk.tags.resize(n);
-    blk.nodes.resize(n * k);
-    if (c.binary()) {
-      c.tags_split(blk.nodes.data(), blk.tags.data(), n, k, "element or node");
-    } else {
-      for (size_t i = 0; i < n; ++i) {
-        blk.tags[i] = c.tag("element");
-        for (size_t j = 0; j < k; ++j) blk.nodes[i * k + j] = c.tag("node");
-      }
-    }
-    total += n;
-    f.element_blocks.push_back(std::move(blk));
+// The data format of $MeshFormat: version, file type and size_t width, and for binary files the
+// int 1 that shows the byte order.
+struct Format {
+  bool binary = false, big_endian = false;
+  size_t size_width = 8;
+};
+
+Format read_format(Bytes& b) {
+  size_t start;
+  if (b.word(&start) != "$MeshFormat") parse_error(start, "a Gmsh file starts with $MeshFormat");
+  const std::string_view version = b.word(&start);
+  if (version != "4.1")
+    parse_error(start, "version " + quoted(version) + " (this reader reads MSH 4.1)");
+  TextNumbers text(b);
+  Format fmt;
+  const size_t type_at = text.next_offset();
+  const int32_t type = text.i32();
+  const int32_t width = text.i32();
+  if (type != 0 && type != 1)
+    parse_error(type_at, "file type " + std::to_string(type) + " (0 text, 1 binary)");
+  if (width != 4 && width != 8)
+    parse_error(type_at, "size_t width " + std::to_string(width) + " (4 or 8)");
+  fmt.size_width = static_cast<size_t>(width);
+  if (type == 1) {
+    fmt.binary = true;
+    BinaryNumbers(b, false, fmt.size_width).section_body();
+    const size_t one_at = b.at;
+    const unsigned char* one = b.consume(4);
+    const unsigned char little[4] = {1, 0, 0, 0}, big[4] = {0, 0, 0, 1};
+    if (std::memcmp(one, big, 4) == 0)
+      fmt.big_endian = true;
+    else if (std::memcmp(one, little, 4) != 0)
+      parse_error(one_at, "the binary int 1 reads as neither byte order");
   }
-  if (total != num_elements)
-    c.error_at(at, std::to_string(num_elements) + " elements announced, " +
-                       std::to_string(total) + " found");
-  c.end("Elements");
+  expect_end(b, "MeshFormat");
+  return fmt;
+}
+
+// The numeric sections by name, for a decoder type.
+template <class Num>
+struct SectionTable {
+  using Decode = void (*)(Num&, File&);
+  struct Entry {
+    std::string_view name;
+    Decode decode;
+  };
+  static constexpr Entry kEntries[] = {
+      {"Entities", read_entities<Num>},
+      {"PartitionedEntities", read_partitioned_entities<Num>},
+      {"Periodic", read_periodic<Num>},
+      {"Nodes", read_nodes<Num>},
+      {"Elements", read_elements<Num>},
+  };
+  static Decode find(std::string_view name) {
+    for (const Entry& e : kEntries)
+      if (e.name == name) return e.decode;
+    return nullptr;
+  }
+};
+
+// Every section after $MeshFormat, with the file's number decoder.
+template <class Num>
+void read_sections(Bytes& b, Num& in, File& f) {
+  for (b.skip_blanks(); b.left() > 0; b.skip_blanks()) {
+    size_t start;
+    const std::string_view head = b.word(&start);
+    if (head[0] != '$') parse_error(start, quoted(head) + " where a section starts");
+    const std::string_view name = head.substr(1);
+    if (name == "MeshFormat") parse_error(start, "$MeshFormat appears twice");
+    if (name.compare(0, 3, "End") == 0)
+      parse_error(start, quoted(head) + " closes a section that was not opened");
+    if (name == "PhysicalNames") {
+      read_physical_names(b, f);
+    } else if (auto decode = SectionTable<Num>::find(name)) {
+      in.section_body();
+      decode(in, f);
+    } else {
+      // Not a section of the mesh: skip to its $End line.
+      const std::string close = "$End" + std::string(name);
+      const void* hit = b.left() >= close.size()
+                            ? memmem(b.data + b.at, b.left(), close.data(), close.size())
+                            : nullptr;
+      if (!hit) parse_error(start, "section " + quoted(head) + " has no " + quoted(close));
+      b.at = static_cast<const unsigned char*>(hit) - b.data;
+    }
+    expect_end(b, name);
+  }
 }
 
 }  // namespace
 
 File parse(const unsigned char* bytes, size_t len) {
-  Cursor c(bytes, len);
-  size_t at;
-  std::string_view first = c.token(&at);
-  if (first != "$MeshFormat") c.error_at(at, "not a Gmsh file: no $MeshFormat");
-  mesh_format(c);
+  Bytes b{bytes, len};
+  const Format fmt = read_format(b);
   File f;
-  f.binary = c.binary();
-  while (!c.at_end()) {
-    std::string_view header = c.token(&at);
-    if (header.empty() || header[0] != '$')
-      c.error_at(at, "expected a section, found \"" + std::string(header) + "\"");
-    const std::string section(header.substr(1));
-    c.header_end();
-    if (section == "PhysicalNames") {
-      std::vector<PhysicalName> n = physical_names(c);
-      f.names.insert(f.names.end(), n.begin(), n.end());
-    } else if (section == "Entities") {
-      entities(c, f);
-    } else if (section == "PartitionedEntities") {
-      partitioned_entities(c, f);
-    } else if (section == "Periodic") {
-      periodic(c, f);
-    } else if (section == "Nodes") {
-      nodes(c, f);
-    } else if (section == "Elements") {
-      elements(c, f);
-    } else if (section == "MeshFormat") {
-      c.error_at(at, "a second $MeshFormat");
-    } else if (section.compare(0, 3, "End") == 0) {
-      c.error_at(at, std::string(header) + " without its section");
-    } else {
-      c.skip_section(section);
-    }
+  f.binary = fmt.binary;
+  if (fmt.binary) {
+    BinaryNumbers in(b, fmt.big_endian, fmt.size_width);
+    read_sections(b, in, f);
+  } else {
+    TextNumbers in(b);
+    read_sections(b, in, f);
   }
-  (void)kUnsupportedElementType;
   return f;
 }
 

@@ -127,7 +127,7 @@ def _from_handle(h) -> GmshMesh:
     for i in range(info.num_names):
         dim, tag, name = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_char_p()
         lib.tg_gmsh_get_name(h, i, ctypes.byref(dim), ctypes.byref(tag), ctypes.byref(name))
-        mesh.names.append((dim.value, tag.value, name.value.decode()))
+        mesh.names.append((dim.value, tag.value, name.value.decode(errors="replace")))
     for i in range(info.num_element_blocks):
         d = _lib.GmshElementBlock()
         lib.tg_gmsh_get_element_block(h, i, ctypes.byref(d))

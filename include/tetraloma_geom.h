@@ -123,7 +123,9 @@ enum {
    * block's element has a kernel of that form (otherwise the plan stays per block). */
   TG_OPT_PLAN_PERSISTENT = 1u << 11,
   /* FP32 sum-factorised hexahedra: keep the second and third sums on the vector ALU instead of
-   * the matrix cores (v_mfma_f32_16x16x4f32), which run them by default. */
+   * the matrix cores (v_mfma_f32_16x16x4f32), which run them by default where they measure
+   * faster (degree 2 at 3^3 points, degree 3 at 4^3 points) and, under TG_OPT_SUMFACT_ALWAYS,
+   * wherever they apply (2-4 points per direction). */
   TG_OPT_NO_MFMA = 1u << 12
 };
 
