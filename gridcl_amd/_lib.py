@@ -33,6 +33,7 @@ SIGNATURES = {
     "tg_geometry_num_blocks": (c_int32, [c_void_p]),
     "tg_geometry_block_cells": (c_uint64, [c_void_p, c_int32]),
     "tg_geometry_set_fixup_threshold": (c_int32, [c_void_p, c_double]),
+    "tg_geometry_set_nodes_f64": (c_int32, [c_void_p, c_void_p]),
     "tg_pointset_create": (
         c_int32,
         [c_int32, POINTER(c_double), POINTER(c_double), c_uint64, POINTER(c_double),

@@ -261,6 +261,18 @@ class Geometry:
     def set_fixup_threshold(self, tau: float) -> None:
         _check(_lib.load().tg_geometry_set_fixup_threshold(self._h, float(tau)))
 
+    def set_nodes_f64(self, nodes_f64: torch.Tensor | None) -> None:
+        """FP32 geometries: the node coordinates also in FP64 (device [num_nodes][gdim], relative
+        to the same origin), which the evaluations gather so that each cell's offsets are taken
+        in FP64 and rounded once (tg_geometry_set_nodes_f64); None detaches."""
+        if nodes_f64 is not None:
+            if nodes_f64.dtype != torch.float64 or tuple(nodes_f64.shape) != tuple(self.nodes.shape):
+                raise LengthMismatch("FP64 nodes must be float64 of the geometry's node shape")
+            nodes_f64 = nodes_f64.contiguous()
+        self.nodes_f64 = nodes_f64
+        ptr = _dptr(nodes_f64) if nodes_f64 is not None and nodes_f64.numel() else None
+        _check(_lib.load().tg_geometry_set_nodes_f64(self._h, ptr))
+
     def evector(self, b: int) -> torch.Tensor:
         """The E-vector of block b, [k][gdim][cells padded to 16] (evector.rs:197-236)."""
         lib = _lib.load()
@@ -731,9 +743,10 @@ class _FunctionalFn(torch.autograd.Function):
         return g.to(nodes.dtype), None
 
 
-def geometry_from_host(mesh, dtype=torch.float32, device="cuda", origin=None):
+def geometry_from_host(mesh, dtype=torch.float32, device="cuda", origin=None, nodes_f64=False):
     """Uploads a HostMesh: nodes relative to the centre of their bounding box (the reference's
-    origin, geometry.rs:14-19), narrowed to `dtype`; dofmaps and flags as u32 bits."""
+    origin, geometry.rs:14-19), narrowed to `dtype`; dofmaps and flags as u32 bits. nodes_f64
+    (FP32 only): also upload the coordinates in FP64 for the evaluations (set_nodes_f64)."""
     x = np.asarray(mesh.nodes, dtype=np.float64)
     if origin is None:
         origin = 0.5 * (x.min(axis=0) + x.max(axis=0)) if len(x) else np.zeros(mesh.gdim)
@@ -746,7 +759,10 @@ def geometry_from_host(mesh, dtype=torch.float32, device="cuda", origin=None):
         dm = _u32_device(np.asarray(b.cells).reshape(-1), device)
         fl = None if b.flags is None else _u32_device(np.asarray(b.flags), device)
         blocks.append(GeometryBlock(e, dm, fl))
-    return Geometry(blocks, nodes, [float(o) for o in origin])
+    g = Geometry(blocks, nodes, [float(o) for o in origin])
+    if nodes_f64:
+        g.set_nodes_f64(torch.from_numpy(rel).to(device))
+    return g
 
 
 def stored_nodes_f64(g: Geometry) -> np.ndarray:

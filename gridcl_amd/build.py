@@ -20,6 +20,8 @@ ARCH = "gfx950"
 HIP_SOURCES = [
     ("factor_kernels.hip", ["-DTG_FACTOR_TYPE=float", "-DTG_FACTOR_COMMON"]),
     ("factor_kernels.hip", ["-DTG_FACTOR_TYPE=double"]),
+    # FP32 arithmetic on FP64-stored nodes
+    ("factor_kernels.hip", ["-DTG_FACTOR_TYPE=float", "-DTG_FACTOR_NODES64=1"]),
     ("sumfact_kernels.hip", []),
     # The matrix-core kernel's products go straight to VGPRs (not AGPRs): its J is read by the
     # vector ALU at once, and the accumulation registers would otherwise count twice (4 waves

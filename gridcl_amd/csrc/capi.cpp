@@ -42,6 +42,7 @@ DeviceTables::~DeviceTables() {
   if (tab64) (void)hipFree(tab64);
   if (ref_normals) (void)hipFree(ref_normals);
   if (sf_tab) (void)hipFree(sf_tab);
+  if (sf_tab64) (void)hipFree(sf_tab64);
   if (sf_lex) (void)hipFree(sf_lex);
   if (sf_mfma) (void)hipFree(sf_mfma);
 }
@@ -195,6 +196,8 @@ int device_tables(tg_pointset* p, unsigned degree, int family, int dtype, Device
   if (sumfact_tables(p, degree, family, tab1d, lex, &n1, &m)) {
     rc = dtype == TG_F32 ? upload<float>(tab1d, &t->sf_tab) : upload<double>(tab1d, &t->sf_tab);
     if (rc) return rc;
+    rc = upload<double>(tab1d, reinterpret_cast<void**>(&t->sf_tab64));
+    if (rc) return rc;
     // lex, then its inverse (the A.5 index at each lexicographic position).
     const size_t k = lex.size();
     std::vector<uint32_t> both(2 * k);
@@ -339,9 +342,18 @@ int enqueue_block(const tg_geometry* g, const Block& b, tg_pointset* pts, Device
   tg::FactorOutputs o{out->points, out->jacobians, out->detj, out->pinv, out->facet_normals,
                       out->normals};
   const bool shift = f32 && !(options & TG_OPT_NO_CELL_SHIFT);
+  // FP32 arithmetic on the FP64 copy of the nodes, where the geometry has one.
+  const double* n64 = shift ? g->nodes64 : nullptr;
   if (takes_sumfact(g, tabs, factors, tile, options, f32)) {
     if constexpr (sizeof(T) == 4) {
-      if (tabs->sf_mfma && !(options & TG_OPT_NO_MFMA) &&
+      if (n64) {
+        hipError_t e = tg::launch_sumfact_f64_nodes(
+            tabs->sf_n1, tabs->sf_m, n64, b.dofmap, tabs->sf_tab64,
+            tabs->sf_lex, tabs->tab64, p, o, stream);
+        if (e == hipSuccess) return TG_OK;
+        if (e != hipErrorNotSupported) return hip_fail(e, "sum-factorised kernel launch");
+      }
+      if (!n64 && tabs->sf_mfma && !(options & TG_OPT_NO_MFMA) &&
           ((options & TG_OPT_SUMFACT_ALWAYS) || mfma_pays(tabs->sf_n1, tabs->sf_m))) {
         const size_t k = static_cast<size_t>(b.k);
         hipError_t e = tg::launch_sumfact_mfma(
@@ -352,11 +364,25 @@ int enqueue_block(const tg_geometry* g, const Block& b, tg_pointset* pts, Device
         if (e != hipErrorNotSupported) return hip_fail(e, "sum-factorised matrix-core launch");
       }
     }
-    hipError_t e = tg::launch_sumfact<T>(
-        tabs->sf_n1, tabs->sf_m, shift, static_cast<const T*>(g->nodes), b.dofmap,
-        static_cast<const T*>(tabs->sf_tab), tabs->sf_lex, tabs->tab64, p, o, stream);
-    if (e == hipSuccess) return TG_OK;
-    if (e != hipErrorNotSupported) return hip_fail(e, "sum-factorised kernel launch");
+    if (!n64) {
+      hipError_t e = tg::launch_sumfact<T>(
+          tabs->sf_n1, tabs->sf_m, shift, static_cast<const T*>(g->nodes), b.dofmap,
+          static_cast<const T*>(tabs->sf_tab), tabs->sf_lex, tabs->tab64, p, o, stream);
+      if (e == hipSuccess) return TG_OK;
+      if (e != hipErrorNotSupported) return hip_fail(e, "sum-factorised kernel launch");
+    }
+  }
+  if constexpr (sizeof(T) == 4) {
+    if (n64) {
+      hipError_t e = tg::launch_factor_f64_nodes(
+          b.tdim, g->gdim, (options & TG_OPT_FORCE_GENERIC) != 0, n64, b.dofmap, b.flags,
+          static_cast<const float*>(tabs->tab), tabs->tab64,
+          static_cast<const float*>(tabs->ref_normals), p, o, stream);
+      if (e == hipErrorNotSupported)
+        return fail(TG_ERR_UNSUPPORTED_DEGREE, "no kernel for this (tdim, gdim)");
+      if (e != hipSuccess) return hip_fail(e, "factor kernel launch");
+      return TG_OK;
+    }
   }
   hipError_t e = tg::launch_factor<T>(
       b.tdim, g->gdim, shift, (options & TG_OPT_FORCE_GENERIC) != 0,
@@ -372,6 +398,8 @@ template <typename T>
 bool fused_cell_measures(const tg_geometry* g, const Block& b, const DeviceTables* tabs,
                          uint32_t options) {
   const bool f32 = sizeof(T) == 4;
+  // With FP64 nodes the measure takes det J from the evaluation kernels (the field path).
+  if (f32 && g->nodes64 && !(options & TG_OPT_NO_CELL_SHIFT)) return false;
   if ((options & TG_OPT_FORCE_GENERIC) || takes_sumfact(g, tabs, TG_DETJ, 16, options, f32))
     return false;
   return tg::has_measure_cells<T>(b.tdim, g->gdim, b.k, f32 && !(options & TG_OPT_NO_CELL_SHIFT));
@@ -514,6 +542,15 @@ uint64_t tg_geometry_block_cells(const tg_geometry* g, int32_t block) {
 int tg_geometry_set_fixup_threshold(tg_geometry* g, double tau) {
   if (!g || !(tau >= 0.0)) return fail(TG_ERR_INVALID_ARGUMENT, "bad geometry or threshold");
   g->tau = tau;
+  return TG_OK;
+}
+
+int tg_geometry_set_nodes_f64(tg_geometry* g, const double* nodes_f64) {
+  if (!g) return fail(TG_ERR_INVALID_ARGUMENT, "null geometry");
+  if (g->dtype != TG_F32 && nodes_f64)
+    return fail(TG_ERR_UNSUPPORTED_PRECISION,
+                "UnsupportedPrecision: FP64 nodes only widen an FP32 geometry");
+  g->nodes64 = nodes_f64;
   return TG_OK;
 }
 

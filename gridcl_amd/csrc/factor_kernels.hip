@@ -100,9 +100,10 @@ __device__ __forceinline__ void store_from_j(const T* j, T detj, const T* __rest
   }
 }
 
-// FP64 det J of cell c at point q from the stored (FP32) coordinates, for near-degenerate points.
-template <typename T, int TD, int GD>
-__device__ __noinline__ double detj_fp64(const T* __restrict__ nodes,
+// FP64 det J of cell c at point q from the stored coordinates (of type TN), for near-degenerate
+// points.
+template <int TD, int GD, typename TN>
+__device__ __noinline__ double detj_fp64(const TN* __restrict__ nodes,
                                          const uint32_t* __restrict__ row, uint32_t k,
                                          uint32_t last_node, const double* __restrict__ tab64,
                                          uint32_t q) {
@@ -145,23 +146,44 @@ struct Vec16<double> {
 };
 
 // Gathers the K nodes of cell c (indices clamped to the last node); with SHIFT, nodes 1..K-1
-// become offsets from node 0.
-template <typename T, int GD, int K, bool SHIFT>
-__device__ __forceinline__ void gather_cell(const T* __restrict__ nodes,
+// become offsets from node 0. Nodes stored in FP64 for FP32 arithmetic (TN = double, T = float,
+// SHIFT): the offsets are taken in FP64 and rounded once, node 0 is rounded to FP32.
+template <typename T, int GD, int K, bool SHIFT, typename TN = T>
+__device__ __forceinline__ void gather_cell(const TN* __restrict__ nodes,
                                             const uint32_t* __restrict__ row, uint32_t last_node,
                                             T (&x)[K][GD]) {
+  if constexpr (sizeof(TN) != sizeof(T)) {
+    static_assert(SHIFT && sizeof(TN) > sizeof(T), "wider node storage needs the cell shift");
+    TN x0[GD];
+    const uint32_t n0 = row[0] < last_node ? row[0] : last_node;
 #pragma unroll
-  for (int k = 0; k < K; ++k) {
-    uint32_t n = row[k];
-    n = n < last_node ? n : last_node;
+    for (int g = 0; g < GD; ++g) {
+      x0[g] = nodes[static_cast<uint64_t>(n0) * GD + g];
+      x[0][g] = static_cast<T>(x0[g]);
+    }
 #pragma unroll
-    for (int g = 0; g < GD; ++g) x[k][g] = nodes[static_cast<uint64_t>(n) * GD + g];
-  }
-  if constexpr (SHIFT) {
+    for (int k = 1; k < K; ++k) {
+      uint32_t n = row[k];
+      n = n < last_node ? n : last_node;
 #pragma unroll
-    for (int k = 1; k < K; ++k)
+      for (int g = 0; g < GD; ++g)
+        x[k][g] = static_cast<T>(nodes[static_cast<uint64_t>(n) * GD + g] - x0[g]);
+    }
+    return;
+  } else {
 #pragma unroll
-      for (int g = 0; g < GD; ++g) x[k][g] -= x[0][g];
+    for (int k = 0; k < K; ++k) {
+      uint32_t n = row[k];
+      n = n < last_node ? n : last_node;
+#pragma unroll
+      for (int g = 0; g < GD; ++g) x[k][g] = nodes[static_cast<uint64_t>(n) * GD + g];
+    }
+    if constexpr (SHIFT) {
+#pragma unroll
+      for (int k = 1; k < K; ++k)
+#pragma unroll
+        for (int g = 0; g < GD; ++g) x[k][g] -= x[0][g];
+    }
   }
 }
 
@@ -240,9 +262,9 @@ __device__ __forceinline__ void contract(const T (&x)[K][GD], const T* __restric
 }
 
 // det J, recomputed in FP64 from the stored coordinates where |det J| < tau |J|_F^3 (FP32).
-template <typename T, int TD, int GD, int K>
+template <typename T, int TD, int GD, int K, typename TN>
 __device__ __forceinline__ T detj_checked(const T* j, const FactorParams<T>& p,
-                                          const T* __restrict__ nodes,
+                                          const TN* __restrict__ nodes,
                                           const uint32_t* __restrict__ row,
                                           const double* __restrict__ tab64, uint32_t q) {
   T detj = det_of<T, TD, GD>(j);
@@ -252,7 +274,7 @@ __device__ __forceinline__ T detj_checked(const T* j, const FactorParams<T>& p,
 #pragma unroll
       for (int i = 0; i < TD * GD; ++i) n2 = tfma(j[i], j[i], n2);
       if (__builtin_fabsf(detj) < static_cast<T>(p.tau) * n2 * tsqrt(n2))
-        detj = static_cast<T>(detj_fp64<T, TD, GD>(nodes, row, K, p.last_node, tab64, q));
+        detj = static_cast<T>(detj_fp64<TD, GD>(nodes, row, K, p.last_node, tab64, q));
     }
   }
   return detj;
@@ -260,8 +282,8 @@ __device__ __forceinline__ T detj_checked(const T* j, const FactorParams<T>& p,
 
 // All requested factors of the cell of lane cl (cl < padded cells; lanes past the last cell
 // compute the last cell, the padding of the field), stored straight from registers. Any tile.
-template <typename T, int TD, int GD, int K, bool SHIFT>
-__device__ __forceinline__ void direct_cell(const T* __restrict__ nodes,
+template <typename T, int TD, int GD, int K, bool SHIFT, typename TN = T>
+__device__ __forceinline__ void direct_cell(const TN* __restrict__ nodes,
                                             const uint32_t* __restrict__ dofmap,
                                             const uint32_t* __restrict__ flags,
                                             const T* __restrict__ tab,
@@ -273,7 +295,7 @@ __device__ __forceinline__ void direct_cell(const T* __restrict__ nodes,
   const uint64_t c = cl < p.num_cells ? cl : p.num_cells - 1;
   const uint32_t* __restrict__ row = dofmap + c * K;
   T x[K][GD];
-  gather_cell<T, GD, K, SHIFT>(nodes, row, p.last_node, x);
+  gather_cell<T, GD, K, SHIFT, TN>(nodes, row, p.last_node, x);
   const uint32_t tl = p.tile_log2;
   const uint64_t tile = cl >> tl;
   const uint32_t lane = static_cast<uint32_t>(cl) & ((1u << tl) - 1u);
@@ -318,8 +340,8 @@ __device__ __forceinline__ void copy_point(const T* __restrict__ stage, void* ou
 // Lane per cell over tiles of 16, whole wavefronts (four tiles): X, J and det J of each point
 // go through LDS and leave as 16-byte stores, one point at a time. Lanes at or past the padded
 // cell count compute the last padded lane's cell and write nothing of their own.
-template <typename T, int TD, int GD, int K, bool SHIFT>
-__device__ __forceinline__ void direct_cell_staged(const T* __restrict__ nodes,
+template <typename T, int TD, int GD, int K, bool SHIFT, typename TN = T>
+__device__ __forceinline__ void direct_cell_staged(const TN* __restrict__ nodes,
                                                    const uint32_t* __restrict__ dofmap,
                                                    const uint32_t* __restrict__ flags,
                                                    const T* __restrict__ tab,
@@ -334,7 +356,7 @@ __device__ __forceinline__ void direct_cell_staged(const T* __restrict__ nodes,
   const uint64_t c = cl < p.num_cells ? cl : p.num_cells - 1;
   const uint32_t* __restrict__ row = dofmap + c * K;
   T x[K][GD];
-  gather_cell<T, GD, K, SHIFT>(nodes, row, p.last_node, x);
+  gather_cell<T, GD, K, SHIFT, TN>(nodes, row, p.last_node, x);
   const uint32_t w = threadIdx.x & 63u, wt = w >> 4, wl = w & 15u;
   const uint64_t tile0 = (cl_in - w) >> 4, tile = cl >> 4;
   const uint64_t ntiles = p.padded_cells >> 4;
@@ -389,8 +411,8 @@ __device__ __forceinline__ void copy_rows(const T* __restrict__ stage, void* out
 // tile are contiguous in every field. X, J and det J go through LDS and leave as sequential
 // 16-byte non-temporal stores, so each wavefront streams through its tile's output in order;
 // other factors are stored from registers.
-template <typename T, int TD, int GD, int K, bool SHIFT>
-__device__ __forceinline__ void direct_tile(const T* __restrict__ nodes,
+template <typename T, int TD, int GD, int K, bool SHIFT, typename TN = T>
+__device__ __forceinline__ void direct_tile(const TN* __restrict__ nodes,
                                             const uint32_t* __restrict__ dofmap,
                                             const uint32_t* __restrict__ flags,
                                             const T* __restrict__ tab,
@@ -404,7 +426,7 @@ __device__ __forceinline__ void direct_tile(const T* __restrict__ nodes,
   const uint64_t c = cl < p.num_cells ? cl : p.num_cells - 1;
   const uint32_t* __restrict__ row = dofmap + c * K;
   T x[K][GD];
-  gather_cell<T, GD, K, SHIFT>(nodes, row, p.last_node, x);
+  gather_cell<T, GD, K, SHIFT, TN>(nodes, row, p.last_node, x);
   const uint32_t mask = p.mask;
   const bool need_j = (mask & ~kFPoints) != 0;
   const uint32_t ex = 0;
@@ -457,9 +479,9 @@ __device__ __forceinline__ void table_to_lds(const T* __restrict__ src, T* __res
 // One block. kMapTile: one wavefront per tile of 16 cells, the basis table staged in LDS;
 // kMapCellStaged / kMapCell: one lane per cell, the table read through the scalar cache.
 // Dynamic LDS: the staging buffers (staged mappings), then the table (kMapTile).
-template <typename T, int TD, int GD, int K, bool SHIFT>
+template <typename T, int TD, int GD, int K, bool SHIFT, typename TN = T>
 __global__ void __launch_bounds__(kBlock)
-    factor_direct(const T* __restrict__ nodes, const uint32_t* __restrict__ dofmap,
+    factor_direct(const TN* __restrict__ nodes, const uint32_t* __restrict__ dofmap,
                   const uint32_t* __restrict__ flags, const T* __restrict__ tab,
                   const double* __restrict__ tab64, const T* __restrict__ ref_normals,
                   FactorParams<T> p, FactorOutputs o) {
@@ -472,17 +494,17 @@ __global__ void __launch_bounds__(kBlock)
     __syncthreads();
     const uint64_t tile = static_cast<uint64_t>(b) * (kBlock / 64) + (threadIdx.x >> 6);
     if (tile >= (p.padded_cells >> 4)) return;
-    direct_tile<T, TD, GD, K, SHIFT>(nodes, dofmap, flags, ltab, tab64, ref_normals, p, o, tile,
+    direct_tile<T, TD, GD, K, SHIFT, TN>(nodes, dofmap, flags, ltab, tab64, ref_normals, p, o, tile,
                                      stage + (threadIdx.x >> 6) * kStageElems);
   } else if (p.mapping == kMapCellStaged) {
     const uint64_t cl = static_cast<uint64_t>(b) * kBlock + threadIdx.x;
     if (cl - (threadIdx.x & 63u) >= p.padded_cells) return;  // whole wavefronts stay
-    direct_cell_staged<T, TD, GD, K, SHIFT>(nodes, dofmap, flags, tab, tab64, ref_normals, p, o,
+    direct_cell_staged<T, TD, GD, K, SHIFT, TN>(nodes, dofmap, flags, tab, tab64, ref_normals, p, o,
                                             cl, stage + (threadIdx.x >> 6) * kStageElems);
   } else {
     const uint64_t cl = static_cast<uint64_t>(b) * kBlock + threadIdx.x;
     if (cl >= p.padded_cells) return;
-    direct_cell<T, TD, GD, K, SHIFT>(nodes, dofmap, flags, tab, tab64, ref_normals, p, o, cl);
+    direct_cell<T, TD, GD, K, SHIFT, TN>(nodes, dofmap, flags, tab, tab64, ref_normals, p, o, cl);
   }
 }
 
@@ -574,9 +596,9 @@ __global__ void __launch_bounds__(kBlock, 2)
 
 // Runtime node count (the reference's `loops` strategy, layout_loops.rs:68-108): node
 // coordinates are re-read through the dofmap at every point. Any element.
-template <typename T, int TD, int GD, bool SHIFT>
+template <typename T, int TD, int GD, bool SHIFT, typename TN = T>
 __global__ void __launch_bounds__(kBlock)
-    factor_generic(const T* __restrict__ nodes, const uint32_t* __restrict__ dofmap,
+    factor_generic(const TN* __restrict__ nodes, const uint32_t* __restrict__ dofmap,
                    const uint32_t* __restrict__ flags, const T* __restrict__ tab,
                    const double* __restrict__ tab64, const T* __restrict__ ref_normals,
                    FactorParams<T> p, FactorOutputs o) {
@@ -586,7 +608,7 @@ __global__ void __launch_bounds__(kBlock)
   const uint64_t c = cl < p.num_cells ? cl : p.num_cells - 1;
   const uint32_t K = p.k;
   const uint32_t* __restrict__ row = dofmap + c * K;
-  T x0[GD];
+  TN x0[GD];  // node 0 in the storage type: FP64 storage takes the offsets in FP64
   {
     const uint32_t n = row[0] < p.last_node ? row[0] : p.last_node;
 #pragma unroll
@@ -608,8 +630,8 @@ __global__ void __launch_bounds__(kBlock)
       T xk[GD];
 #pragma unroll
       for (int g = 0; g < GD; ++g) {
-        xk[g] = nodes[static_cast<uint64_t>(n) * GD + g];
-        if (SHIFT) xk[g] -= x0[g];
+        const TN v = nodes[static_cast<uint64_t>(n) * GD + g];
+        xk[g] = static_cast<T>(SHIFT ? v - x0[g] : v);
       }
       const T* t = tq + 4 * k;
       if (k == k0) {
@@ -632,7 +654,7 @@ __global__ void __launch_bounds__(kBlock)
     if (mask & kFPoints) {
 #pragma unroll
       for (int g = 0; g < GD; ++g) {
-        const T v = SHIFT ? (x0[g] + xs[g]) + p.origin[g] : xs[g] + p.origin[g];
+        const T v = SHIFT ? (static_cast<T>(x0[g]) + xs[g]) + p.origin[g] : xs[g] + p.origin[g];
         st<T>(o.x, (((row_q * GD) + g) << tl) | lane, v);
       }
     }
@@ -646,7 +668,7 @@ __global__ void __launch_bounds__(kBlock)
 #pragma unroll
             for (int i = 0; i < TD * GD; ++i) n2 = tfma(j[i], j[i], n2);
             if (__builtin_fabsf(detj) < static_cast<T>(p.tau) * n2 * tsqrt(n2))
-              detj = static_cast<T>(detj_fp64<T, TD, GD>(nodes, row, K, p.last_node, tab64, q));
+              detj = static_cast<T>(detj_fp64<TD, GD>(nodes, row, K, p.last_node, tab64, q));
           }
         }
       }
@@ -673,8 +695,8 @@ __global__ void __launch_bounds__(kBlock)
   }
 }
 
-template <typename T>
-using FactorFn = void (*)(const T*, const uint32_t*, const uint32_t*, const T*, const double*,
+template <typename T, typename TN = T>
+using FactorFn = void (*)(const TN*, const uint32_t*, const uint32_t*, const T*, const double*,
                           const T*, FactorParams<T>, FactorOutputs);
 
 // Register-resident kernels exist for these (tdim, gdim, k): degrees 1-3 of every cell (tets,
@@ -685,17 +707,24 @@ using FactorFn = void (*)(const T*, const uint32_t*, const uint32_t*, const T*, 
   X(2, 3, 3) X(2, 3, 6) X(2, 3, 10) X(2, 3, 4) X(2, 3, 9) X(2, 3, 16) X(1, 1, 2) X(1, 1, 3)    \
   X(1, 1, 4) X(1, 2, 2) X(1, 2, 3) X(1, 2, 4) X(1, 3, 2) X(1, 3, 3) X(1, 3, 4)
 
-template <typename T>
-FactorFn<T> direct_kernel(int td, int gd, uint32_t k, bool shift) {
+template <typename T, typename TN>
+FactorFn<T, TN> direct_kernel(int td, int gd, uint32_t k, bool shift) {
   // FP64 keeps at most 27 nodes in registers; larger elements use the runtime-K kernel.
   if (sizeof(T) == 8 && k > 27) return nullptr;
-#define TG_CASE(TD_, GD_, K_)                                                   \
-  if (td == TD_ && gd == GD_ && k == K_)                                        \
-    return shift ? static_cast<FactorFn<T>>(factor_direct<T, TD_, GD_, K_, true>) \
-                 : static_cast<FactorFn<T>>(factor_direct<T, TD_, GD_, K_, false>);
   if (sizeof(T) == 4 && !shift) return nullptr;  // F32 without the shift runs generic
-  TG_DIRECT_SHAPES(TG_CASE)
+  if constexpr (sizeof(TN) != sizeof(T)) {  // wider node storage: only with the shift
+#define TG_CASE(TD_, GD_, K_) \
+  if (td == TD_ && gd == GD_ && k == K_) return factor_direct<T, TD_, GD_, K_, true, TN>;
+    TG_DIRECT_SHAPES(TG_CASE)
 #undef TG_CASE
+  } else {
+#define TG_CASE(TD_, GD_, K_)                                                       \
+  if (td == TD_ && gd == GD_ && k == K_)                                            \
+    return shift ? static_cast<FactorFn<T, TN>>(factor_direct<T, TD_, GD_, K_, true>) \
+                 : static_cast<FactorFn<T, TN>>(factor_direct<T, TD_, GD_, K_, false>);
+    TG_DIRECT_SHAPES(TG_CASE)
+#undef TG_CASE
+  }
   return nullptr;
 }
 
@@ -768,27 +797,34 @@ MeasureFn<T> measure_kernel(int td, int gd, uint32_t k, bool shift) {
   return nullptr;
 }
 
-template <typename T>
-FactorFn<T> generic_kernel(int td, int gd, bool shift) {
+template <typename T, typename TN>
+FactorFn<T, TN> generic_kernel(int td, int gd, bool shift) {
+  if constexpr (sizeof(TN) != sizeof(T)) {
+    if (!shift) return nullptr;
+#define TG_G(TD_, GD_) \
+  if (td == TD_ && gd == GD_) return factor_generic<T, TD_, GD_, true, TN>;
+    TG_G(1, 1) TG_G(1, 2) TG_G(1, 3) TG_G(2, 2) TG_G(2, 3) TG_G(3, 3)
+#undef TG_G
+  } else {
 #define TG_G(TD_, GD_)                                                              \
   if (td == TD_ && gd == GD_)                                                       \
-    return shift ? static_cast<FactorFn<T>>(factor_generic<T, TD_, GD_, true>)      \
-                 : static_cast<FactorFn<T>>(factor_generic<T, TD_, GD_, false>);
-  TG_G(1, 1) TG_G(1, 2) TG_G(1, 3) TG_G(2, 2) TG_G(2, 3) TG_G(3, 3)
+    return shift ? static_cast<FactorFn<T, TN>>(factor_generic<T, TD_, GD_, true>)  \
+                 : static_cast<FactorFn<T, TN>>(factor_generic<T, TD_, GD_, false>);
+    TG_G(1, 1) TG_G(1, 2) TG_G(1, 3) TG_G(2, 2) TG_G(2, 3) TG_G(3, 3)
 #undef TG_G
+  }
   return nullptr;
 }
 
-}  // namespace
-
-template <typename T>
-hipError_t launch_factor(int tdim, int gdim, bool shift, bool generic, const T* nodes,
-                         const uint32_t* dofmap, const uint32_t* flags, const T* tab,
-                         const double* tab64, const T* ref_normals, const FactorParams<T>& pin,
-                         const FactorOutputs& out, hipStream_t stream) {
-  FactorFn<T> fn = generic ? nullptr : direct_kernel<T>(tdim, gdim, pin.k, shift);
+template <typename T, typename TN>
+hipError_t launch_factor_impl(int tdim, int gdim, bool shift, bool generic, const TN* nodes,
+                              const uint32_t* dofmap, const uint32_t* flags, const T* tab,
+                              const double* tab64, const T* ref_normals,
+                              const FactorParams<T>& pin, const FactorOutputs& out,
+                              hipStream_t stream) {
+  FactorFn<T, TN> fn = generic ? nullptr : direct_kernel<T, TN>(tdim, gdim, pin.k, shift);
   const bool direct = fn != nullptr;
-  if (fn == nullptr) fn = generic_kernel<T>(tdim, gdim, shift);
+  if (fn == nullptr) fn = generic_kernel<T, TN>(tdim, gdim, shift);
   if (fn == nullptr) return hipErrorNotSupported;
   FactorParams<T> p = pin;
   // The staged mappings need tiles of 16 and a register-resident kernel; kMapTile a table that
@@ -808,6 +844,28 @@ hipError_t launch_factor(int tdim, int gdim, bool shift, bool generic, const T* 
                      ref_normals, p, out);
   return hipGetLastError();
 }
+
+}  // namespace
+
+#ifndef TG_FACTOR_NODES64
+template <typename T>
+hipError_t launch_factor(int tdim, int gdim, bool shift, bool generic, const T* nodes,
+                         const uint32_t* dofmap, const uint32_t* flags, const T* tab,
+                         const double* tab64, const T* ref_normals, const FactorParams<T>& pin,
+                         const FactorOutputs& out, hipStream_t stream) {
+  return launch_factor_impl<T, T>(tdim, gdim, shift, generic, nodes, dofmap, flags, tab, tab64,
+                                  ref_normals, pin, out, stream);
+}
+#else
+hipError_t launch_factor_f64_nodes(int tdim, int gdim, bool generic, const double* nodes,
+                                   const uint32_t* dofmap, const uint32_t* flags,
+                                   const float* tab, const double* tab64,
+                                   const float* ref_normals, const FactorParams<float>& p,
+                                   const FactorOutputs& out, hipStream_t stream) {
+  return launch_factor_impl<float, double>(tdim, gdim, true, generic, nodes, dofmap, flags, tab,
+                                           tab64, ref_normals, p, out, stream);
+}
+#endif
 
 template <typename T>
 bool has_measure_cells(int tdim, int gdim, uint32_t k, bool shift) {
@@ -892,6 +950,7 @@ hipError_t launch_evector(const T* nodes, const uint32_t* dofmap, uint64_t num_c
 #ifndef TG_FACTOR_TYPE
 #error "compile with -DTG_FACTOR_TYPE=float or -DTG_FACTOR_TYPE=double"
 #endif
+#ifndef TG_FACTOR_NODES64
 using FT = TG_FACTOR_TYPE;
 template hipError_t hybrid_grid<FT>(bool, size_t, uint32_t*);
 template hipError_t launch_hybrid<FT>(bool, const HybridBlock<FT>*, const WorkItem*,
@@ -905,5 +964,6 @@ template bool has_measure_cells<FT>(int, int, uint32_t, bool);
 template hipError_t launch_measure_cells<FT>(int, int, bool, const FT*, const uint32_t*, const FT*,
                                              const double*, const double*,
                                              const FactorParams<FT>&, FT*, hipStream_t);
+#endif
 
 }  // namespace tg

@@ -110,6 +110,14 @@ hipError_t launch_factor(int tdim, int gdim, bool shift, bool generic, const T* 
                          const double* tab64, const T* ref_normals, const FactorParams<T>& p,
                          const FactorOutputs& out, hipStream_t stream);
 
+// launch_factor for FP32 arithmetic on node coordinates stored in FP64 (always with the cell
+// shift): each cell's offsets from its node 0 are taken in FP64 and rounded once to FP32.
+hipError_t launch_factor_f64_nodes(int tdim, int gdim, bool generic, const double* nodes,
+                                   const uint32_t* dofmap, const uint32_t* flags,
+                                   const float* tab, const double* tab64,
+                                   const float* ref_normals, const FactorParams<float>& p,
+                                   const FactorOutputs& out, hipStream_t stream);
+
 // Sum-factorised hexahedra (tiles of 16; X, J, det J only): degree n1 - 1 in each direction
 // (n1 = 2..4 nodes per direction) at an m x m x m tensor point set (m = 1..4, first coordinate
 // fastest). tab1d is [2][m][n1] (1D basis values, then derivatives, at the 1D points); lex[k]
@@ -133,6 +141,14 @@ hipError_t launch_sumfact_mfma(int n1, int m, bool shift, const float* nodes,
                                const float* tab2d, const double* tab64,
                                const FactorParams<float>& p, const FactorOutputs& out,
                                hipStream_t stream);
+
+// Sum-factorised hexahedra of an FP32 geometry on its FP64 nodes: the sums in FP64 (the
+// arithmetic of the FP64 kernel) and X, J and det J rounded once to FP32 at their stores; tab1d
+// is the FP64 1D table.
+hipError_t launch_sumfact_f64_nodes(int n1, int m, const double* nodes, const uint32_t* dofmap,
+                                    const double* tab1d, const uint32_t* lex, const double* tab64,
+                                    const FactorParams<float>& p, const FactorOutputs& out,
+                                    hipStream_t stream);
 
 template <typename T>
 hipError_t launch_evector(const T* nodes, const uint32_t* dofmap, uint64_t num_cells,

@@ -38,6 +38,7 @@ struct DeviceTables {
   double* tab64 = nullptr;
   void* ref_normals = nullptr;
   void* sf_tab = nullptr;
+  double* sf_tab64 = nullptr;  // the same 1D table in f64 (the FP64-node path of FP32 geometries)
   uint32_t* sf_lex = nullptr;
   void* sf_mfma = nullptr;
   int sf_n1 = 0, sf_m = 0;
@@ -66,6 +67,9 @@ struct tg_geometry {
   int dtype;
   int gdim;
   const void* nodes;
+  // FP32 geometries: the same coordinates stored in FP64 (caller-owned), which the evaluation
+  // kernels gather instead of `nodes` when set (tg_geometry_set_nodes_f64).
+  const double* nodes64 = nullptr;
   uint64_t num_nodes;
   double origin[3];
   std::vector<tg::Block> blocks;

@@ -229,7 +229,9 @@ int tg_plan_create(const tg_geometry* g, tg_pointset* const* pts, uint32_t facto
   plan->tile = tile;
   plan->options = options;
   plan->shift = g->dtype == TG_F32 && !(options & TG_OPT_NO_CELL_SHIFT);
-  bool all_hybrid = !(options & TG_OPT_FORCE_GENERIC) && (g->dtype == TG_F64 || plan->shift);
+  // The persistent kernel reads the geometry's own node array: not with an FP64 copy attached.
+  bool all_hybrid = !(options & TG_OPT_FORCE_GENERIC) && (g->dtype == TG_F64 || plan->shift) &&
+                    !(plan->shift && g->nodes64);
   for (size_t bi = 0; bi < g->blocks.size(); ++bi) {
     const tg::Block& b = g->blocks[bi];
     tg_pointset* p = pts[bi];

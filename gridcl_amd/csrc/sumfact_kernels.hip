@@ -24,32 +24,32 @@
 namespace tg {
 namespace {
 
-template <typename T, int N1, int M, int PC, bool SHIFT>
+template <typename T, int N1, int M, int PC, bool SHIFT, typename TN = T, typename TO = T>
 __global__ void __launch_bounds__(sf::kThreads)
-    factor_sumfact(const T* __restrict__ nodes, const uint32_t* __restrict__ dofmap,
+    factor_sumfact(const TN* __restrict__ nodes, const uint32_t* __restrict__ dofmap,
                    const T* __restrict__ tab1d, const uint32_t* __restrict__ lex,
-                   const double* __restrict__ tab64, FactorParams<T> p, FactorOutputs o) {
+                   const double* __restrict__ tab64, FactorParams<TO> p, FactorOutputs o) {
   __shared__ __align__(16) T lds[sf::Lds<N1, M, PC>::kTotal];
   // Tiles in dispatch order: giving each XCD a contiguous range of tiles instead (as the
   // per-block dense kernels do) measured 9-14 % slower here.
-  sf::tile<T, N1, M, PC, SHIFT>(nodes, dofmap, tab1d, lex, tab64, p, o, blockIdx.x, lds);
+  sf::tile<T, N1, M, PC, SHIFT, TN, TO>(nodes, dofmap, tab1d, lex, tab64, p, o, blockIdx.x, lds);
 }
 
-template <typename T>
-using SumfactFn = void (*)(const T*, const uint32_t*, const T*, const uint32_t*, const double*,
+template <typename T, typename TN = T>
+using SumfactFn = void (*)(const TN*, const uint32_t*, const T*, const uint32_t*, const double*,
                            FactorParams<T>, FactorOutputs);
 
 // FP32 keeps its points of an r-slab in registers; FP64 takes the p direction two at a time.
-template <typename T, int N1, int M, bool SHIFT>
-constexpr SumfactFn<T> sumfact_fn() {
+template <typename T, int N1, int M, bool SHIFT, typename TN>
+constexpr SumfactFn<T, TN> sumfact_fn() {
   constexpr int PC = sizeof(T) == 4 ? M : (M < 2 ? M : 2);  // FP64: p two at a time
-  return factor_sumfact<T, N1, M, PC, SHIFT>;
+  return factor_sumfact<T, N1, M, PC, SHIFT, TN>;
 }
 
-template <typename T, bool SHIFT>
-SumfactFn<T> pick(int n1, int m) {
+template <typename T, bool SHIFT, typename TN = T>
+SumfactFn<T, TN> pick(int n1, int m) {
 #define TG_SF(N1_, M_) \
-  if (n1 == N1_ && m == M_) return sumfact_fn<T, N1_, M_, SHIFT>();
+  if (n1 == N1_ && m == M_) return sumfact_fn<T, N1_, M_, SHIFT, TN>();
   TG_SF(2, 1) TG_SF(2, 2) TG_SF(2, 3) TG_SF(2, 4)
   TG_SF(3, 1) TG_SF(3, 2) TG_SF(3, 3) TG_SF(3, 4)
   TG_SF(4, 1) TG_SF(4, 2) TG_SF(4, 3) TG_SF(4, 4)
@@ -76,6 +76,35 @@ template hipError_t launch_sumfact<float>(int, int, bool, const float*, const ui
                                           const float*, const uint32_t*, const double*,
                                           const FactorParams<float>&, const FactorOutputs&,
                                           hipStream_t);
+// FP64 nodes of an FP32 geometry: the sums in FP64 (as the FP64 kernel, unshifted, p two at a
+// time) and X, J, det J rounded to FP32 at their stores.
+using AccurateFn = void (*)(const double*, const uint32_t*, const double*, const uint32_t*,
+                            const double*, FactorParams<float>, FactorOutputs);
+
+AccurateFn pick_accurate(int n1, int m) {
+#define TG_SFA(N1_, M_)        \
+  if (n1 == N1_ && m == M_)    \
+    return factor_sumfact<double, N1_, M_, (M_ < 2 ? M_ : 2), false, double, float>;
+  TG_SFA(2, 1) TG_SFA(2, 2) TG_SFA(2, 3) TG_SFA(2, 4)
+  TG_SFA(3, 1) TG_SFA(3, 2) TG_SFA(3, 3) TG_SFA(3, 4)
+  TG_SFA(4, 1) TG_SFA(4, 2) TG_SFA(4, 3) TG_SFA(4, 4)
+#undef TG_SFA
+  return nullptr;
+}
+
+hipError_t launch_sumfact_f64_nodes(int n1, int m, const double* nodes, const uint32_t* dofmap,
+                                    const double* tab1d, const uint32_t* lex, const double* tab64,
+                                    const FactorParams<float>& p, const FactorOutputs& out,
+                                    hipStream_t stream) {
+  AccurateFn fn = pick_accurate(n1, m);
+  if (!fn || p.tile_log2 != 4) return hipErrorNotSupported;
+  const uint64_t blocks = p.padded_cells >> 4;
+  if (blocks > 0x7fffffffu) return hipErrorInvalidConfiguration;
+  hipLaunchKernelGGL(fn, dim3(static_cast<uint32_t>(blocks)), dim3(sf::kThreads), 0, stream, nodes,
+                     dofmap, tab1d, lex, tab64, p, out);
+  return hipGetLastError();
+}
+
 template hipError_t launch_sumfact<double>(int, int, bool, const double*, const uint32_t*,
                                            const double*, const uint32_t*, const double*,
                                            const FactorParams<double>&, const FactorOutputs&,

@@ -267,7 +267,7 @@ __global__ void __launch_bounds__(64 * kWaves)
 #pragma unroll
         for (int pp = 0; pp < M; ++pp)
           if (near & (1u << pp))
-            det[pp] = static_cast<float>(sf::detj_fp64<float, K>(
+            det[pp] = static_cast<float>(sf::detj_fp64<K>(
                 nodes, row, p.last_node, tab64, static_cast<uint32_t>(pp + M * q + M * M * r)));
       }
 #pragma unroll

@@ -41,10 +41,10 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// FP64 det J of cell c at point q from the stored coordinates (offsets from node 0 in FP64),
-// for near-degenerate points of FP32 geometries.
-template <typename T, int K>
-__device__ __noinline__ double detj_fp64(const T* __restrict__ nodes,
+// FP64 det J of cell c at point q from the stored coordinates (of type TN; offsets from node 0
+// in FP64), for near-degenerate points of FP32 geometries.
+template <int K, typename TN>
+__device__ __noinline__ double detj_fp64(const TN* __restrict__ nodes,
                                          const uint32_t* __restrict__ row, uint32_t last_node,
                                          const double* __restrict__ tab64, uint32_t q) {
   double x0[3], j[9];
@@ -127,10 +127,14 @@ struct Lds {
 // any FP64 recomputation) of all r.
 // `lds` holds Lds<N1, M, PC>::kTotal elements (16-byte aligned); every thread of the 256-thread
 // workgroup calls with the same tile.
-template <typename T, int N1, int M, int PC, bool SHIFT>
-__device__ __forceinline__ void tile(const T* __restrict__ nodes, const uint32_t* __restrict__ dofmap,
+// TN is the type the node coordinates are stored in: T, or double for FP32 arithmetic on FP64
+// nodes (with SHIFT; the vertex mean and the offsets are then taken in FP64 and rounded once).
+// TO is the type of the outputs and of the launch's parameters: T, or float for FP64 arithmetic
+// with FP32 outputs (each value rounded once, at its store).
+template <typename T, int N1, int M, int PC, bool SHIFT, typename TN = T, typename TO = T>
+__device__ __forceinline__ void tile(const TN* __restrict__ nodes, const uint32_t* __restrict__ dofmap,
                                      const T* __restrict__ tab1d, const uint32_t* __restrict__ lex,
-                                     const double* __restrict__ tab64, const FactorParams<T>& p,
+                                     const double* __restrict__ tab64, const FactorParams<TO>& p,
                                      const FactorOutputs& o, uint64_t tile, T* __restrict__ lds) {
   using L = Lds<N1, M, PC>;
   constexpr int K = N1 * N1 * N1;
@@ -146,12 +150,13 @@ __device__ __forceinline__ void tile(const T* __restrict__ nodes, const uint32_t
   const uint32_t t = threadIdx.x, lane = t & 63u, cell = t & 15u, r_lane = (t >> 4) & 3u,
                  w = t >> 6;
 
+  static_assert(sizeof(TN) == sizeof(T) || SHIFT, "wider node storage needs the shift");
   struct P3 {
-    T v[3];
+    TN v[3];
   };
   // Gather: wavefront w takes the cells w, w + 4, w + 8, w + 12 of the tile, lane k < K node k
   // (A.5 order: the eight vertices are lanes 0-7).
-  T held[4][3];
+  TN held[4][3];
   const bool node_lane = lane < static_cast<uint32_t>(K);
   const uint32_t lex_lane = node_lane ? lex[lane] : 0u;
 #pragma unroll
@@ -159,7 +164,7 @@ __device__ __forceinline__ void tile(const T* __restrict__ nodes, const uint32_t
     const uint32_t cc = w + 4u * i;
     uint64_t c = tile * 16 + cc;
     c = c < p.num_cells ? c : p.num_cells - 1;
-    held[i][0] = held[i][1] = held[i][2] = T(0);
+    held[i][0] = held[i][1] = held[i][2] = TN(0);
     if (node_lane) {
       uint32_t n = dofmap[c * K + lane];
       n = n < p.last_node ? n : p.last_node;
@@ -176,15 +181,15 @@ __device__ __forceinline__ void tile(const T* __restrict__ nodes, const uint32_t
     const uint32_t cc = w + 4u * i;
 #pragma unroll
     for (int g = 0; g < 3; ++g) {
-      T sg = T(0);
+      TN sg = TN(0);
       if constexpr (SHIFT) {
-        T acc = __builtin_bit_cast(T, readlane_bits(held[i][g], 0));
+        TN acc = __builtin_bit_cast(TN, readlane_bits(held[i][g], 0));
 #pragma unroll
-        for (int v = 1; v < 8; ++v) acc += __builtin_bit_cast(T, readlane_bits(held[i][g], v));
-        sg = acc * T(0.125);
-        if (lane == 0) shift[g * 16 + cc] = sg;
+        for (int v = 1; v < 8; ++v) acc += __builtin_bit_cast(TN, readlane_bits(held[i][g], v));
+        sg = acc * TN(0.125);
+        if (lane == 0) shift[g * 16 + cc] = static_cast<T>(sg);
       }
-      if (node_lane) xs[(g * 16 + cc) * (K + 1) + lex_lane] = held[i][g] - sg;
+      if (node_lane) xs[(g * 16 + cc) * (K + 1) + lex_lane] = static_cast<T>(held[i][g] - sg);
     }
   }
   __syncthreads();
@@ -225,7 +230,8 @@ __device__ __forceinline__ void tile(const T* __restrict__ nodes, const uint32_t
   // Lanes with r >= M compute r = M - 1 and stage nothing.
   const bool r_live = r_lane < static_cast<uint32_t>(M);
   const uint32_t r = r_live ? r_lane : M - 1;
-  T* __restrict__ stage = stage_base + w * SLEN;
+  // The output stage holds TO values (no wider than T) in the same LDS.
+  TO* __restrict__ stage = reinterpret_cast<TO*>(stage_base + w * SLEN);
   T sg[3];
 #pragma unroll
   for (int g = 0; g < 3; ++g) sg[g] = SHIFT ? shift[g * 16 + cell] : T(0);
@@ -314,19 +320,20 @@ __device__ __forceinline__ void tile(const T* __restrict__ nodes, const uint32_t
             if (p0 + pp >= M) continue;
 #pragma unroll
             for (int g = 0; g < 3; ++g) {
-              stage[SX + (pp * 3 + g) * 16 + cell] = (acc[pp][0][g] + sg[g]) + p.origin[g];
+              stage[SX + (pp * 3 + g) * 16 + cell] =
+                  static_cast<TO>((acc[pp][0][g] + sg[g]) + static_cast<T>(p.origin[g]));
 #pragma unroll
               for (int d = 0; d < 3; ++d)
-                stage[SJ + (pp * 9 + d * 3 + g) * 16 + cell] = acc[pp][1 + d][g];
+                stage[SJ + (pp * 9 + d * 3 + g) * 16 + cell] = static_cast<TO>(acc[pp][1 + d][g]);
             }
           }
         }
         wave_sync();
         const uint64_t pt0 = tile * NQ + static_cast<uint32_t>(p0 + M * q + M * M * rr);
         if (mask & kFPoints)
-          flush_run<T>(stage + SX, static_cast<T*>(o.x) + ((pt0 * 3) << 4), npp * 3 * 16, lane);
+          flush_run<TO>(stage + SX, static_cast<TO*>(o.x) + ((pt0 * 3) << 4), npp * 3 * 16, lane);
         if (mask & kFJac)
-          flush_run<T>(stage + SJ, static_cast<T*>(o.j) + ((pt0 * 9) << 4), npp * 9 * 16, lane);
+          flush_run<TO>(stage + SJ, static_cast<TO*>(o.j) + ((pt0 * 9) << 4), npp * 9 * 16, lane);
       }
     }
     if (mask & kFDetJ) {
@@ -335,7 +342,7 @@ __device__ __forceinline__ void tile(const T* __restrict__ nodes, const uint32_t
 #pragma unroll
           for (int pp = 0; pp < PC; ++pp)
             if (near & (1u << pp))
-              det[pp] = static_cast<T>(detj_fp64<T, K>(
+              det[pp] = static_cast<T>(detj_fp64<K>(
                   nodes, dofmap + c_own * K, p.last_node, tab64,
                   static_cast<uint32_t>(p0 + pp + M * q) + M * M * r));
         }
@@ -345,12 +352,12 @@ __device__ __forceinline__ void tile(const T* __restrict__ nodes, const uint32_t
       if (r_live)
 #pragma unroll
         for (int pp = 0; pp < PC; ++pp)
-          if (p0 + pp < M) stage[(r * PC + pp) * 16 + cell] = det[pp];
+          if (p0 + pp < M) stage[(r * PC + pp) * 16 + cell] = static_cast<TO>(det[pp]);
       wave_sync();
 #pragma unroll 1
       for (int rr = 0; rr < M; ++rr) {
         const uint64_t pt0 = tile * NQ + static_cast<uint32_t>(p0 + M * q + M * M * rr);
-        flush_run<T>(stage + rr * PC * 16, static_cast<T*>(o.detj) + (pt0 << 4), npp * 16, lane);
+        flush_run<TO>(stage + rr * PC * 16, static_cast<TO*>(o.detj) + (pt0 << 4), npp * 16, lane);
       }
     }
   }

@@ -184,6 +184,16 @@ uint64_t tg_geometry_block_cells(const tg_geometry* g, int32_t block);
 /* Threshold tau of the FP64 fix-up: points with |det J| / |J|_F^3 < tau get det J recomputed in
  * FP64 (F32 geometries). Default 1e-2. */
 int tg_geometry_set_fixup_threshold(tg_geometry* g, double tau);
+/* FP32 geometries: the node coordinates also stored in FP64, [num_nodes][gdim] relative to the
+ * same origin (device, caller-owned, kept by pointer; NULL detaches). No counterpart in the
+ * reference, whose FP32 path reads FP32 nodes (geometry.rs:106-150). With it, the evaluations of
+ * tg_eval_block, the plans and tg_measure_block gather the FP64 nodes: each cell's offsets from
+ * its shift point (node 0, or the vertex mean of the sum-factorised hexahedra) are taken in FP64
+ * and rounded once to FP32, and the contraction stays FP32, so the FP32 rounding of node
+ * coordinates far from the cell no longer enters J (it is about 1e-5 of J at h = 1/128). The
+ * FP32 node array is still used by the other operations. Unsupported for FP64 geometries and
+ * under TG_OPT_NO_CELL_SHIFT (the evaluations then read the FP32 nodes). */
+int tg_geometry_set_nodes_f64(tg_geometry* g, const double* nodes_f64);
 
 /* A point set of `cell` with points [nq][tdim] and weights [nq] (host f64), optionally with the
  * outward reference normal of each point's facet [nq][tdim] (a FacetPointSet,
