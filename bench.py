@@ -4,8 +4,9 @@ One step = one fused launch evaluating X, J and det J of every cell of the curve
 tetrahedral mesh (64^3 hexahedra x 6 Kuhn tetrahedra = 1,572,864 cells, 2,146,689 nodes) at the
 14 points of the degree-5 rule, in FP32 (nodes shifted to each cell's first node, FP64 det J at
 near-degenerate points). Inputs are resident in HBM before the timed region. The same run times
-the plain FP64 kernel on the same mesh, a device copy and a write-only fill (measured streaming
-rates), and the FP64 CPU port of the reference's CPU path on a bounded sample of the same mesh.
+the same launch on FP64-stored nodes, the plain FP64 kernel on the same mesh, a device copy and a
+write-only fill (measured streaming rates), and the C++ port of the reference's CPU path (FP64,
+host cores) on the same mesh and on the P1 tetrahedra 32^3 x 6.
 
 Prints one JSON line. ``--gpus N`` with N > 1 exits: this build runs on one GPU.
 """
@@ -75,17 +76,25 @@ def time_graph(fn, replays: int, calls: int = 20):
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "factor_direct_p2tet_64cube_hbm_traffic.txt")
 
 
-def profiled_traffic():
-    """HBM bytes per launch of the FP32 kernel on the 64^3 x 6 mesh, as measured by the two
+def profiled_traffic(kernel_sources):
+    """HBM bytes per launch of the headline kernel on the 64^3 x 6 mesh, as measured by the two
     rocprofv3 counter runs (FETCH_SIZE and WRITE_SIZE, calibrated on a device copy) whose
-    summary is committed with the benchmark; None if that summary is absent."""
+    summary is committed with the benchmark, and whether that summary is stale: the digest of
+    the kernel sources it records differs from the tree's. (None, None) if the summary is
+    absent."""
+    from gridcl_amd.profsum import source_digest
+
+    traffic = digest = None
     try:
         for line in open(TRAFFIC_PROFILE):
-            if line.strip().startswith("traffic_bytes_per_launch:"):
-                return int(line.split(":")[1])
+            line = line.strip()
+            if line.startswith("traffic_bytes_per_launch:"):
+                traffic = int(line.split(":")[1])
+            elif line.startswith("kernel_source_digest:"):
+                digest = line.split(":")[1].strip()
     except OSError:
-        pass
-    return None
+        return None, None
+    return traffic, digest != source_digest(*kernel_sources)
 
 
 def spin_up(fn, seconds: float) -> int:
@@ -117,47 +126,120 @@ def stream_ceilings_gbs() -> tuple[float, float]:
     return 2 * n * 4 / (ms_copy * 1e-3) / 1e9, n * 4 / (ms_fill * 1e-3) / 1e9
 
 
-def cpu_baseline(mesh, ps, sample_cells: int):
-    """The oracle's CPU path (E-vector gather, then X, J, det J in f64 into tiles of 4, OpenMP
-    on the host cores) on the first `sample_cells` cells, timed with the reference protocol
-    (warm-up, calibration to >= 1 s per run, 10 runs, median)."""
-    from oracle import oracle as O
-
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
-    blk = mesh.blocks[0]
-    cells = np.ascontiguousarray(blk.cells[:sample_cells])
-    # The coordinates the FP32 geometry stores (relative to the bounding-box centre, rounded to
-    # FP32), widened to FP64: the same mesh the GPU evaluates.
-    origin = 0.5 * (mesh.nodes.min(0) + mesh.nodes.max(0))
-    rel = (mesh.nodes - origin).astype(np.float32).astype(np.float64)
-    values, derivs = O.tabulate(blk.cell, blk.degree, 0, ps.points)
-    run = lambda: O.cpu_factors(3, 3, blk.nodes_per_cell, rel, origin, cells, values, derivs,
-                                tile=4, threads=threads)
-    out = run()
+def protocol_seconds(fn):
+    """Seconds per call by the reference's timing protocol (tetraloma-bench/src/protocol.rs:1-4,
+    58-72; families/degree.rs:1377-1462): one warm-up call, one calibration call of t_cal, then
+    ten runs of k = ceil(20 ms / t_cal) calls (1 <= k <= 10,000); the median of the runs' mean
+    time per call. Returns (seconds, k)."""
+    fn()
     t0 = time.perf_counter()
-    run()
-    t1 = time.perf_counter() - t0
-    calls = max(1, int(np.ceil(1.0 / max(t1, 1e-6))))
-    times = []
+    fn()
+    cal = time.perf_counter() - t0
+    calls = int(min(10_000, max(1, np.ceil(0.020 / max(cal, 1e-9)))))
+    runs = []
     for _ in range(10):
         t0 = time.perf_counter()
         for _ in range(calls):
-            run()
-        times.append((time.perf_counter() - t0) / calls)
-    med = float(np.median(times))
-    evals = sample_cells * len(ps)
-    model = "unknown"
+            fn()
+        runs.append((time.perf_counter() - t0) / calls)
+    return float(np.median(runs)), calls
+
+
+def host_cpu_model() -> str:
     try:
         with open("/proc/cpuinfo") as f:
-            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")),
-                         model)
+            return next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")),
+                        "unknown")
     except OSError:
-        pass
-    return {"value": evals / med, "unit": "evals/s", "cores": threads, "kind": "port",
-            "cpu_model": model, "host_cpus": os.cpu_count(),
-            "sample": f"first {sample_cells} cells of the benchmark mesh x {len(ps)} points "
-                      f"(FP64, E-vector + X + J + det J, tiles of 4), median of 10 runs of "
-                      f"{calls} calls"}, out
+        return "unknown"
+
+
+def host_threads() -> int:
+    return int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
+
+
+def cpu_port_leg(mesh, ps, label):
+    """The C++ port of the reference's CPU path (oracle/cpu_port.cpp: the E-vector, then the X,
+    J and det J launches over tiles of 4 cells held as 4-wide SIMD lanes, node count fixed at
+    compile time, multiply-adds fused, OpenMP over tiles) in FP64 on `mesh`'s first block. Each
+    launch is timed alone by the reference protocol and the four times are summed, as the
+    reference's degree benchmark sums them. Returns the record and the port (outputs filled)."""
+    from oracle import oracle as O
+
+    threads = host_threads()
+    blk = mesh.blocks[0]
+    origin = 0.5 * (mesh.nodes.min(0) + mesh.nodes.max(0))
+    values, derivs = O.tabulate(blk.cell, blk.degree, 0, ps.points)
+    port = O.CpuPort(blk.nodes_per_cell, mesh.nodes - origin, origin, blk.cells, values, derivs,
+                     threads=threads)
+    port()
+    per, calls = {}, {}
+    for name, bit in (("evector", port.EVECTOR), ("x", port.X), ("j", port.J),
+                      ("detj", port.DETJ)):
+        sec, calls[name] = protocol_seconds(lambda bit=bit: port(bit))
+        per[name] = sec * 1e3
+    total = sum(per.values())
+    evals = blk.num_cells * len(ps)
+    return {"value": evals / (total * 1e-3), "unit": "evals/s", "cores": threads, "kind": "port",
+            "isa": port.lib.isa, "cpu_model": host_cpu_model(), "host_cpus": os.cpu_count(),
+            "ms_per_launch": per, "calls_per_run": calls, "ms_summed": total,
+            "sample": label}, port
+
+
+def untile4(a, cells, nq, length):
+    """[tile of 4][q][length][4] -> [cell][q][length]."""
+    pad = (cells + 3) // 4 * 4
+    a = a.reshape(pad // 4, nq, length, 4).transpose(0, 3, 1, 2)
+    return a.reshape(pad, nq, length)[:cells]
+
+
+def device_rel_err(field, want) -> float:
+    """max |field - want| / max |want| over two fields (QuadFields or their tensors) of one
+    layout, on the device in FP64."""
+    import torch
+
+    a = getattr(field, "data", field).to(torch.float64)
+    b = getattr(want, "data", want).to(torch.float64)
+    return float((a - b).abs().max() / b.abs().max())
+
+
+def host_rel_err(field, want) -> float:
+    got = field.to_host_cell_major().astype(np.float64)
+    return float(np.abs(got - want).max() / np.abs(want).max())
+
+
+def bench_config1(api, M, steps, warmup, cpu: bool):
+    """P1 tetrahedra 32^3 x 6 (196,608 cells, 35,937 nodes) at the 4 Keast points, FP64: the
+    GPU kernel (eager and replayed from a graph; a launch moves 86 MB, so eager calls are
+    launch-bound) and the CPU port, which must agree to FP64 rounding."""
+    import torch
+
+    mesh = M.config1_mesh()
+    ps = api.PointSet.tet_keast4()
+    blk = mesh.blocks[0]
+    C, N, nq = blk.num_cells, mesh.num_nodes, len(ps)
+    names = ["points", "jacobians", "detj"]
+    g = api.geometry_from_host(mesh, dtype=torch.float64)
+    out = {n: torch.empty(int(api._lib.load().tg_field_elems(C, nq, api.factor_len(n, 3, 3), 16)),
+                          dtype=torch.float64, device="cuda") for n in names}
+    step = lambda: api.geometric_factors_block(g, 0, ps, names, out=out)
+    ms = time_steps(step, steps, warmup)
+    ms_graph = time_graph(step, steps)
+    nbytes = algorithmic_bytes(4, C, N, nq, 8)
+    rec = {"cells": C, "nodes": N, "evals": C * nq,
+           "gpu_f64": {"ms": ms, "ms_graph": ms_graph, "evals_per_s": C * nq / (ms * 1e-3),
+                       "evals_per_s_graph": C * nq / (ms_graph * 1e-3),
+                       "achieved_gbs_graph": nbytes / (ms_graph * 1e-3) / 1e9}}
+    if cpu:
+        cpu_rec, port = cpu_port_leg(mesh, ps, f"P1 tetrahedra 32^3x6 = {C} cells x {nq} Keast "
+                                                 "points, whole mesh, FP64")
+        fields = {n: api.QuadField(out[n], C, nq, api.factor_len(n, 3, 3), 16) for n in names}
+        rec["gpu_f64_vs_cpu_port"] = {
+            "points": host_rel_err(fields["points"], untile4(port.x, C, nq, 3)),
+            "jacobians": host_rel_err(fields["jacobians"], untile4(port.j, C, nq, 9)),
+            "detj": host_rel_err(fields["detj"], untile4(port.detj, C, nq, 1))}
+        rec["cpu"] = cpu_rec
+    return rec
 
 
 def bench_hybrid(api, M, steps, warmup, scale=1.0):
@@ -205,14 +287,24 @@ def bench_hybrid(api, M, steps, warmup, scale=1.0):
     }
 
 
-def bench_adjoint(api, M, steps, warmup):
-    """Forward functional and its vjp on the config-5 mesh (P2 tets 32^3 x 6, 14 points)."""
+def adjoint_bytes(k: int, cells: int, nodes: int, s: int):
+    """Algorithmic bytes of the functional's forward (dofmap 4kC + unique nodes 3sN) and vjp
+    (cell pass: dofmap and CSR slots 8kC, nodes 3sN, contributions written 3skC; node pass:
+    contributions read 3skC, CSR offsets 8N, FP64 gradient written 24N)."""
+    return 4 * k * cells + 3 * s * nodes, 8 * k * cells + 3 * s * nodes + 6 * s * k * cells + 32 * nodes
+
+
+def bench_adjoint(api, M, mesh, steps, warmup):
+    """Forward functional and its vjp on a P2 tetrahedral mesh at the 14-point rule, FP32 and
+    FP64: eager calls and graph replays, with each one's fraction of 8 TB/s by the algorithmic
+    bytes of adjoint_bytes()."""
     import torch
 
-    h = M.config5_mesh()
+    h = mesh
     ps = api.PointSet.tet_14()
+    blk = h.blocks[0]
     out = {}
-    for dtype, tdt in (("f32", torch.float32), ("f64", torch.float64)):
+    for dtype, tdt, s in (("f32", torch.float32, 4), ("f64", torch.float64, 8)):
         g = api.geometry_from_host(h, dtype=tdt)
         fn = api.MeshFunctional(g, 0, ps, beta=0.1)
         val = torch.empty((), dtype=torch.float64, device="cuda")
@@ -236,11 +328,18 @@ def bench_adjoint(api, M, steps, warmup):
         ms_bg = time_graph(vjp, steps)
         torch.cuda.synchronize()
         assert torch.equal(grad, grad_eager), "graph-replayed vjp differs from the eager one"
+        bf, bb = adjoint_bytes(blk.nodes_per_cell, blk.num_cells, h.num_nodes, s)
+        gbs = lambda b, ms: b / (ms * 1e-3) / 1e9
         out[dtype] = {"ms_forward": ms_f, "ms_vjp": ms_b, "vjp_over_forward": ms_b / ms_f,
                       "ms_forward_graph": ms_fg, "ms_vjp_graph": ms_bg,
-                      "vjp_over_forward_graph": ms_bg / ms_fg}
+                      "vjp_over_forward_graph": ms_bg / ms_fg,
+                      "bytes_forward": bf, "bytes_vjp": bb,
+                      "roofline_forward": {"achieved_gbs": gbs(bf, ms_fg),
+                                           "frac": gbs(bf, ms_fg) / PEAK_HBM_GBS},
+                      "roofline_vjp": {"achieved_gbs": gbs(bb, ms_bg),
+                                       "frac": gbs(bb, ms_bg) / PEAK_HBM_GBS}}
         del fn, g
-    out["cells"] = h.blocks[0].num_cells
+    out["cells"] = blk.num_cells
     out["nodes"] = h.num_nodes
     return out
 
@@ -401,27 +500,60 @@ def bench_equispaced(api, M, steps, warmup, n=64):
 
 
 def bench_q3(api, M, steps, warmup, n=96):
-    """Q3 hexahedra (96^3 cells, 64 nodes and 64 Gauss points per cell), X + J + det J, FP32."""
+    """Q3 hexahedra (96^3 cells, 64 nodes and 64 Gauss points per cell), X + J + det J: the FP32
+    sum-factorised kernel with its second and third sums on the matrix cores (the default), the
+    same on the vector ALU, FP32 on FP64-stored nodes (sums in FP64), the dense register kernel,
+    and the FP64 kernel; each FP32 result's error against the FP64 kernel over the whole mesh."""
     import torch
 
     h = M.config3_mesh(n=n)
-    g = api.geometry_from_host(h, dtype=torch.float32)
     ps = api.PointSet.gauss(5, 7)
     names = ["points", "jacobians", "detj"]
     C, N = h.blocks[0].num_cells, h.num_nodes
-    out = {nm: torch.empty(int(api._lib.load().tg_field_elems(C, 64, api.factor_len(nm, 3, 3),
-                                                               16)),
-                           dtype=torch.float32, device="cuda") for nm in names}
-    step = lambda: api.geometric_factors_block(g, 0, ps, names, out=out)
-    spin_up(step, 0.3)
-    ms = time_steps(step, steps, warmup)
-    ms_dense = time_steps(lambda: api.geometric_factors_block(
-        g, 0, ps, names, out=out, options=api.OPT_NO_SUMFACT), max(2, steps // 5), 1)
+    lib = api._lib.load()
+
+    def buffers(tdt):
+        return {nm: torch.empty(int(lib.tg_field_elems(C, 64, api.factor_len(nm, 3, 3), 16)),
+                                dtype=tdt, device="cuda") for nm in names}
+
+    g64 = api.geometry_from_host(h, dtype=torch.float64)
+    want = buffers(torch.float64)
+    step64 = lambda: api.geometric_factors_block(g64, 0, ps, names, out=want)
+    ms64 = time_steps(step64, max(2, steps // 2), 1)
+    del g64
+    g = api.geometry_from_host(h, dtype=torch.float32)
+    out = buffers(torch.float32)
+    rec = {"cells": C, "nodes": N}
     nbytes = algorithmic_bytes(64, C, N, 64, 4)
-    return {"cells": C, "nodes": N, "ms": ms, "evals_per_s": C * 64 / (ms * 1e-3),
-            "achieved_gbs": nbytes / (ms * 1e-3) / 1e9, "frac": nbytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
-            "kernel": "sum-factorised FP32 (vertex-mean shift)",
-            "ms_dense_register_kernel": ms_dense}
+    for key, opt in (("mfma", 0), ("valu", api.OPT_NO_MFMA)):
+        step = lambda opt=opt: api.geometric_factors_block(g, 0, ps, names, out=out, options=opt)
+        spin_up(step, 0.3)
+        ms = time_steps(step, steps, warmup)
+        torch.cuda.synchronize()
+        rec[key] = {"ms": ms, "evals_per_s": C * 64 / (ms * 1e-3),
+                    "achieved_gbs": nbytes / (ms * 1e-3) / 1e9,
+                    "frac": nbytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                    "rel_err_vs_fp64_kernel": {nm: device_rel_err(out[nm], want[nm])
+                                               for nm in names}}
+    rec["ms"], rec["frac"] = rec["mfma"]["ms"], rec["mfma"]["frac"]
+    rec["ms_dense_register_kernel"] = time_steps(lambda: api.geometric_factors_block(
+        g, 0, ps, names, out=out, options=api.OPT_NO_SUMFACT), max(2, steps // 5), 1)
+    g.set_nodes_f64(torch.tensor(h.nodes - np.asarray(g.origin), dtype=torch.float64,
+                                 device="cuda"))
+    step = lambda: api.geometric_factors_block(g, 0, ps, names, out=out)
+    ms = time_steps(step, steps, warmup)
+    torch.cuda.synchronize()
+    nb64 = nbytes + 3 * 4 * N
+    rec["fp32_nodes64"] = {"ms": ms, "evals_per_s": C * 64 / (ms * 1e-3),
+                           "achieved_gbs": nb64 / (ms * 1e-3) / 1e9,
+                           "frac": nb64 / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                           "rel_err_vs_fp64_kernel": {nm: device_rel_err(out[nm], want[nm])
+                                                      for nm in names}}
+    rec["fp64"] = {"ms": ms64, "evals_per_s": C * 64 / (ms64 * 1e-3)}
+    return rec
+
+
+HEADLINE_SOURCES = ("factor_kernels.hip",)
 
 
 def main() -> int:
@@ -430,12 +562,11 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=64, help="hexahedra per direction (64: full size)")
-    ap.add_argument("--cpu-sample", type=int, default=262_144)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--spin-up", type=float, default=0.5,
                     help="seconds of untimed back-to-back steps before the warm-up")
     ap.add_argument("--no-extras", action="store_true",
-                    help="skip the Q3, mixed-mesh and adjoint legs")
+                    help="skip the config-1, Q3, mixed-mesh, adjoint and other legs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): cell blocks would shard "
@@ -456,10 +587,13 @@ def main() -> int:
     nq = len(ps)
     names = ["points", "jacobians", "detj"]
 
+    # (key, output type, node storage): FP32 on FP32 nodes, FP32 on FP64 nodes, FP64.
     results = {}
     fields = {}
-    for dtype, tdt, s in (("f32", torch.float32, 4), ("f64", torch.float64, 8)):
-        g = api.geometry_from_host(mesh, dtype=tdt)
+    for key, tdt, s_out, nodes64 in (("f32", torch.float32, 4, False),
+                                     ("f32_nodes64", torch.float32, 4, True),
+                                     ("f64", torch.float64, 8, False)):
+        g = api.geometry_from_host(mesh, dtype=tdt, nodes_f64=nodes64)
         out = {n: torch.empty(int(api._lib.load().tg_field_elems(C, nq, api.factor_len(n, 3, 3),
                                                                   16)),
                               dtype=tdt, device="cuda") for n in names}
@@ -467,54 +601,55 @@ def main() -> int:
                                                                 out=out)
         spun = spin_up(step, args.spin_up)
         ms = time_steps(step, args.steps, args.warmup)
-        nbytes = algorithmic_bytes(k, C, N, nq, s)
-        results[dtype] = {"ms": ms, "evals_per_s": C * nq / (ms * 1e-3), "bytes": nbytes,
-                          "achieved_gbs": nbytes / (ms * 1e-3) / 1e9}
-        fields[dtype] = {n: api.QuadField(out[n], C, nq, api.factor_len(n, 3, 3), 16)
-                         for n in names}
-        log(f"{dtype}: {spun} spin-up calls, {ms * 1e3:.1f} us/step, {results[dtype]['evals_per_s'] / 1e9:.1f} G evals/s, "
-            f"{results[dtype]['achieved_gbs']:.0f} GB/s")
+        nbytes = algorithmic_bytes(k, C, N, nq, s_out) + (3 * 4 * N if nodes64 else 0)
+        results[key] = {"ms": ms, "evals_per_s": C * nq / (ms * 1e-3), "bytes": nbytes,
+                        "achieved_gbs": nbytes / (ms * 1e-3) / 1e9}
+        fields[key] = {n: api.QuadField(out[n], C, nq, api.factor_len(n, 3, 3), 16)
+                       for n in names}
+        log(f"{key}: {spun} spin-up calls, {ms * 1e3:.1f} us/step, "
+            f"{results[key]['evals_per_s'] / 1e9:.1f} G evals/s, "
+            f"{results[key]['achieved_gbs']:.0f} GB/s")
         del g
     ceiling, write_ceiling = stream_ceilings_gbs()
     log(f"device copy {ceiling:.0f} GB/s, fill (write only) {write_ceiling:.0f} GB/s")
 
+    # FP32 against the FP64 kernel over the whole mesh: with FP32 node storage the rounding of
+    # the stored coordinates dominates; with FP64 node storage only the FP32 arithmetic is left.
+    vs_fp64 = {key: {n: device_rel_err(fields[key][n].data, fields["f64"][n].data)
+                     for n in names} for key in ("f32", "f32_nodes64")}
+    log(f"FP32 vs FP64 kernel (whole mesh): {vs_fp64}")
+
     cpu = None
-    accuracy = None
     if not args.no_cpu:
-        sample = min(args.cpu_sample, C)
-        cpu, (x, j, dj) = cpu_baseline(mesh, ps, sample)
-        log(f"cpu baseline {cpu['value'] / 1e9:.3f} G evals/s on {cpu['cores']} threads")
-        # The CPU leg's FP64 values on its sample against the GPU FP32 fields (rel_err).
-        pad = (sample + 3) // 4 * 4
-
-        def untile(a, length, tile=4):
-            a = a.reshape(pad // tile, nq, length, tile).transpose(0, 3, 1, 2)
-            return a.reshape(pad, nq, length)[:sample]
-
-        want = {"points": untile(x, 3), "jacobians": untile(j, 9), "detj": untile(dj, 1)}
-        accuracy = {}
-        for n in names:
-            got = fields["f32"][n].to_host_cell_major()[:sample].astype(np.float64)
-            accuracy[n] = float(np.abs(got - want[n]).max() / np.abs(want[n]).max())
-        log(f"FP32 max rel_err vs FP64 on the sample: {accuracy}")
-
-    # FP32 against the FP64 kernel over the whole mesh: rounding of the stored coordinates plus
-    # the FP32 arithmetic.
-    vs_fp64_kernel = {}
-    for n in names:
-        a = fields["f32"][n].to_host_cell_major().astype(np.float64)
-        b = fields["f64"][n].to_host_cell_major()
-        vs_fp64_kernel[n] = float(np.abs(a - b).max() / np.abs(b).max())
-        del a, b
-    log(f"FP32 vs FP64 kernel (whole mesh, incl. FP32 storage of nodes): {vs_fp64_kernel}")
+        cpu, port = cpu_port_leg(mesh, ps, f"curved P2 tetrahedra {args.n}^3x6 = {C} cells x "
+                                           f"{nq} points, whole mesh, FP64")
+        log(f"cpu port {cpu['value'] / 1e9:.3f} G evals/s on {cpu['cores']} threads "
+            f"({cpu['isa']}), per launch ms {cpu['ms_per_launch']}")
+        # The CPU port and the GPU FP64 kernel compute the same FP64 values (to rounding).
+        cpu["gpu_f64_vs_port"] = {
+            "points": host_rel_err(fields["f64"]["points"], untile4(port.x, C, nq, 3)),
+            "jacobians": host_rel_err(fields["f64"]["jacobians"], untile4(port.j, C, nq, 9)),
+            "detj": host_rel_err(fields["f64"]["detj"], untile4(port.detj, C, nq, 1))}
+        log(f"GPU FP64 vs CPU port: {cpu['gpu_f64_vs_port']}")
+        del port
 
     extras = None
     if not args.no_extras:
         del fields
         torch.cuda.empty_cache()
         extras = {}
-        extras["adjoint_p2tet_32"] = bench_adjoint(api, M, args.steps, args.warmup)
-        log(f"adjoint: {extras['adjoint_p2tet_32']}")
+        c1 = bench_config1(api, M, args.steps, args.warmup, not args.no_cpu)
+        if cpu is not None:
+            cpu["config1"] = c1.pop("cpu")
+            cpu["config1"]["gpu_f64_vs_port"] = c1.pop("gpu_f64_vs_cpu_port")
+        extras["config1_p1tet_32_f64"] = c1
+        log(f"config 1: {c1}; cpu {cpu and cpu.get('config1')}")
+        extras["adjoint_p2tet_32"] = bench_adjoint(api, M, M.config5_mesh(), args.steps,
+                                                   args.warmup)
+        log(f"adjoint 32: {extras['adjoint_p2tet_32']}")
+        extras["adjoint_p2tet_64"] = bench_adjoint(api, M, mesh, args.steps, args.warmup)
+        log(f"adjoint 64: {extras['adjoint_p2tet_64']}")
+        torch.cuda.empty_cache()
         extras["hybrid"] = bench_hybrid(api, M, args.steps, args.warmup)
         log(f"hybrid: {extras['hybrid']}")
         torch.cuda.empty_cache()
@@ -531,7 +666,8 @@ def main() -> int:
         extras["gmsh_read"] = bench_gmsh(api, M, mesh)
         log(f"gmsh: {extras['gmsh_read']}")
 
-    r32, r64 = results["f32"], results["f64"]
+    r32, r64, rn = results["f32"], results["f64"], results["f32_nodes64"]
+    traffic, stale = profiled_traffic(HEADLINE_SOURCES) if args.n == 64 else (None, None)
     line = {
         "metric": METRIC,
         "value": r32["evals_per_s"],
@@ -552,8 +688,12 @@ def main() -> int:
         },
         "roofline": {"bound": "hbm", "achieved": r32["achieved_gbs"], "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": r32["achieved_gbs"] / PEAK_HBM_GBS,
-                     "traffic": profiled_traffic() if args.n == 64 else None},
+                     "traffic": traffic, "traffic_profile_stale": stale},
         "cpu_baseline": cpu,
+        "fp32_nodes64": {"evals_per_s": rn["evals_per_s"], "ms_per_step": rn["ms"],
+                         "achieved_gbs": rn["achieved_gbs"],
+                         "frac": rn["achieved_gbs"] / PEAK_HBM_GBS,
+                         "rel_err_vs_fp64_kernel": vs_fp64["f32_nodes64"]},
         "fp64": {"evals_per_s": r64["evals_per_s"], "ms_per_step": r64["ms"],
                  "achieved_gbs": r64["achieved_gbs"],
                  "frac": r64["achieved_gbs"] / PEAK_HBM_GBS},
@@ -562,10 +702,15 @@ def main() -> int:
         "copy_gbs": ceiling,
         "write_ceiling_gbs": write_ceiling,
         "frac_of_write_ceiling": r32["achieved_gbs"] / write_ceiling,
-        "fp32_rel_err_vs_fp64_sample": accuracy,
-        "fp32_rel_err_vs_fp64_kernel": vs_fp64_kernel,
+        "fp32_rel_err_vs_fp64_kernel": vs_fp64["f32"],
         "extras": extras,
     }
+    if extras:
+        line["roofline_adjoint"] = {
+            name: {dt: {"forward": extras[name][dt]["roofline_forward"]["frac"],
+                        "vjp": extras[name][dt]["roofline_vjp"]["frac"]}
+                   for dt in ("f32", "f64")}
+            for name in ("adjoint_p2tet_32", "adjoint_p2tet_64")}
     print(json.dumps(line), flush=True)
     return 0
 

@@ -3,20 +3,46 @@
     python -m gridcl_amd.profsum stats  <prefix>_kernel_stats.csv [--top N]
     python -m gridcl_amd.profsum pmc    <prefix>_counter_collection.csv [--match SUBSTRING]
     python -m gridcl_amd.profsum trace  <prefix>_kernel_trace.csv [--match SUBSTRING] [--last N]
+    python -m gridcl_amd.profsum digest <kernel source> [...]
 
 ``stats`` prints calls and average/min/max duration per kernel; ``pmc`` prints, per kernel,
 counter and grid size, the number of dispatches and the mean, min and max counter value per
 dispatch (FETCH_SIZE and WRITE_SIZE are in KiB); ``trace`` prints, per kernel and grid size, the
 dispatch count and the average, min and max duration, optionally of the last N dispatches only
-(the timed steps of a run).
+(the timed steps of a run); ``digest`` prints the content digest of kernel sources and the local
+headers they include, which each profile summary records so that a reader (and the benchmark,
+for the traffic figure) can tell whether the profiled kernel is still the one in the tree.
 """
 from __future__ import annotations
 
 import argparse
 import csv
+import hashlib
+import os
 import re
 import sys
 from collections import defaultdict
+
+CSRC = os.path.join(os.path.dirname(os.path.abspath(__file__)), "csrc")
+
+
+def source_digest(*names: str) -> str:
+    """The first 16 hex digits of SHA-256 over the named csrc files and the local headers they
+    include, transitively, in sorted order (name and content)."""
+    seen, todo = set(), list(names)
+    while todo:
+        n = todo.pop()
+        if n in seen:
+            continue
+        seen.add(n)
+        with open(os.path.join(CSRC, n), encoding="utf-8") as f:
+            todo += re.findall(r'^#include "([^"]+)"', f.read(), flags=re.M)
+    h = hashlib.sha256()
+    for n in sorted(seen):
+        h.update(n.encode() + b"\0")
+        with open(os.path.join(CSRC, n), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
 
 
 def short_name(name: str, width: int = 96) -> str:
@@ -86,8 +112,12 @@ def main(argv=None) -> int:
     c.add_argument("--last", type=int, default=0)
     c.add_argument("--drop", type=int, default=0,
                    help="leave out the last DROP dispatches of each (kernel, grid) first")
+    d = sub.add_parser("digest")
+    d.add_argument("sources", nargs="+")
     args = ap.parse_args(argv)
-    if args.cmd == "stats":
+    if args.cmd == "digest":
+        print(source_digest(*args.sources))
+    elif args.cmd == "stats":
         stats(args.csv, args.top)
     elif args.cmd == "pmc":
         pmc(args.csv, args.match, args.drop)

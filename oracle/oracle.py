@@ -186,6 +186,69 @@ def pull_back(cell, degree, family, gdim, nodes, dofmap, x, max_it=16, tol=1e-13
     return xi, step
 
 
+_port = None
+
+
+def cpu_port_lib() -> ctypes.CDLL:
+    """The CPU-baseline port (cpu_port.cpp): the AVX-512 build where the host CPU has AVX-512F,
+    else the AVX2/FMA build."""
+    global _port
+    if _port is None:
+        flags = ""
+        try:
+            with open("/proc/cpuinfo") as f:
+                flags = next((ln for ln in f if ln.startswith("flags")), "")
+        except OSError:
+            pass
+        name = "libcpuport_v4.so" if " avx512f" in flags else "libcpuport_v3.so"
+        path = os.path.join(HERE, "_build", name)
+        if not os.path.exists(path):
+            build()
+        lib = ctypes.CDLL(path)
+        lib.cpu_port_factors.restype = c_int
+        lib.cpu_port_factors.argtypes = [c_int, ctypes.c_uint, POINTER(c_double), c_size_t,
+                                         POINTER(c_double),
+                                         c_void_p, c_size_t, POINTER(c_double), POINTER(c_double),
+                                         c_size_t, c_void_p, c_void_p, c_void_p, c_void_p, c_int]
+        lib.isa = name
+        _port = lib
+    return _port
+
+
+class CpuPort:
+    """The port's buffers for one tetrahedral block (k = 4, 10 or 20) at one point set; call()
+    runs the passes named by `passes` (bit 0 E-vector, 1 X, 2 J, 3 det J; outputs in tiles of 4:
+    [tile][q][len][4])."""
+
+    EVECTOR, X, J, DETJ, ALL = 1, 2, 4, 8, 15
+
+    def __init__(self, k, nodes_rel, origin, dofmap, values, derivs, threads=0):
+        self.lib = cpu_port_lib()
+        self.k = k
+        self.nodes = np.ascontiguousarray(nodes_rel, dtype=np.float64)
+        self.origin = np.ascontiguousarray(origin, dtype=np.float64)
+        self.dofmap = np.ascontiguousarray(dofmap, dtype=np.uint32).reshape(-1)
+        self.values = np.ascontiguousarray(values, dtype=np.float64)
+        self.derivs = np.ascontiguousarray(derivs, dtype=np.float64)
+        self.cells = self.dofmap.size // k
+        self.nq = self.values.shape[0]
+        padded = (self.cells + 3) // 4 * 4
+        self.ev = np.empty(3 * k * padded)
+        self.x = np.empty(padded * self.nq * 3)
+        self.j = np.empty(padded * self.nq * 9)
+        self.detj = np.empty(padded * self.nq)
+        self.threads = threads
+
+    def __call__(self, passes=15):
+        rc = self.lib.cpu_port_factors(self.k, passes, _d(self.nodes), self.nodes.shape[0],
+                                       _d(self.origin), _p(self.dofmap), self.cells,
+                                       _d(self.values), _d(self.derivs), self.nq, _p(self.ev),
+                                       _p(self.x), _p(self.j), _p(self.detj), self.threads)
+        if rc:
+            raise ValueError("cpu_port_factors: unsupported element")
+        return self.x, self.j, self.detj
+
+
 def rel_err(got, want) -> float:
     got = np.ascontiguousarray(got, dtype=np.float64).reshape(-1)
     want = np.ascontiguousarray(want, dtype=np.float64).reshape(-1)

@@ -148,3 +148,27 @@ def test_pull_back_surface_closest_point():
     J = f["jacobians"].reshape(*f["jacobians"].shape[:2], 2, 3)
     r = x - f["points"]
     assert np.abs(np.einsum("cqdg,cqg->cqd", J, r)).max() <= 1e-15
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+def test_cpu_port_matches_oracle(p):
+    """The CPU-baseline port (the reference's CPU schedule: E-vector, then X, J and det J passes
+    over tiles of 4 cells, multiply-adds fused) gives the oracle's values to FP64 rounding."""
+    mesh = M.jitter(M.cube_box(TET, 3, p), 0.05 / p, 7, interior_only=True)
+    blk = mesh.blocks[0]
+    ps = PointSet.tet_keast4() if p == 1 else PointSet.tet_14()
+    origin = np.array([0.5, 0.5, 0.5])
+    values, derivs = O.tabulate(TET, p, 0, ps.points)
+    port = O.CpuPort(blk.nodes_per_cell, mesh.nodes - origin, origin, blk.cells, values, derivs,
+                     threads=2)
+    x, j, dj = port()
+    want = O.eval_block(TET, p, 0, 3, mesh.nodes, blk.cells.reshape(-1), ps.points)
+    c, nq = blk.num_cells, len(ps)
+    pad = (c + 3) // 4 * 4
+
+    def untile(a, ln):
+        return a.reshape(pad // 4, nq, ln, 4).transpose(0, 3, 1, 2).reshape(pad, nq, ln)[:c]
+
+    assert O.rel_err(untile(x, 3), want["points"]) < 1e-14
+    assert O.rel_err(untile(j, 9), want["jacobians"]) < 1e-13
+    assert O.rel_err(untile(dj, 1), want["detj"]) < 1e-13
