@@ -233,6 +233,24 @@ int device_tables(tg_pointset* p, unsigned degree, int family, int dtype, Device
   return TG_OK;
 }
 
+int resident_tables(tg_pointset* p, unsigned degree, int family, int dtype, DeviceTables** out) {
+  auto it = p->cache.find(std::make_tuple(degree, family, dtype));
+  if (it == p->cache.end())
+    return fail(TG_ERR_INVALID_ARGUMENT,
+                "the point set's device tables for this degree, node family and precision are "
+                "not resident (an evaluation or tg_measure_block with them uploads them)");
+  *out = it->second.get();
+  return TG_OK;
+}
+
+int resident_weights(const tg_pointset* p, const double** out) {
+  if (!p->weights_device)
+    return fail(TG_ERR_INVALID_ARGUMENT,
+                "the point set's weights are not resident (tg_measure_block uploads them)");
+  *out = p->weights_device;
+  return TG_OK;
+}
+
 int device_weights(tg_pointset* p, const double** out) {
   if (!p->weights_device) {
     void* d = nullptr;

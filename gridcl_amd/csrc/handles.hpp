@@ -97,6 +97,9 @@ namespace tg {
 int device_tables(tg_pointset* p, unsigned degree, int family, int dtype, DeviceTables** out);
 // The point weights on the device (f64), uploaded on first use.
 int device_weights(tg_pointset* p, const double** out);
+// The same two lookups without the upload: TG_ERR_INVALID_ARGUMENT where nothing is resident.
+int resident_tables(tg_pointset* p, unsigned degree, int family, int dtype, DeviceTables** out);
+int resident_weights(const tg_pointset* p, const double** out);
 // Enqueues every factor in `factors` of block b at pts (the dispatch of tg_eval_block: the
 // sum-factorised kernel where it applies, else the register-resident or runtime-K kernel).
 // Returns a TG status.

@@ -3,7 +3,11 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
 #include <cstdint>
+
+struct tg_geometry;
+struct tg_pointset;
 
 namespace tg {
 
@@ -23,5 +27,13 @@ hipError_t launch_measure(const T* detj, const double* weights, uint32_t nq, uin
 // len + ceil(len / 64) elements and is overwritten.
 template <typename T>
 hipError_t launch_measure_sums(uint64_t len, T* scratch, double* value, hipStream_t stream);
+
+// tg_measure_block with caller-provided work memory: `work` (256-byte aligned) of work_bytes
+// bytes, or nullptr to allocate it stream-ordered for the call. With `work` nothing is
+// allocated: the point set's device tables and weights must be resident (TG_ERR_INVALID_ARGUMENT
+// otherwise), and a block that needs more than work_bytes fails with TG_ERR_TOO_LARGE; both
+// before anything is enqueued.
+int measure_block_with(const tg_geometry* g, int32_t block, tg_pointset* pts, uint32_t options,
+                       double* value, void* work, size_t work_bytes, hipStream_t stream);
 
 }  // namespace tg

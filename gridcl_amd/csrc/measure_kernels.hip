@@ -4,12 +4,12 @@
 #include <hip/hip_runtime.h>
 
 #include "measure_internal.hpp"
+#include "measure_sums.hpp"
 
 namespace tg {
 namespace {
 
 constexpr uint32_t kBlock = 256;
-constexpr uint32_t kSumBlock = 64;  // values per output of one blocked pass (measure.rs:44)
 
 // out[c] = sum_q det J(c, q) w_q for the cells c < num_cells of a det J field tiled by 16
 // (cell c at ((c / 16) nq + q) 16 + c % 16), starting from q = 0 (measure.rs:17-41); the
@@ -27,28 +27,6 @@ __global__ void __launch_bounds__(kBlock)
   T acc = d[0] * static_cast<T>(weights[0]);
   for (uint32_t q = 1; q < nq; ++q) acc += d[q * 16] * static_cast<T>(weights[q]);
   out[c] = acc;
-}
-
-// x[0] + x[1] + ... + x[count - 1] in order (0 < count <= 64), the values loaded a chunk at a
-// time into registers before they are added, so the loads overlap.
-template <typename T>
-__device__ __forceinline__ T ordered_sum(const T* __restrict__ x, uint32_t count) {
-  constexpr uint32_t kChunk = sizeof(T) == 4 ? 64 : 32;
-  T acc = T(0);
-#pragma unroll 1
-  for (uint32_t c0 = 0; c0 < kSumBlock; c0 += kChunk) {
-    T v[kChunk];
-#pragma unroll
-    for (uint32_t j = 0; j < kChunk; ++j) v[j] = c0 + j < count ? x[c0 + j] : T(0);
-#pragma unroll
-    for (uint32_t j = 0; j < kChunk; ++j) {
-      if (c0 + j == 0)
-        acc = v[0];
-      else if (c0 + j < count)
-        acc += v[j];
-    }
-  }
-  return acc;
 }
 
 // out[i] = x[64 i] + x[64 i + 1] + ... over the entries below len, in order (measure.rs:46-72).
@@ -96,6 +74,7 @@ __global__ void __launch_bounds__(kFinish)
 __global__ void store_zero(double* __restrict__ value) {
   if (threadIdx.x == 0) *value = 0.0;
 }
+
 
 uint32_t grid_of(uint64_t n) { return static_cast<uint32_t>((n + kBlock - 1) / kBlock); }
 

@@ -183,16 +183,19 @@ def unit_square_triangles(n):
     return M.HostMesh(2, nodes, [M.HostBlock(TRI, 1, cells.astype(np.uint32))])
 
 
+@pytest.mark.parametrize("n", [1448, 1500])
 @pytest.mark.parametrize("options", [0, api.OPT_FORCE_GENERIC])
 @pytest.mark.parametrize("dtype", ["f32", "f64"])
-def test_measure_passes_beyond_one_workgroup(cuda, options, dtype):
-    """4,500,000 triangles: more first-pass sums (70,313) than the finishing workgroup takes, so
-    a grid-wide pass of 64-value sums runs first, on the one-kernel path and on the det J field
-    (FORCE_GENERIC); bit for bit the host's reduction of the field, and the square's area 1
-    (FP64 1e-12; FP32 1e-4, the rounding of 4.5 million sums in FP32)."""
+def test_measure_passes_beyond_one_workgroup(cuda, n, options, dtype):
+    """2 n^2 triangles. n = 1500: 4,500,000 cells, more first-pass sums (70,313) than the
+    finishing workgroup takes (65,536), so a grid-wide pass of 64-value sums runs first; n = 1448:
+    4,193,408 cells, 65,522 first-pass sums, nearly the most the finishing workgroup takes alone.
+    On the one-kernel path and on the det J field (FORCE_GENERIC): bit for bit the host's
+    reduction of the field, and the square's area 1 (FP64 1e-12; FP32 1e-4, the rounding of
+    millions of sums in FP32)."""
     import torch
 
-    mesh = unit_square_triangles(1500)
+    mesh = unit_square_triangles(n)
     tdt = torch.float64 if dtype == "f64" else torch.float32
     npdt = np.float64 if dtype == "f64" else np.float32
     g = api.geometry_from_host(mesh, dtype=tdt)
