@@ -66,6 +66,8 @@ SIGNATURES = {
     "tg_plan_launch": (c_int32, [c_void_p, c_void_p]),
     "tg_plan_graph_launch": (c_int32, [c_void_p, c_void_p]),
     "tg_measure_block": (c_int32, [c_void_p, c_int32, c_void_p, c_uint32, c_void_p, c_void_p]),
+    "tg_measure_block_noalloc": (c_int32, [c_void_p, c_int32, c_void_p, c_uint32, c_void_p,
+                                           c_void_p]),
     "tg_equispaced_nodes": (c_int32, [c_void_p, c_void_p, c_void_p]),
     "tg_pull_back_block": (
         c_int32, [c_void_p, c_int32, c_void_p, c_uint64, c_uint32, c_uint32, c_double, c_uint32,

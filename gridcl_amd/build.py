@@ -29,6 +29,7 @@ HIP_SOURCES = [
     ("sumfact_mfma.hip", ["-mllvm", "-amdgpu-mfma-vgpr-form"]),
     ("adjoint_kernels.hip", []),
     ("measure_kernels.hip", []),
+    ("measure_noalloc.hip", []),
     ("pullback_kernels.hip", []),
     ("equispaced_kernels.hip", []),
 ]

@@ -280,6 +280,22 @@ int tg_plan_graph_launch(tg_plan* plan, void* stream);
 int tg_measure_block(const tg_geometry* g, int32_t block, tg_pointset* pts, uint32_t options,
                      double* value, void* stream);
 
+/* No counterpart in the reference: tg_measure_block for callers that cannot allocate device
+ * memory (code under graph capture, tight serving loops, harnesses that forbid hipMalloc). Its
+ * work memory is a fixed array of TG_MEASURE_SCRATCH_BYTES per device inside the library, held
+ * until the process ends; nothing is allocated. It enqueues the kernels of tg_measure_block, so
+ * the value is the same bit for bit. The point set's device tables and weights must be resident
+ * (a tg_measure_block call with the same point set, degree, node family and precision uploads
+ * them), else TG_ERR_INVALID_ARGUMENT. A block whose work memory exceeds the array is
+ * TG_ERR_TOO_LARGE, before anything is enqueued: blocks measured by the register kernels need
+ * ceil(cells / 64) values plus 1/64 of that, up to about 16.5 M FP32 or 8.2 M FP64 cells; the
+ * others (sum-factorised, runtime-K) need the det J field, cells x points + cells values.
+ * All calls on a device share its array: calls that overlap on one device (several streams or
+ * host threads) race, and the caller must order them. */
+#define TG_MEASURE_SCRATCH_BYTES (1u << 20)
+int tg_measure_block_noalloc(const tg_geometry* g, int32_t block, tg_pointset* pts,
+                             uint32_t options, double* value, void* stream);
+
 /* ---- Pull-back --------------------------------------------------------------------------------
  *
  * pull_back_block (tetraloma-geometry/src/pull_back.rs:566): the reference points of physical
