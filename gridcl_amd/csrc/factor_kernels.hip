@@ -36,9 +36,20 @@ __device__ __forceinline__ double tsqrt(double x) { return __builtin_sqrt(x); }
 __device__ __forceinline__ float tfma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 __device__ __forceinline__ double tfma(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
+// Global-memory views of pointers. The kernels' outputs, nodes, dofmaps and tables all live in
+// global memory; accessed through generic pointers read from memory (the mixed-mesh kernel's
+// block records) they would become flat accesses, which also count against the LDS wait counter,
+// so that every wait for an LDS read would wait for the stores in flight as well.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+template <typename T>
+__device__ __forceinline__ gptr<T> gp(T* p) {
+  return (gptr<T>)p;
+}
+
 template <typename T>
 __device__ __forceinline__ void st(void* base, uint64_t idx, T v) {
-  __builtin_nontemporal_store(v, reinterpret_cast<T*>(base) + idx);
+  __builtin_nontemporal_store(v, gp(reinterpret_cast<T*>(base)) + idx);
 }
 
 // Stores every factor of J that the mask asks for at point q (layout.rs:275-369).
@@ -47,6 +58,7 @@ __device__ __forceinline__ void store_from_j(const T* j, T detj, const T* __rest
                                              const uint32_t* __restrict__ flags, uint64_t c,
                                              uint32_t q, uint32_t mask, uint64_t row,
                                              uint32_t tl, uint32_t lane, const FactorOutputs& o) {
+#pragma clang fp contract(off)  // as in jacobian.hpp: the same values in every kernel
   constexpr int JL = TD * GD;
   if (mask & kFJac) {
 #pragma unroll
@@ -66,9 +78,9 @@ __device__ __forceinline__ void store_from_j(const T* j, T detj, const T* __rest
         T s2 = T(0);
 #pragma unroll
         for (int g = 0; g < GD; ++g) {
-          T acc = k[g * TD] * ref_normals[q * TD];
+          T acc = k[g * TD] * gp(ref_normals)[q * TD];
 #pragma unroll
-          for (int t = 1; t < TD; ++t) acc += k[g * TD + t] * ref_normals[q * TD + t];
+          for (int t = 1; t < TD; ++t) acc += k[g * TD + t] * gp(ref_normals)[q * TD + t];
           m[g] = acc;
           s2 = g == 0 ? acc * acc : s2 + acc * acc;
         }
@@ -93,7 +105,7 @@ __device__ __forceinline__ void store_from_j(const T* j, T detj, const T* __rest
 #pragma unroll
       for (int g = 1; g < GD; ++g) s2 += n[g] * n[g];
       const T nrm = tsqrt(s2);
-      const T s = (flags != nullptr && (flags[c] & 1u)) ? T(-1) : T(1);
+      const T s = (flags != nullptr && (gp(flags)[c] & 1u)) ? T(-1) : T(1);
 #pragma unroll
       for (int g = 0; g < GD; ++g) st<T>(o.normals, (((row * GD) + g) << tl) | lane, (s * n[g]) / nrm);
     }
@@ -261,12 +273,13 @@ __device__ __forceinline__ void contract(const T (&x)[K][GD], const T* __restric
     contract_fixed<T, TD, GD, K, SHIFT, true, false>(x, t, xv, j);
 }
 
-// det J, recomputed in FP64 from the stored coordinates where |det J| < tau |J|_F^3 (FP32).
-template <typename T, int TD, int GD, int K, typename TN>
-__device__ __forceinline__ T detj_checked(const T* j, const FactorParams<T>& p,
-                                          const TN* __restrict__ nodes,
-                                          const uint32_t* __restrict__ row,
-                                          const double* __restrict__ tab64, uint32_t q) {
+// det J, recomputed in FP64 from the stored coordinates where |det J| < tau |J|_F^3 (FP32), for
+// a cell of k nodes.
+template <typename T, int TD, int GD, typename TN>
+__device__ __forceinline__ T detj_checked_k(const T* j, const FactorParams<T>& p,
+                                            const TN* __restrict__ nodes,
+                                            const uint32_t* __restrict__ row, uint32_t k,
+                                            const double* __restrict__ tab64, uint32_t q) {
   T detj = det_of<T, TD, GD>(j);
   if constexpr (TD == GD && sizeof(T) == 4) {
     if (p.fixup) {
@@ -274,10 +287,18 @@ __device__ __forceinline__ T detj_checked(const T* j, const FactorParams<T>& p,
 #pragma unroll
       for (int i = 0; i < TD * GD; ++i) n2 = tfma(j[i], j[i], n2);
       if (__builtin_fabsf(detj) < static_cast<T>(p.tau) * n2 * tsqrt(n2))
-        detj = static_cast<T>(detj_fp64<TD, GD>(nodes, row, K, p.last_node, tab64, q));
+        detj = static_cast<T>(detj_fp64<TD, GD>(nodes, row, k, p.last_node, tab64, q));
     }
   }
   return detj;
+}
+
+template <typename T, int TD, int GD, int K, typename TN>
+__device__ __forceinline__ T detj_checked(const T* j, const FactorParams<T>& p,
+                                          const TN* __restrict__ nodes,
+                                          const uint32_t* __restrict__ row,
+                                          const double* __restrict__ tab64, uint32_t q) {
+  return detj_checked_k<T, TD, GD, TN>(j, p, nodes, row, K, tab64, q);
 }
 
 // All requested factors of the cell of lane cl (cl < padded cells; lanes past the last cell
@@ -331,8 +352,8 @@ __device__ __forceinline__ void copy_point(const T* __restrict__ stage, void* ou
     const uint64_t ta = tile0 + t;
     if (ta < ntiles) {
       const V v = *reinterpret_cast<const V*>(stage + t * LEN * 16 + jj * E);
-      T* dst = static_cast<T*>(out) + ((ta * nq + q) * LEN) * 16 + jj * E;
-      __builtin_nontemporal_store(v, reinterpret_cast<V*>(dst));
+      gptr<T> dst = gp(static_cast<T*>(out)) + ((ta * nq + q) * LEN) * 16 + jj * E;
+      __builtin_nontemporal_store(v, reinterpret_cast<gptr<V>>(dst));
     }
   }
 }
@@ -399,10 +420,10 @@ __device__ __forceinline__ void copy_rows(const T* __restrict__ stage, void* out
   using V = typename Vec16<T>::type;
   constexpr uint32_t E = 16 / sizeof(T);
   const uint32_t n = nv * (LEN * 16 / E);
-  T* dst = static_cast<T*>(out) + ((tile * nq + q0) * LEN) * 16;
+  gptr<T> dst = gp(static_cast<T*>(out)) + ((tile * nq + q0) * LEN) * 16;
   for (uint32_t s = w; s < n; s += 64) {
     const V v = *reinterpret_cast<const V*>(stage + s * E);
-    __builtin_nontemporal_store(v, reinterpret_cast<V*>(dst + s * E));
+    __builtin_nontemporal_store(v, reinterpret_cast<gptr<V>>(dst + s * E));
   }
 }
 
@@ -472,8 +493,8 @@ __device__ __forceinline__ void table_to_lds(const T* __restrict__ src, T* __res
                                              uint32_t n) {
   using V = typename Vec16<T>::type;
   constexpr uint32_t E = 16 / sizeof(T);
-  for (uint32_t i = threadIdx.x; i < n / E; i += kBlock)
-    reinterpret_cast<V*>(dst)[i] = reinterpret_cast<const V*>(src)[i];
+  for (uint32_t i = threadIdx.x; i < n / E; i += blockDim.x)
+    reinterpret_cast<V*>(dst)[i] = reinterpret_cast<gptr<const V>>(gp(src))[i];
 }
 
 // One block. kMapTile: one wavefront per tile of 16 cells, the basis table staged in LDS;
@@ -516,81 +537,197 @@ __global__ void __launch_bounds__(kBlock)
   X(12, 2, 3, 4) X(13, 2, 3, 9) X(14, 2, 2, 3) X(15, 2, 2, 6) X(16, 2, 2, 10) X(17, 2, 2, 4) \
   X(18, 2, 2, 9)
 
-// One persistent launch over the work items of every block of a mesh (at least two waves per
-// SIMD: the kernel holds every shape's code, and unbounded it would take one): items, in block order,
-// are split on the host into one contiguous range of about equal work per workgroup, and the
-// ranges of the workgroups of one XCD are adjacent (workgroup b runs on XCD b % 8), so each XCD
-// walks one stretch of the mesh. An item is kBlock / 64 tiles of 16 cells (kMapTile) or kBlock
-// cells. No counters: the launch is deterministic and can be replayed from a graph as is.
+// Gather rounds of a tile in the persistent kernel: four lanes per cell, at most kHybridMaxNodes
+// nodes.
+constexpr uint32_t kHybridGatherRounds = (kHybridMaxNodes + 3) / 4;
+
+// The node coordinates of the 16 cells of a tile into LDS, xs[(k * GD + g) * 16 + cell]; with
+// SHIFT nodes 1.. as offsets from node 0, as gather_cell takes them. Lane = cell + 16 x (node
+// group); every round loads (past the last node, node k - 1 again) before anything is stored,
+// so the loads overlap.
+template <typename T, int GD, bool SHIFT>
+__device__ __forceinline__ void gather_tile(const T* __restrict__ nodes,
+                                            const uint32_t* __restrict__ row, uint32_t k,
+                                            uint32_t last_node, uint32_t w, T* __restrict__ xs) {
+  const uint32_t cl16 = w & 15u, k0 = w >> 4;
+  uint32_t idx[kHybridGatherRounds];
+#pragma unroll
+  for (uint32_t i = 0; i < kHybridGatherRounds; ++i) {
+    const uint32_t kk = k0 + 4 * i;
+    const uint32_t n = gp(row)[kk < k ? kk : k - 1];  // unconditional: the loads go together
+    idx[i] = n < last_node ? n : last_node;
+  }
+  T x0[GD];
+  if constexpr (SHIFT) {
+    const uint32_t r0 = gp(row)[0];
+    const uint32_t n0 = r0 < last_node ? r0 : last_node;
+#pragma unroll
+    for (int g = 0; g < GD; ++g) x0[g] = gp(nodes)[static_cast<uint64_t>(n0) * GD + g];
+  }
+  T v[kHybridGatherRounds][GD];
+#pragma unroll
+  for (uint32_t i = 0; i < kHybridGatherRounds; ++i)
+#pragma unroll
+    for (int g = 0; g < GD; ++g) v[i][g] = gp(nodes)[static_cast<uint64_t>(idx[i]) * GD + g];
+#pragma unroll
+  for (uint32_t i = 0; i < kHybridGatherRounds; ++i) {
+    const uint32_t kk = k0 + 4 * i;
+    if (kk < k) {
+#pragma unroll
+      for (int g = 0; g < GD; ++g) {
+        T x = v[i][g];
+        if constexpr (SHIFT) {
+          if (kk > 0) x -= x0[g];
+        }
+        xs[(kk * GD + g) * 16 + cl16] = x;
+      }
+    }
+  }
+}
+
+// All requested factors of one tile of 16 cells of a block, by one wavefront, with the node count
+// k known only at run time: the cells' coordinates are gathered into LDS (gather_tile) and every
+// sum reads them there, against the block's table in LDS. Lanes = 16 cells x 4 points, and X, J
+// and det J leave through the staging buffer as direct_tile's do. The sums are those of
+// contract_fixed term for term (first product, then fused multiply-adds in ascending node order),
+// so the values equal the register-resident kernels' bit for bit; only the registers differ (a
+// few dozen, whatever k).
+template <typename T, int TD, int GD, int KC, bool SHIFT>
+__device__ __forceinline__ void hybrid_tile(const HybridBlock<T>* __restrict__ hb,
+                                            const T* __restrict__ ltab, uint64_t tile,
+                                            T* __restrict__ xs, T* __restrict__ stage) {
+  constexpr int JL = TD * GD;
+  constexpr uint32_t K0 = SHIFT ? 1u : 0u;
+  // The block's scalars and pointers, read once: the kernel's stores could otherwise make the
+  // compiler read them again after every store.
+  const FactorParams<T> p = hb->p;
+  const FactorOutputs o = hb->out;
+  const T* __restrict__ nodes = hb->nodes;
+  const double* __restrict__ tab64 = hb->tab64;
+  const T* __restrict__ ref_normals = hb->ref_normals;
+  const uint32_t* __restrict__ flags = hb->flags;
+  const uint32_t w = threadIdx.x & 63u, cl16 = w & 15u, qg = w >> 4;
+  const uint32_t K = KC > 0 ? static_cast<uint32_t>(KC) : p.k;  // KC: the node count, if fixed
+  const uint64_t cl = tile * 16 + cl16;
+  const uint64_t c = cl < p.num_cells ? cl : p.num_cells - 1;
+  const uint32_t* __restrict__ row = hb->dofmap + c * K;
+  gather_tile<T, GD, SHIFT>(nodes, row, K, p.last_node, w, xs);
+  wave_sync();
+  T x0[GD];
+#pragma unroll
+  for (int g = 0; g < GD; ++g) x0[g] = xs[g * 16 + cl16];
+  const uint32_t mask = p.mask;
+  const uint32_t ex = 0;
+  const uint32_t ej = ex + ((mask & kFPoints) ? 64u * GD : 0u);
+  const uint32_t ed = ej + ((mask & kFJac) ? 64u * JL : 0u);
+  const uint32_t own = mask & ~(kFPoints | kFJac | kFDetJ);
+  const bool need_j = (mask & ~kFPoints) != 0;
+  for (uint32_t q0 = 0; q0 < p.nq; q0 += 4) {
+    const uint32_t nv = p.nq - q0 < 4u ? p.nq - q0 : 4u;
+    const bool qlive = qg < nv;
+    const uint32_t q = qlive ? q0 + qg : p.nq - 1;
+    const T* __restrict__ t = ltab + static_cast<uint64_t>(q) * K * 4;
+    T ax[GD], aj[JL];
+    {
+      const TabRow<T> r = load_row<T>(t + 4 * K0);
+#pragma unroll
+      for (int g = 0; g < GD; ++g) {
+        const T xk = xs[(K0 * GD + g) * 16 + cl16];
+        ax[g] = xk * r.v[0];
+#pragma unroll
+        for (int d = 0; d < TD; ++d) aj[d * GD + g] = xk * r.v[1 + d];
+      }
+    }
+#pragma unroll 3
+    for (uint32_t k = K0 + 1; k < K; ++k) {
+      const TabRow<T> r = load_row<T>(t + 4 * k);
+#pragma unroll
+      for (int g = 0; g < GD; ++g) {
+        const T xk = xs[(k * GD + g) * 16 + cl16];
+        ax[g] = tfma(xk, r.v[0], ax[g]);
+#pragma unroll
+        for (int d = 0; d < TD; ++d) aj[d * GD + g] = tfma(xk, r.v[1 + d], aj[d * GD + g]);
+      }
+    }
+    if (mask & kFPoints) {
+#pragma unroll
+      for (int g = 0; g < GD; ++g) {
+        const T xv = SHIFT ? x0[g] + ax[g] : ax[g];
+        stage[ex + (qg * GD + g) * 16 + cl16] = xv + p.origin[g];
+      }
+    }
+    if (need_j) {
+      const T detj =
+          (mask & kFDetJ) ? detj_checked_k<T, TD, GD>(aj, p, nodes, row, K, tab64, q) : T(0);
+      if (mask & kFJac) {
+#pragma unroll
+        for (int i = 0; i < JL; ++i) stage[ej + (qg * JL + i) * 16 + cl16] = aj[i];
+      }
+      if (mask & kFDetJ) stage[ed + qg * 16 + cl16] = detj;
+      if (own && qlive)
+        store_from_j<T, TD, GD>(aj, detj, ref_normals, flags, c, q, own, tile * p.nq + q, 4u,
+                                cl16, o);
+    }
+    wave_sync();
+    if (mask & kFPoints) copy_rows<T, GD>(stage + ex, o.x, tile, p.nq, q0, nv, w);
+    if (mask & kFJac) copy_rows<T, JL>(stage + ej, o.j, tile, p.nq, q0, nv, w);
+    if (mask & kFDetJ) copy_rows<T, 1>(stage + ed, o.detj, tile, p.nq, q0, nv, w);
+    wave_sync();
+  }
+}
+
+// One persistent launch over the work items of every block of a mesh. An item is kHybridWaves
+// tiles of 16 cells of one block, one per wavefront (hybrid_tile); the workgroup's wavefronts
+// share the table in LDS. Items, in block order, are split on the
+// host into one contiguous range of about equal work per workgroup, and the ranges of the
+// workgroups of one XCD are adjacent (workgroup b runs on XCD b % 8), so each XCD walks one
+// stretch of the mesh. No counters: the launch is deterministic and can be replayed from a graph
+// as is. Dynamic LDS: the wavefronts' staging buffers, their coordinate buffers (xs_elems each)
+// and the table of the block in hand.
 template <typename T, bool SHIFT>
-__global__ void __launch_bounds__(kBlock, 2)
+__global__ void __launch_bounds__(kHybridWaves * 64)
     factor_hybrid(const HybridBlock<T>* __restrict__ blocks, const WorkItem* __restrict__ items,
-                  const uint32_t* __restrict__ ranges) {
+                  const uint32_t* __restrict__ ranges, uint32_t xs_elems) {
   extern __shared__ __align__(16) unsigned char tg_lds[];
-  T* stages = reinterpret_cast<T*>(tg_lds);
-  T* ltab = stages + (kBlock / 64) * kStageElems;  // the table of the kMapTile block in hand
+  T* lds = reinterpret_cast<T*>(tg_lds);
+  const uint32_t wv = threadIdx.x >> 6;
+  T* stage = lds + wv * kStageElems;
+  T* xs = lds + kHybridWaves * kStageElems + wv * xs_elems;
+  T* ltab = lds + kHybridWaves * (kStageElems + xs_elems);
   uint32_t table_block = 0xffffffffu;
   const uint32_t r = xcd_block(blockIdx.x, gridDim.x);
   const uint32_t begin = ranges[r], end = ranges[r + 1];
   for (uint32_t it = begin; it < end; ++it) {
     const WorkItem w = items[it];
     const HybridBlock<T>* __restrict__ hb = blocks + w.block;
-    if (hb->p.mapping == kMapTile && w.block != table_block) {
+    if (w.block != table_block) {
       // A new block's table: the workgroup is in step (items are uniform), so it reloads it.
       __syncthreads();
       table_to_lds<T>(hb->tab, ltab, hb->p.nq * hb->p.k * 4);
       __syncthreads();
       table_block = w.block;
     }
-    if (hb->p.mapping == kMapCellStaged) {
-      const uint64_t cl = static_cast<uint64_t>(w.first_cell) + threadIdx.x;
-      if (cl - (threadIdx.x & 63u) < hb->p.padded_cells) {
-        T* stage = stages + (threadIdx.x >> 6) * kStageElems;
-        switch (hb->shape) {
-#define TG_HCASE(S_, TD_, GD_, K_)                                                        \
-  case S_:                                                                                 \
-    direct_cell_staged<T, TD_, GD_, K_, SHIFT>(hb->nodes, hb->dofmap, hb->flags, hb->tab,  \
-                                               hb->tab64, hb->ref_normals, hb->p, hb->out, \
-                                               cl, stage);                                 \
+    const uint64_t tile = (static_cast<uint64_t>(w.first_cell) >> 4) + wv;
+    if (tile >= (hb->p.padded_cells >> 4)) continue;
+    // One body per (tdim, gdim), the node count read at run time: unrolling each shape's node
+    // loop measured slower (more registers, longer code) than the runtime loop.
+    uint32_t cls = 0;
+    switch (hb->shape) {
+#define TG_HCLASS(S_, TD_, GD_, K_) \
+  case S_:                          \
+    cls = TD_ * 4 + GD_;            \
     break;
-          TG_HYBRID_SHAPES(TG_HCASE)
-#undef TG_HCASE
-          default:
-            break;
-        }
-      }
-    } else if (hb->p.mapping == kMapTile) {
-      const uint64_t tile = (static_cast<uint64_t>(w.first_cell) >> 4) + (threadIdx.x >> 6);
-      if (tile < (hb->p.padded_cells >> 4)) {
-        T* stage = stages + (threadIdx.x >> 6) * kStageElems;
-        switch (hb->shape) {
-#define TG_HCASE(S_, TD_, GD_, K_)                                                      \
-  case S_:                                                                               \
-    direct_tile<T, TD_, GD_, K_, SHIFT>(hb->nodes, hb->dofmap, hb->flags, ltab, hb->tab64, \
-                                        hb->ref_normals, hb->p, hb->out, tile, stage);   \
-    break;
-          TG_HYBRID_SHAPES(TG_HCASE)
-#undef TG_HCASE
-          default:
-            break;
-        }
-      }
-    } else {
-      const uint64_t cl = static_cast<uint64_t>(w.first_cell) + threadIdx.x;
-      if (cl < hb->p.padded_cells) {
-        switch (hb->shape) {
-#define TG_HCASE(S_, TD_, GD_, K_)                                                           \
-  case S_:                                                                                    \
-    direct_cell<T, TD_, GD_, K_, SHIFT>(hb->nodes, hb->dofmap, hb->flags, hb->tab, hb->tab64, \
-                                        hb->ref_normals, hb->p, hb->out, cl);                 \
-    break;
-          TG_HYBRID_SHAPES(TG_HCASE)
-#undef TG_HCASE
-          default:
-            break;
-        }
-      }
+      TG_HYBRID_SHAPES(TG_HCLASS)
+#undef TG_HCLASS
+      default:
+        break;
     }
+    if (cls == 3 * 4 + 3)
+      hybrid_tile<T, 3, 3, 0, SHIFT>(hb, ltab, tile, xs, stage);
+    else if (cls == 2 * 4 + 3)
+      hybrid_tile<T, 2, 3, 0, SHIFT>(hb, ltab, tile, xs, stage);
+    else if (cls == 2 * 4 + 2)
+      hybrid_tile<T, 2, 2, 0, SHIFT>(hb, ltab, tile, xs, stage);
   }
 }
 
@@ -888,8 +1025,12 @@ hipError_t launch_measure_cells(int tdim, int gdim, bool shift, const T* nodes,
 }
 
 #ifdef TG_FACTOR_COMMON  // type-independent helpers: defined in one of the two objects
-size_t hybrid_lds_bytes(size_t elem, size_t max_table_bytes) {
-  return (kBlock / 64) * kStageElems * elem + max_table_bytes;
+uint32_t hybrid_xs_elems(uint32_t max_k, int gdim) {
+  return (max_k * static_cast<uint32_t>(gdim) * 16 + 3) / 4 * 4;
+}
+
+size_t hybrid_lds_bytes(size_t elem, uint32_t xs_elems, size_t max_table_bytes) {
+  return kHybridWaves * (kStageElems + xs_elems) * elem + max_table_bytes;
 }
 
 int hybrid_shape(int tdim, int gdim, uint32_t k) {
@@ -908,8 +1049,9 @@ static auto hybrid_fn(bool shift) {
 
 template <typename T>
 hipError_t hybrid_grid(bool shift, size_t lds, uint32_t* grid) {
-  // Resident workgroups per CU from the kernel's registers and LDS: 512 VGPRs per SIMD lane, one
-  // wavefront of each workgroup per SIMD, 160 KiB of LDS per CU, at most 8 wavefronts per SIMD.
+  // Resident workgroups per CU from the kernel's registers and LDS: 512 VGPRs per SIMD lane,
+  // kHybridWaves / 4 wavefronts of each workgroup per SIMD, 160 KiB of LDS per CU, at most 8
+  // wavefronts per SIMD.
   int dev = 0, cus = 0;
   hipFuncAttributes attr{};
   hipError_t e = hipGetDevice(&dev);
@@ -917,8 +1059,21 @@ hipError_t hybrid_grid(bool shift, size_t lds, uint32_t* grid) {
   if (e == hipSuccess)
     e = hipFuncGetAttributes(&attr, reinterpret_cast<const void*>(hybrid_fn<T>(shift)));
   if (e != hipSuccess) return e;
+  // The dynamic LDS must fit one workgroup; past the default limit the kernel opts in.
+  int lds_default = 0, lds_optin = 0;
+  e = hipDeviceGetAttribute(&lds_default, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+  if (e == hipSuccess)
+    e = hipDeviceGetAttribute(&lds_optin, hipDeviceAttributeSharedMemPerBlockOptin, dev);
+  if (e != hipSuccess) return e;
+  const size_t lds_limit = static_cast<size_t>(std::max(lds_default, lds_optin));
+  if (lds + attr.sharedSizeBytes > lds_limit) return hipErrorInvalidValue;
+  if (lds + attr.sharedSizeBytes > static_cast<size_t>(lds_default)) {
+    e = hipFuncSetAttribute(reinterpret_cast<const void*>(hybrid_fn<T>(shift)),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+    if (e != hipSuccess) return e;
+  }
   const int regs = std::max(8, (attr.numRegs + 7) / 8 * 8);
-  const int by_regs = std::min(8, 512 / regs);
+  const int by_regs = std::min(8, 512 / regs) / (kHybridWaves / 4);
   const size_t per_wg = lds + attr.sharedSizeBytes;
   const int by_lds = per_wg ? static_cast<int>((160 * 1024) / per_wg) : 8;
   const int per_cu = std::max(1, std::min(by_regs, by_lds));
@@ -928,9 +1083,11 @@ hipError_t hybrid_grid(bool shift, size_t lds, uint32_t* grid) {
 
 template <typename T>
 hipError_t launch_hybrid(bool shift, const HybridBlock<T>* blocks, const WorkItem* items,
-                         const uint32_t* ranges, uint32_t grid, size_t lds, hipStream_t stream) {
-  hipLaunchKernelGGL(hybrid_fn<T>(shift), dim3(grid), dim3(kBlock), static_cast<uint32_t>(lds),
-                     stream, blocks, items, ranges);
+                         const uint32_t* ranges, uint32_t grid, size_t lds, uint32_t xs_elems,
+                         hipStream_t stream) {
+  hipLaunchKernelGGL(hybrid_fn<T>(shift), dim3(grid), dim3(kHybridWaves * 64),
+                     static_cast<uint32_t>(lds),
+                     stream, blocks, items, ranges, xs_elems);
   return hipGetLastError();
 }
 
@@ -954,7 +1111,7 @@ hipError_t launch_evector(const T* nodes, const uint32_t* dofmap, uint64_t num_c
 using FT = TG_FACTOR_TYPE;
 template hipError_t hybrid_grid<FT>(bool, size_t, uint32_t*);
 template hipError_t launch_hybrid<FT>(bool, const HybridBlock<FT>*, const WorkItem*,
-                                      const uint32_t*, uint32_t, size_t, hipStream_t);
+                                      const uint32_t*, uint32_t, size_t, uint32_t, hipStream_t);
 template hipError_t launch_factor<FT>(int, int, bool, bool, const FT*, const uint32_t*,
                                       const uint32_t*, const FT*, const double*, const FT*,
                                       const FactorParams<FT>&, const FactorOutputs&, hipStream_t);

@@ -90,16 +90,25 @@ constexpr size_t kMaxLdsTableBytes = 32 * 1024;
 
 // Shape index of (tdim, gdim, k) in the persistent mixed-mesh kernel, or -1.
 int hybrid_shape(int tdim, int gdim, uint32_t k);
-// Dynamic LDS of the mixed-mesh kernel: staging buffers plus the largest kMapTile table.
-size_t hybrid_lds_bytes(size_t elem, size_t max_table_bytes);
+// Elements of one wavefront's coordinate buffer in the mixed-mesh kernel: 16 cells of up to
+// max_k nodes in gdim coordinates (a multiple of 4). At most kHybridMaxNodes nodes.
+uint32_t hybrid_xs_elems(uint32_t max_k, int gdim);
+constexpr uint32_t kHybridMaxNodes = 27;
+// Wavefronts per workgroup of the mixed-mesh kernel (a multiple of 4); an item is one tile of 16
+// cells per wavefront.
+constexpr uint32_t kHybridWaves = 8;
+// Dynamic LDS of the mixed-mesh kernel: staging and coordinate buffers plus the largest table.
+size_t hybrid_lds_bytes(size_t elem, uint32_t xs_elems, size_t max_table_bytes);
 // Workgroups that are resident on the device at once for the mixed-mesh kernel.
+// hipErrorInvalidValue: `lds` does not fit one workgroup on this device.
 template <typename T>
 hipError_t hybrid_grid(bool shift, size_t lds, uint32_t* grid);
 // Launches the mixed-mesh kernel: workgroup b's items are [ranges[r], ranges[r + 1]) with r the
 // XCD-contiguous index of b (grid + 1 entries).
 template <typename T>
 hipError_t launch_hybrid(bool shift, const HybridBlock<T>* blocks, const WorkItem* items,
-                         const uint32_t* ranges, uint32_t grid, size_t lds, hipStream_t stream);
+                         const uint32_t* ranges, uint32_t grid, size_t lds, uint32_t xs_elems,
+                         hipStream_t stream);
 
 // Launches the factor kernel for (T, tdim, gdim, k); `shift` selects the per-cell origin shift,
 // `generic` forces the runtime-K kernel. Returns hipSuccess or the launch error, and

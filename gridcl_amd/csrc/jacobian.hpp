@@ -1,5 +1,8 @@
 // det J, its pseudo-determinant and the pseudo-inverse K of one Jacobian J (column-major,
-// J(g, d) at d * GD + g; K at g * TD + t), shared by the factor and pull-back kernels.
+// J(g, d) at d * GD + g; K at g * TD + t), shared by the factor and pull-back kernels. Every
+// product is rounded before it is added (no contraction into fused multiply-adds), so the values
+// do not depend on the code around an inlined call: kernels of different shapes of code agree
+// bit for bit.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -13,6 +16,7 @@ __device__ __forceinline__ double jsqrt(double x) { return __builtin_sqrt(x); }
 // det J (tdim = gdim) or sqrt(det(J^T J)) (layout.rs:186-213).
 template <typename T, int TD, int GD>
 __device__ __forceinline__ T col_dot(const T* j, int a, int b) {
+#pragma clang fp contract(off)
   T acc = j[a * GD] * j[b * GD];
 #pragma unroll
   for (int g = 1; g < GD; ++g) acc += j[a * GD + g] * j[b * GD + g];
@@ -21,6 +25,7 @@ __device__ __forceinline__ T col_dot(const T* j, int a, int b) {
 
 template <typename T, int TD, int GD>
 __device__ __forceinline__ T det_of(const T* j) {
+#pragma clang fp contract(off)
   if constexpr (TD == GD) {
     if constexpr (TD == 1) {
       return j[0];
@@ -42,6 +47,7 @@ __device__ __forceinline__ T det_of(const T* j) {
 // K = (J^T J)^-1 J^T, column-major tdim x gdim (layout.rs:215-268).
 template <typename T, int TD, int GD>
 __device__ __forceinline__ void pinv_of(const T* j, T* k) {
+#pragma clang fp contract(off)
   if constexpr (TD == GD) {
     const T inv = T(1) / det_of<T, TD, GD>(j);
     if constexpr (TD == 1) {

@@ -30,6 +30,7 @@ struct tg_plan {
   uint32_t grid = 0;
   uint32_t num_items = 0;
   size_t lds = 0;  // dynamic LDS of the persistent kernel
+  uint32_t xs_elems = 0;  // its per-wavefront coordinate buffer
   std::vector<PlanBlock> blocks;
   void* d_blocks = nullptr;
   tg::WorkItem* d_items = nullptr;
@@ -105,23 +106,36 @@ int upload_persistent(tg_plan* plan) {
     h.p = params<T>(g, b, pb.pts, pb.mask, plan->tile, plan->options);
     h.shape = static_cast<uint32_t>(tg::hybrid_shape(b.tdim, g->gdim, b.k));
     hb.push_back(h);
-    cost.push_back(static_cast<double>(pb.pts->nq) * (b.k + 4.0) * (b.tdim + 1.0));
+    // Work of one item (one tile per wavefront, four points at a time): per group of four
+    // points a fixed part (det J, staging, the output) and one part per node; fitted to the
+    // kernel's times on single-block meshes (P2 tetrahedra, Q2 hexahedra, P2 prisms, P3
+    // triangles: 38 node-equivalents of fixed work per point group).
+    cost.push_back(static_cast<double>((pb.pts->nq + 3) / 4) * (38.0 + b.k));
   }
-  // Work items block by block; eight contiguous ranges of about equal cost, one per XCD.
+  // Work items (one tile of 16 cells per wavefront) block by block.
   std::vector<tg::WorkItem> items;
   std::vector<double> item_cost;
   for (size_t i = 0; i < hb.size(); ++i)
-    for (uint64_t c = 0; c < hb[i].p.padded_cells; c += tg::cells_per_workgroup(hb[i].p.mapping)) {
+    for (uint64_t c = 0; c < hb[i].p.padded_cells; c += 16 * tg::kHybridWaves) {
       items.push_back(tg::WorkItem{static_cast<uint32_t>(i), static_cast<uint32_t>(c)});
       item_cost.push_back(cost[i]);
     }
   plan->num_items = static_cast<uint32_t>(items.size());
   size_t max_table = 0;
-  for (const auto& h : hb)
-    if (h.p.mapping == tg::kMapTile)
-      max_table = std::max<size_t>(max_table, static_cast<size_t>(h.p.nq) * h.p.k * 4 * sizeof(T));
-  plan->lds = tg::hybrid_lds_bytes(sizeof(T), max_table);
+  uint32_t max_k = 1;
+  for (const auto& h : hb) {
+    max_table = std::max<size_t>(max_table, static_cast<size_t>(h.p.nq) * h.p.k * 4 * sizeof(T));
+    max_k = std::max(max_k, h.p.k);
+  }
+  plan->xs_elems = tg::hybrid_xs_elems(max_k, g->gdim);
+  plan->lds = tg::hybrid_lds_bytes(sizeof(T), plan->xs_elems, max_table);
   hipError_t e = tg::hybrid_grid<T>(plan->shift, plan->lds, &plan->grid);
+  if (e == hipErrorInvalidValue) {  // too much LDS for one workgroup: launch block by block
+    (void)hipGetLastError();
+    plan->persistent = false;
+    plan->num_items = 0;
+    return TG_OK;
+  }
   if (e != hipSuccess) return tg::hip_fail(e, "plan occupancy");
   plan->grid = std::max(1u, std::min(plan->grid, plan->num_items));
   // One contiguous range of about equal cost per workgroup, in item (block) order.
@@ -205,10 +219,12 @@ int plan_enqueue(tg_plan* plan, hipStream_t s, bool fork) {
       plan->g->dtype == TG_F32
           ? tg::launch_hybrid<float>(plan->shift,
                                      static_cast<const tg::HybridBlock<float>*>(plan->d_blocks),
-                                     plan->d_items, plan->d_ranges, plan->grid, plan->lds, s)
+                                     plan->d_items, plan->d_ranges, plan->grid, plan->lds,
+                                     plan->xs_elems, s)
           : tg::launch_hybrid<double>(plan->shift,
                                       static_cast<const tg::HybridBlock<double>*>(plan->d_blocks),
-                                      plan->d_items, plan->d_ranges, plan->grid, plan->lds, s);
+                                      plan->d_items, plan->d_ranges, plan->grid, plan->lds,
+                                      plan->xs_elems, s);
   return e == hipSuccess ? TG_OK : tg::hip_fail(e, "persistent plan launch");
 }
 
@@ -254,7 +270,10 @@ int tg_plan_create(const tg_geometry* g, tg_pointset* const* pts, uint32_t facto
     PlanBlock pb{static_cast<int>(bi), p, nullptr, mask, outs[bi]};
     int rc = tg::device_tables(p, b.degree, b.family, g->dtype, &pb.tabs);
     if (rc) return rc;
-    if (tg::hybrid_shape(b.tdim, g->gdim, b.k) < 0) all_hybrid = false;
+    // The persistent kernel: a shape of its list, tiles of 16, the table within its LDS budget.
+    if (tg::hybrid_shape(b.tdim, g->gdim, b.k) < 0 || tile != 16 ||
+        p->nq * b.k * 4 * (g->dtype == TG_F32 ? 4 : 8) > tg::kMaxLdsTableBytes)
+      all_hybrid = false;
     plan->blocks.push_back(pb);
   }
   plan->persistent =
