@@ -427,6 +427,59 @@ __device__ __forceinline__ void copy_rows(const T* __restrict__ stage, void* out
   }
 }
 
+// Gather rounds of a tile in the persistent kernel: four lanes per cell, at most kHybridMaxNodes
+// nodes.
+constexpr uint32_t kHybridGatherRounds = (kHybridMaxNodes + 3) / 4;
+
+// The node coordinates of the 16 cells of a tile into LDS, xs[(k * GD + g) * 16 + cell]; with
+// SHIFT nodes 1.. as offsets from node 0, as gather_cell takes them (nodes stored wider than T:
+// the offsets taken in the storage type and rounded once). Lane = cell + 16 x (node group), at
+// most 4 ROUNDS nodes per cell; every round loads (past the last node, node k - 1 again) before
+// anything is stored, so the loads overlap.
+template <typename T, int GD, bool SHIFT, uint32_t ROUNDS, typename TN = T>
+__device__ __forceinline__ void gather_tile(const TN* __restrict__ nodes,
+                                            const uint32_t* __restrict__ row, uint32_t k,
+                                            uint32_t last_node, uint32_t w, T* __restrict__ xs) {
+  const uint32_t cl16 = w & 15u, k0 = w >> 4;
+  uint32_t idx[ROUNDS];
+#pragma unroll
+  for (uint32_t i = 0; i < ROUNDS; ++i) {
+    const uint32_t kk = k0 + 4 * i;
+    const uint32_t n = gp(row)[kk < k ? kk : k - 1];  // unconditional: the loads go together
+    idx[i] = n < last_node ? n : last_node;
+  }
+  TN x0[GD];
+  if constexpr (SHIFT) {
+    const uint32_t r0 = gp(row)[0];
+    const uint32_t n0 = r0 < last_node ? r0 : last_node;
+#pragma unroll
+    for (int g = 0; g < GD; ++g) x0[g] = gp(nodes)[static_cast<uint64_t>(n0) * GD + g];
+  }
+  TN v[ROUNDS][GD];
+#pragma unroll
+  for (uint32_t i = 0; i < ROUNDS; ++i)
+#pragma unroll
+    for (int g = 0; g < GD; ++g) v[i][g] = gp(nodes)[static_cast<uint64_t>(idx[i]) * GD + g];
+#pragma unroll
+  for (uint32_t i = 0; i < ROUNDS; ++i) {
+    const uint32_t kk = k0 + 4 * i;
+    if (kk < k) {
+#pragma unroll
+      for (int g = 0; g < GD; ++g) {
+        if constexpr (sizeof(TN) != sizeof(T)) {
+          xs[(kk * GD + g) * 16 + cl16] = static_cast<T>(kk > 0 ? v[i][g] - x0[g] : v[i][g]);
+        } else {
+          T x = v[i][g];
+          if constexpr (SHIFT) {
+            if (kk > 0) x -= x0[g];
+          }
+          xs[(kk * GD + g) * 16 + cl16] = x;
+        }
+      }
+    }
+  }
+}
+
 // All requested factors of the 16 cells of one tile (tiles of 16 cells), by one wavefront: the
 // lanes are 16 cells x 4 point groups, and each step covers points q0..q0+3, whose rows of the
 // tile are contiguous in every field. X, J and det J go through LDS and leave as sequential
@@ -440,14 +493,26 @@ __device__ __forceinline__ void direct_tile(const TN* __restrict__ nodes,
                                             const double* __restrict__ tab64,
                                             const T* __restrict__ ref_normals,
                                             const FactorParams<T>& p, const FactorOutputs& o,
-                                            uint64_t tile, T* __restrict__ stage) {
+                                            uint64_t tile, T* __restrict__ stage,
+                                            T* __restrict__ xs = nullptr) {
   constexpr int JL = TD * GD;
   const uint32_t w = threadIdx.x & 63u, cl16 = w & 15u, qg = w >> 4;
   const uint64_t cl = tile * 16 + cl16;
   const uint64_t c = cl < p.num_cells ? cl : p.num_cells - 1;
   const uint32_t* __restrict__ row = dofmap + c * K;
   T x[K][GD];
-  gather_cell<T, GD, K, SHIFT, TN>(nodes, row, p.last_node, x);
+  if constexpr (sizeof(TN) != sizeof(T)) {
+    // Wide node storage: the four lanes of a cell share its gather through LDS (xs, K x GD x 16
+    // values) instead of each loading every node, which doubled the load traffic of FP64 nodes.
+    gather_tile<T, GD, SHIFT, (K + 3) / 4, TN>(nodes, row, K, p.last_node, w, xs);
+    wave_sync();
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+      for (int g = 0; g < GD; ++g) x[k][g] = xs[(k * GD + g) * 16 + cl16];
+  } else {
+    gather_cell<T, GD, K, SHIFT, TN>(nodes, row, p.last_node, x);
+  }
   const uint32_t mask = p.mask;
   const bool need_j = (mask & ~kFPoints) != 0;
   const uint32_t ex = 0;
@@ -515,8 +580,10 @@ __global__ void __launch_bounds__(kBlock)
     __syncthreads();
     const uint64_t tile = static_cast<uint64_t>(b) * (kBlock / 64) + (threadIdx.x >> 6);
     if (tile >= (p.padded_cells >> 4)) return;
+    // Wide node storage: each wavefront's gather buffer follows the table.
+    T* xs = ltab + p.nq * K * 4 + (threadIdx.x >> 6) * (K * GD * 16);
     direct_tile<T, TD, GD, K, SHIFT, TN>(nodes, dofmap, flags, ltab, tab64, ref_normals, p, o, tile,
-                                     stage + (threadIdx.x >> 6) * kStageElems);
+                                     stage + (threadIdx.x >> 6) * kStageElems, xs);
   } else if (p.mapping == kMapCellStaged) {
     const uint64_t cl = static_cast<uint64_t>(b) * kBlock + threadIdx.x;
     if (cl - (threadIdx.x & 63u) >= p.padded_cells) return;  // whole wavefronts stay
@@ -536,54 +603,6 @@ __global__ void __launch_bounds__(kBlock)
   X(6, 3, 3, 18) X(7, 3, 3, 5) X(8, 3, 3, 14) X(9, 2, 3, 3) X(10, 2, 3, 6) X(11, 2, 3, 10)    \
   X(12, 2, 3, 4) X(13, 2, 3, 9) X(14, 2, 2, 3) X(15, 2, 2, 6) X(16, 2, 2, 10) X(17, 2, 2, 4) \
   X(18, 2, 2, 9)
-
-// Gather rounds of a tile in the persistent kernel: four lanes per cell, at most kHybridMaxNodes
-// nodes.
-constexpr uint32_t kHybridGatherRounds = (kHybridMaxNodes + 3) / 4;
-
-// The node coordinates of the 16 cells of a tile into LDS, xs[(k * GD + g) * 16 + cell]; with
-// SHIFT nodes 1.. as offsets from node 0, as gather_cell takes them. Lane = cell + 16 x (node
-// group); every round loads (past the last node, node k - 1 again) before anything is stored,
-// so the loads overlap.
-template <typename T, int GD, bool SHIFT>
-__device__ __forceinline__ void gather_tile(const T* __restrict__ nodes,
-                                            const uint32_t* __restrict__ row, uint32_t k,
-                                            uint32_t last_node, uint32_t w, T* __restrict__ xs) {
-  const uint32_t cl16 = w & 15u, k0 = w >> 4;
-  uint32_t idx[kHybridGatherRounds];
-#pragma unroll
-  for (uint32_t i = 0; i < kHybridGatherRounds; ++i) {
-    const uint32_t kk = k0 + 4 * i;
-    const uint32_t n = gp(row)[kk < k ? kk : k - 1];  // unconditional: the loads go together
-    idx[i] = n < last_node ? n : last_node;
-  }
-  T x0[GD];
-  if constexpr (SHIFT) {
-    const uint32_t r0 = gp(row)[0];
-    const uint32_t n0 = r0 < last_node ? r0 : last_node;
-#pragma unroll
-    for (int g = 0; g < GD; ++g) x0[g] = gp(nodes)[static_cast<uint64_t>(n0) * GD + g];
-  }
-  T v[kHybridGatherRounds][GD];
-#pragma unroll
-  for (uint32_t i = 0; i < kHybridGatherRounds; ++i)
-#pragma unroll
-    for (int g = 0; g < GD; ++g) v[i][g] = gp(nodes)[static_cast<uint64_t>(idx[i]) * GD + g];
-#pragma unroll
-  for (uint32_t i = 0; i < kHybridGatherRounds; ++i) {
-    const uint32_t kk = k0 + 4 * i;
-    if (kk < k) {
-#pragma unroll
-      for (int g = 0; g < GD; ++g) {
-        T x = v[i][g];
-        if constexpr (SHIFT) {
-          if (kk > 0) x -= x0[g];
-        }
-        xs[(kk * GD + g) * 16 + cl16] = x;
-      }
-    }
-  }
-}
 
 // All requested factors of one tile of 16 cells of a block, by one wavefront, with the node count
 // k known only at run time: the cells' coordinates are gathered into LDS (gather_tile) and every
@@ -611,7 +630,7 @@ __device__ __forceinline__ void hybrid_tile(const HybridBlock<T>* __restrict__ h
   const uint64_t cl = tile * 16 + cl16;
   const uint64_t c = cl < p.num_cells ? cl : p.num_cells - 1;
   const uint32_t* __restrict__ row = hb->dofmap + c * K;
-  gather_tile<T, GD, SHIFT>(nodes, row, K, p.last_node, w, xs);
+  gather_tile<T, GD, SHIFT, kHybridGatherRounds>(nodes, row, K, p.last_node, w, xs);
   wave_sync();
   T x0[GD];
 #pragma unroll
@@ -970,7 +989,10 @@ hipError_t launch_factor_impl(int tdim, int gdim, bool shift, bool generic, cons
   const size_t table_bytes = static_cast<size_t>(p.nq) * p.k * 4 * sizeof(T);
   if (p.mapping == kMapTile && table_bytes > kMaxLdsTableBytes) p.mapping = kMapCellStaged;
   const size_t stage_bytes = (kBlock / 64) * kStageElems * sizeof(T);
-  const size_t lds = p.mapping == kMapTile         ? stage_bytes + table_bytes
+  const size_t gather_bytes =
+      sizeof(TN) != sizeof(T) ? (kBlock / 64) * static_cast<size_t>(p.k) * gdim * 16 * sizeof(T)
+                              : 0;
+  const size_t lds = p.mapping == kMapTile         ? stage_bytes + table_bytes + gather_bytes
                      : p.mapping == kMapCellStaged ? stage_bytes
                                                    : 0;
   const uint64_t per_block = cells_per_workgroup(p.mapping);
