@@ -554,6 +554,7 @@ def bench_q3(api, M, steps, warmup, n=96):
 
 
 HEADLINE_SOURCES = ("factor_kernels.hip",)
+EXTRAS = ("config1", "adjoint", "hybrid", "q3", "pullback", "measure", "equispaced", "gmsh")
 
 
 def main() -> int:
@@ -567,6 +568,9 @@ def main() -> int:
                     help="seconds of untimed back-to-back steps before the warm-up")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the config-1, Q3, mixed-mesh, adjoint and other legs")
+    ap.add_argument("--extras", default=",".join(EXTRAS),
+                    help="comma-separated extra legs to run (default: all of "
+                         f"{','.join(EXTRAS)})")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}): cell blocks would shard "
@@ -634,35 +638,48 @@ def main() -> int:
         del port
 
     extras = None
-    if not args.no_extras:
+    legs = set() if args.no_extras else {x for x in args.extras.split(",") if x}
+    unknown = legs - set(EXTRAS)
+    if unknown:
+        print(f"unknown extra legs: {sorted(unknown)}", file=sys.stderr)
+        return 2
+    if legs:
         del fields
         torch.cuda.empty_cache()
         extras = {}
+    if "config1" in legs:
         c1 = bench_config1(api, M, args.steps, args.warmup, not args.no_cpu)
         if cpu is not None:
             cpu["config1"] = c1.pop("cpu")
             cpu["config1"]["gpu_f64_vs_port"] = c1.pop("gpu_f64_vs_cpu_port")
         extras["config1_p1tet_32_f64"] = c1
         log(f"config 1: {c1}; cpu {cpu and cpu.get('config1')}")
+    if "adjoint" in legs:
         extras["adjoint_p2tet_32"] = bench_adjoint(api, M, M.config5_mesh(), args.steps,
                                                    args.warmup)
         log(f"adjoint 32: {extras['adjoint_p2tet_32']}")
         extras["adjoint_p2tet_64"] = bench_adjoint(api, M, mesh, args.steps, args.warmup)
         log(f"adjoint 64: {extras['adjoint_p2tet_64']}")
         torch.cuda.empty_cache()
+    if "hybrid" in legs:
         extras["hybrid"] = bench_hybrid(api, M, args.steps, args.warmup)
         log(f"hybrid: {extras['hybrid']}")
         torch.cuda.empty_cache()
+    if "q3" in legs:
         extras["q3_hex_96"] = bench_q3(api, M, max(5, args.steps // 5), 2)
         log(f"q3: {extras['q3_hex_96']}")
         torch.cuda.empty_cache()
+    if "pullback" in legs:
         extras["pull_back_p2tet_64"] = bench_pull_back(api, M, args.steps, args.warmup)
         log(f"pull-back: {extras['pull_back_p2tet_64']}")
+    if "measure" in legs:
         extras["measure_p2tet_48"] = bench_measure(api, M, args.steps, args.warmup)
         log(f"measure: {extras['measure_p2tet_48']}")
+    if "equispaced" in legs:
         extras["equispaced_q3hex_64"] = bench_equispaced(api, M, args.steps, args.warmup)
         log(f"equispaced: {extras['equispaced_q3hex_64']}")
         torch.cuda.empty_cache()
+    if "gmsh" in legs:
         extras["gmsh_read"] = bench_gmsh(api, M, mesh)
         log(f"gmsh: {extras['gmsh_read']}")
 
@@ -705,7 +722,7 @@ def main() -> int:
         "fp32_rel_err_vs_fp64_kernel": vs_fp64["f32"],
         "extras": extras,
     }
-    if extras:
+    if extras and "adjoint_p2tet_32" in extras:
         line["roofline_adjoint"] = {
             name: {dt: {"forward": extras[name][dt]["roofline_forward"]["frac"],
                         "vjp": extras[name][dt]["roofline_vjp"]["frac"]}
