@@ -61,6 +61,7 @@ OPT_SUMFACT_ALWAYS = 512
 OPT_POINTS_RELATIVE = 1024
 OPT_PLAN_PERSISTENT = 2048
 OPT_NO_MFMA = 4096
+OPT_COOPERATIVE = 8192
 
 
 # ---- Errors: one class per tetraloma_device::GridError variant of the path -------------------

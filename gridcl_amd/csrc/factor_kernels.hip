@@ -884,6 +884,113 @@ FactorFn<T, TN> direct_kernel(int td, int gd, uint32_t k, bool shift) {
   return nullptr;
 }
 
+// The `cooperative` strategy (tetraloma-geometry/src/cooperative/mod.rs:1-33, factors.rs:1-183):
+// each sum over a cell's k nodes is split among L lanes, L the largest power of two not above
+// min(32, k), so no lane is empty. Leaf l sums the nodes l, l + L, l + 2L, ... in ascending
+// order from the product of node l; the leaves are then added by the reference's pairwise tree
+// (at strides t = 1, 2, ..., leaf u with u mod 2t = 0 adds leaf u + t's partial, left + right).
+// A cell's L lanes are consecutive lanes of one wavefront, so the tree runs on cross-lane
+// shuffles, and the cell's first lane forms and stores every requested factor. FP32 works on
+// offsets from the cell's node 0 (node 0 itself then adds zero), X adds node 0 back. For
+// high-degree cells, whose nodes do not fit one lane's registers and which one lane per cell
+// would sum in a long series.
+constexpr uint32_t kCoopRounds = 8;  // nodes per leaf: at most 8 x 32 = 256 nodes per cell
+
+template <typename T, int TD, int GD, bool SHIFT>
+__global__ void __launch_bounds__(kBlock)
+    factor_coop(const T* __restrict__ nodes, const uint32_t* __restrict__ dofmap,
+                const uint32_t* __restrict__ flags, const T* __restrict__ tab,
+                const double* __restrict__ tab64, const T* __restrict__ ref_normals,
+                FactorParams<T> p, FactorOutputs o, uint32_t leaves) {
+  constexpr int JL = TD * GD;
+  const uint32_t L = leaves;  // a power of two, 1..32
+  const uint32_t leaf = threadIdx.x & (L - 1);
+  const uint32_t b = xcd_block(blockIdx.x, gridDim.x);
+  const uint64_t cl = static_cast<uint64_t>(b) * (kBlock / L) + threadIdx.x / L;
+  if (cl >= p.padded_cells) return;  // a cell's L lanes leave together
+  const uint64_t c = cl < p.num_cells ? cl : p.num_cells - 1;
+  const uint32_t K = p.k;
+  const uint32_t* __restrict__ row = dofmap + c * K;
+  T x0[GD];
+  {
+    const uint32_t n0 = row[0] < p.last_node ? row[0] : p.last_node;
+#pragma unroll
+    for (int g = 0; g < GD; ++g) x0[g] = nodes[static_cast<uint64_t>(n0) * GD + g];
+  }
+  // The leaf's nodes (past the last node, node k - 1 again: loaded, never summed).
+  T x[kCoopRounds][GD];
+#pragma unroll
+  for (uint32_t i = 0; i < kCoopRounds; ++i) {
+    const uint32_t k = leaf + i * L;
+    const uint32_t r = row[k < K ? k : K - 1];
+    const uint32_t n = r < p.last_node ? r : p.last_node;
+#pragma unroll
+    for (int g = 0; g < GD; ++g) {
+      const T v = nodes[static_cast<uint64_t>(n) * GD + g];
+      x[i][g] = SHIFT ? v - x0[g] : v;
+    }
+  }
+  const uint32_t tl = p.tile_log2;
+  const uint64_t tile = cl >> tl;
+  const uint32_t lane = static_cast<uint32_t>(cl) & ((1u << tl) - 1u);
+  const uint32_t mask = p.mask;
+  const bool need_j = (mask & ~kFPoints) != 0;
+  for (uint32_t q = 0; q < p.nq; ++q) {
+    const T* __restrict__ t = tab + static_cast<uint64_t>(q) * K * 4;
+    T ax[GD], aj[JL];
+    {
+      const TabRow<T> r = load_row<T>(t + 4 * leaf);
+#pragma unroll
+      for (int g = 0; g < GD; ++g) {
+        ax[g] = x[0][g] * r.v[0];
+#pragma unroll
+        for (int d = 0; d < TD; ++d) aj[d * GD + g] = x[0][g] * r.v[1 + d];
+      }
+    }
+#pragma unroll
+    for (uint32_t i = 1; i < kCoopRounds; ++i) {
+      const uint32_t k = leaf + i * L;
+      if (k < K) {
+        const TabRow<T> r = load_row<T>(t + 4 * k);
+#pragma unroll
+        for (int g = 0; g < GD; ++g) {
+          ax[g] = tfma(x[i][g], r.v[0], ax[g]);
+#pragma unroll
+          for (int d = 0; d < TD; ++d) aj[d * GD + g] = tfma(x[i][g], r.v[1 + d], aj[d * GD + g]);
+        }
+      }
+    }
+    for (uint32_t st = 1; st < L; st <<= 1) {
+      const bool left = (leaf & (2 * st - 1)) == 0;
+#pragma unroll
+      for (int g = 0; g < GD; ++g) {
+        const T right = __shfl_down(ax[g], st, L);
+        if (left) ax[g] = ax[g] + right;
+      }
+#pragma unroll
+      for (int i = 0; i < JL; ++i) {
+        const T right = __shfl_down(aj[i], st, L);
+        if (left) aj[i] = aj[i] + right;
+      }
+    }
+    if (leaf == 0) {
+      const uint64_t row_q = tile * p.nq + q;
+      if (mask & kFPoints) {
+#pragma unroll
+        for (int g = 0; g < GD; ++g) {
+          const T xv = SHIFT ? x0[g] + ax[g] : ax[g];
+          st<T>(o.x, (((row_q * GD) + g) << tl) | lane, xv + p.origin[g]);
+        }
+      }
+      if (need_j) {
+        const T detj =
+            (mask & kFDetJ) ? detj_checked_k<T, TD, GD>(aj, p, nodes, row, K, tab64, q) : T(0);
+        store_from_j<T, TD, GD>(aj, detj, ref_normals, flags, c, q, mask, row_q, tl, lane, o);
+      }
+    }
+  }
+}
+
 // A product rounded before it is added to the running sum, as the measure's cell sums read
 // (measure.rs:17-41): the first point starts the sum.
 template <typename T>
@@ -1007,6 +1114,32 @@ hipError_t launch_factor_impl(int tdim, int gdim, bool shift, bool generic, cons
 }  // namespace
 
 #ifndef TG_FACTOR_NODES64
+template <typename T>
+hipError_t launch_factor_coop(int tdim, int gdim, bool shift, const T* nodes,
+                              const uint32_t* dofmap, const uint32_t* flags, const T* tab,
+                              const double* tab64, const T* ref_normals, const FactorParams<T>& p,
+                              const FactorOutputs& out, hipStream_t stream) {
+  using Fn = void (*)(const T*, const uint32_t*, const uint32_t*, const T*, const double*,
+                      const T*, FactorParams<T>, FactorOutputs, uint32_t);
+  Fn fn = nullptr;
+#define TG_C(TD_, GD_)                                                                \
+  if (tdim == TD_ && gdim == GD_)                                                     \
+    fn = shift ? static_cast<Fn>(factor_coop<T, TD_, GD_, true>)                      \
+               : static_cast<Fn>(factor_coop<T, TD_, GD_, false>);
+  TG_C(1, 1) TG_C(1, 2) TG_C(1, 3) TG_C(2, 2) TG_C(2, 3) TG_C(3, 3)
+#undef TG_C
+  if (fn == nullptr || p.k == 0) return hipErrorNotSupported;
+  uint32_t leaves = 1;
+  while (leaves * 2 <= std::min<uint32_t>(32u, p.k)) leaves *= 2;
+  if (p.k > leaves * kCoopRounds) return hipErrorNotSupported;
+  const uint64_t per_block = kBlock / leaves;
+  const uint64_t blocks = (p.padded_cells + per_block - 1) / per_block;
+  if (blocks > 0x7fffffffu) return hipErrorInvalidConfiguration;
+  hipLaunchKernelGGL(fn, dim3(static_cast<uint32_t>(blocks)), dim3(kBlock), 0, stream, nodes,
+                     dofmap, flags, tab, tab64, ref_normals, p, out, leaves);
+  return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_factor(int tdim, int gdim, bool shift, bool generic, const T* nodes,
                          const uint32_t* dofmap, const uint32_t* flags, const T* tab,
@@ -1137,6 +1270,10 @@ template hipError_t launch_hybrid<FT>(bool, const HybridBlock<FT>*, const WorkIt
 template hipError_t launch_factor<FT>(int, int, bool, bool, const FT*, const uint32_t*,
                                       const uint32_t*, const FT*, const double*, const FT*,
                                       const FactorParams<FT>&, const FactorOutputs&, hipStream_t);
+template hipError_t launch_factor_coop<FT>(int, int, bool, const FT*, const uint32_t*,
+                                           const uint32_t*, const FT*, const double*, const FT*,
+                                           const FactorParams<FT>&, const FactorOutputs&,
+                                           hipStream_t);
 template hipError_t launch_evector<FT>(const FT*, const uint32_t*, uint64_t, uint64_t, uint32_t,
                                        int, uint32_t, FT*, hipStream_t);
 template bool has_measure_cells<FT>(int, int, uint32_t, bool);

@@ -159,6 +159,14 @@ hipError_t launch_sumfact_f64_nodes(int n1, int m, const double* nodes, const ui
                                     const FactorParams<float>& p, const FactorOutputs& out,
                                     hipStream_t stream);
 
+// The cooperative strategy (k up to 256 nodes): L lanes per cell, L the largest power of two not
+// above min(32, k), leaves added by a pairwise tree. hipErrorNotSupported for other shapes.
+template <typename T>
+hipError_t launch_factor_coop(int tdim, int gdim, bool shift, const T* nodes,
+                              const uint32_t* dofmap, const uint32_t* flags, const T* tab,
+                              const double* tab64, const T* ref_normals, const FactorParams<T>& p,
+                              const FactorOutputs& out, hipStream_t stream);
+
 template <typename T>
 hipError_t launch_evector(const T* nodes, const uint32_t* dofmap, uint64_t num_cells,
                           uint64_t padded, uint32_t k, int gdim, uint32_t last_node, T* out,

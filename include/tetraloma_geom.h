@@ -126,7 +126,13 @@ enum {
    * the matrix cores (v_mfma_f32_16x16x4f32), which run them by default where they measure
    * faster (degree 2 at 3^3 points, degree 3 at 4^3 points) and, under TG_OPT_SUMFACT_ALWAYS,
    * wherever they apply (2-4 points per direction). */
-  TG_OPT_NO_MFMA = 1u << 12
+  TG_OPT_NO_MFMA = 1u << 12,
+  /* The cooperative strategy (tetraloma-geometry/src/cooperative/factors.rs:100-183) for any
+   * shape: each cell's node sums split among L lanes (L the largest power of two not above
+   * min(32, k)), leaf l summing nodes l, l + L, ... from the product of node l, the leaves added
+   * by a pairwise tree. It runs by default for cells of at least 56 nodes in FP32 and 84 in FP64
+   * (tetrahedra from degree 5 and 6), where it measured faster than the runtime-K kernel. */
+  TG_OPT_COOPERATIVE = 1u << 13
 };
 
 typedef struct tg_geometry tg_geometry;
