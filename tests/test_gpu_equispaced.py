@@ -132,3 +132,61 @@ def test_equispaced_second_call_after_nodes_move(cuda, dtype):
         second = api.equispaced_nodes(g)
     side.synchronize()
     assert torch.equal(second, 2.0 * first)
+
+
+def lattice_x(g, b, cell, p):
+    """X (relative to the origin) of block b at its cells' equispaced lattice, [c][k][gdim] in A.5
+    order, through the lexicographic point set the conversion uses."""
+    import torch
+
+    a5 = M.reference_nodes(cell, p, EQ)
+    lex = np.rint(a5 * p).astype(int) @ (p + 1) ** np.arange(a5.shape[1])
+    pts = np.empty_like(a5)
+    pts[lex] = a5
+    lat = api.PointSet.custom(cell, pts, np.ones(len(pts)))
+    f = api.geometric_factors_block(g, b, lat, ["points"], options=api.OPT_POINTS_RELATIVE)
+    torch.cuda.synchronize()
+    return f["points"].to_host_cell_major()[:, lex]
+
+
+@pytest.mark.parametrize("cell,p,gdim", [(QUAD, 3, 2), (HEX, 3, 3)])
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_equispaced_two_gll_blocks_sharing_nodes(cuda, cell, p, gdim, dtype):
+    """One GLL box cut into two blocks that share the nodes of the cut, the second half of the
+    cells first: each shared node takes its value from its first occurrence in (block, cell,
+    local node) order, i.e. from block 0, through the cached table's per-block start offsets.
+    Bitwise against that rule repeated on the host from each block's X at the lattice, then again
+    on another stream after the nodes moved (doubled: the conversion is linear, so bit for bit
+    twice the first result)."""
+    import torch
+
+    tdt = torch.float64 if dtype == "f64" else torch.float32
+    box = gll_box(cell, 4 if cell == QUAD else 3, p, gdim)
+    cells = box.blocks[0].cells
+    half = len(cells) // 2
+    blocks = [M.HostBlock(cell, p, np.ascontiguousarray(cells[half:]), family=GLL),
+              M.HostBlock(cell, p, np.ascontiguousarray(cells[:half]), family=GLL)]
+    mesh = M.HostMesh(gdim, box.nodes, blocks)
+    g = api.geometry_from_host(mesh, dtype=tdt)
+    out = api.equispaced_nodes(g)
+    torch.cuda.synchronize()
+    want = g.nodes.cpu().numpy().copy()
+    seen = set()
+    shared = set(np.unique(blocks[0].cells)) & set(np.unique(blocks[1].cells))
+    assert shared, "the two blocks share the nodes of the cut"
+    for b, blk in enumerate(mesh.blocks):
+        xl = lattice_x(g, b, cell, p)
+        for c, row in enumerate(blk.cells):
+            for kk, node in enumerate(row):
+                if node not in seen:
+                    seen.add(node)
+                    want[node] = xl[c, kk]
+    assert np.array_equal(out.cpu().numpy(), want)
+    first = out.clone()
+    g.nodes.mul_(2.0)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        second = api.equispaced_nodes(g)
+    side.synchronize()
+    assert torch.equal(second, 2.0 * first)
