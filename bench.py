@@ -502,8 +502,9 @@ def bench_equispaced(api, M, steps, warmup, n=64):
 def bench_q3(api, M, steps, warmup, n=96):
     """Q3 hexahedra (96^3 cells, 64 nodes and 64 Gauss points per cell), X + J + det J: the FP32
     sum-factorised kernel with its second and third sums on the matrix cores (the default), the
-    same on the vector ALU, FP32 on FP64-stored nodes (sums in FP64), the dense register kernel,
-    and the FP64 kernel; each FP32 result's error against the FP64 kernel over the whole mesh."""
+    same on the vector ALU, FP32 on FP64-stored nodes (matrix cores with FP64 first sums split
+    hi/lo, and the vector ALU with every sum in FP64), the dense register kernel, and the FP64
+    kernel; each FP32 result's error against the FP64 kernel over the whole mesh."""
     import torch
 
     h = M.config3_mesh(n=n)
@@ -540,15 +541,19 @@ def bench_q3(api, M, steps, warmup, n=96):
         g, 0, ps, names, out=out, options=api.OPT_NO_SUMFACT), max(2, steps // 5), 1)
     g.set_nodes_f64(torch.tensor(h.nodes - np.asarray(g.origin), dtype=torch.float64,
                                  device="cuda"))
-    step = lambda: api.geometric_factors_block(g, 0, ps, names, out=out)
-    ms = time_steps(step, steps, warmup)
-    torch.cuda.synchronize()
     nb64 = nbytes + 3 * 4 * N
-    rec["fp32_nodes64"] = {"ms": ms, "evals_per_s": C * 64 / (ms * 1e-3),
-                           "achieved_gbs": nb64 / (ms * 1e-3) / 1e9,
-                           "frac": nb64 / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
-                           "rel_err_vs_fp64_kernel": {nm: device_rel_err(out[nm], want[nm])
-                                                      for nm in names}}
+    # FP64 nodes: the matrix-core kernel (FP64 first sums, hi/lo products; the default) and the
+    # vector-ALU kernel with every sum in FP64.
+    for key, opt in (("fp32_nodes64", 0), ("fp32_nodes64_valu", api.OPT_NO_MFMA)):
+        step = lambda opt=opt: api.geometric_factors_block(g, 0, ps, names, out=out, options=opt)
+        spin_up(step, 0.3)
+        ms = time_steps(step, steps, warmup)
+        torch.cuda.synchronize()
+        rec[key] = {"ms": ms, "evals_per_s": C * 64 / (ms * 1e-3),
+                    "achieved_gbs": nb64 / (ms * 1e-3) / 1e9,
+                    "frac": nb64 / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS,
+                    "rel_err_vs_fp64_kernel": {nm: device_rel_err(out[nm], want[nm])
+                                               for nm in names}}
     rec["fp64"] = {"ms": ms64, "evals_per_s": C * 64 / (ms64 * 1e-3)}
     return rec
 

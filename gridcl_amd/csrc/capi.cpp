@@ -364,6 +364,16 @@ int enqueue_block(const tg_geometry* g, const Block& b, tg_pointset* pts, Device
   const double* n64 = shift ? g->nodes64 : nullptr;
   if (takes_sumfact(g, tabs, factors, tile, options, f32)) {
     if constexpr (sizeof(T) == 4) {
+      const bool mfma = tabs->sf_mfma && !(options & TG_OPT_NO_MFMA) &&
+                        ((options & TG_OPT_SUMFACT_ALWAYS) || mfma_pays(tabs->sf_n1, tabs->sf_m));
+      if (n64 && mfma) {
+        const size_t k = static_cast<size_t>(b.k);
+        hipError_t e = tg::launch_sumfact_mfma_f64_nodes(
+            tabs->sf_n1, tabs->sf_m, n64, b.dofmap, tabs->sf_tab64, tabs->sf_lex + k,
+            static_cast<const float*>(tabs->sf_mfma), tabs->tab64, p, o, stream);
+        if (e == hipSuccess) return TG_OK;
+        if (e != hipErrorNotSupported) return hip_fail(e, "sum-factorised matrix-core launch");
+      }
       if (n64) {
         hipError_t e = tg::launch_sumfact_f64_nodes(
             tabs->sf_n1, tabs->sf_m, n64, b.dofmap, tabs->sf_tab64,
@@ -371,8 +381,7 @@ int enqueue_block(const tg_geometry* g, const Block& b, tg_pointset* pts, Device
         if (e == hipSuccess) return TG_OK;
         if (e != hipErrorNotSupported) return hip_fail(e, "sum-factorised kernel launch");
       }
-      if (!n64 && tabs->sf_mfma && !(options & TG_OPT_NO_MFMA) &&
-          ((options & TG_OPT_SUMFACT_ALWAYS) || mfma_pays(tabs->sf_n1, tabs->sf_m))) {
+      if (!n64 && mfma) {
         const size_t k = static_cast<size_t>(b.k);
         hipError_t e = tg::launch_sumfact_mfma(
             tabs->sf_n1, tabs->sf_m, shift, static_cast<const float*>(g->nodes), b.dofmap,

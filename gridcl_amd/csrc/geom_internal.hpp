@@ -151,6 +151,15 @@ hipError_t launch_sumfact_mfma(int n1, int m, bool shift, const float* nodes,
                                const FactorParams<float>& p, const FactorOutputs& out,
                                hipStream_t stream);
 
+// The matrix-core kernel on the FP64 nodes of an FP32 geometry: vertex mean, offsets and first
+// sums in FP64 (tab1d the FP64 1D table), the second and third sums as products of the first
+// sums' hi and lo FP32 parts. hipErrorNotSupported where launch_sumfact_mfma is.
+hipError_t launch_sumfact_mfma_f64_nodes(int n1, int m, const double* nodes,
+                                         const uint32_t* dofmap, const double* tab1d,
+                                         const uint32_t* alex, const float* tab2d,
+                                         const double* tab64, const FactorParams<float>& p,
+                                         const FactorOutputs& out, hipStream_t stream);
+
 // Sum-factorised hexahedra of an FP32 geometry on its FP64 nodes: the sums in FP64 (the
 // arithmetic of the FP64 kernel) and X, J and det J rounded once to FP32 at their stores; tab1d
 // is the FP64 1D table.

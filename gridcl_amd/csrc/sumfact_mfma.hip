@@ -34,9 +34,16 @@
 //
 // FP32: the coordinates are shifted to the mean of each cell's eight vertices before the sums
 // (summed in vertex order, as sumfact_kernels.hip), and X adds it back.
+//
+// FP64-stored nodes of an FP32 geometry (TN = double): the gathered coordinates stay in FP64 in
+// LDS; the vertex mean, the offsets from it and the first sums are taken in FP64, and each first
+// sum s enters the matrix cores as a hi/lo pair, hi = fl32(s) and lo = fl32(s - hi): X takes the
+// hi products, J the hi and then the lo products into the same FP32 accumulator. What remains in
+// FP32 is the 2D table rounding and the accumulation, about 3e-7 of |J| on curved Q3 hexahedra.
 #include <hip/hip_runtime.h>
 
 #include <cstdlib>
+#include <type_traits>
 
 #include "geom_internal.hpp"
 #include "sumfact_tile.hpp"
@@ -75,36 +82,42 @@ __device__ __forceinline__ void flush(const float* __restrict__ src, float* __re
   }
 }
 
-template <int N1, int M>
+// LDS of one workgroup: the A operands, the four wavefronts' output stages (floats), then the
+// tile's nodes (TN). FP64 nodes: 80,896 bytes for Q3, two workgroups per CU as for FP32.
+template <int N1, int M, typename TN>
 struct StreamLds {
   static constexpr int kTab = 3 * N1 * 64;         // the A operands [kind][b][lane]
   static constexpr int kX = 3 * N1 * N1 * 64;      // the tile's nodes [g][b][c][lane]
   static constexpr int kRun = M * M * 16;          // one component over the points of one r
   static constexpr int kStage = 13 * kRun;         // per wavefront: its X, J and det J runs
-  static constexpr int kTotal = kTab + kX + kWaves * kStage;
-  static_assert(kTab % 4 == 0 && kX % 4 == 0 && kRun % 4 == 0, "16-byte aligned stages");
+  static constexpr int kFloats = kTab + kWaves * kStage;
+  static constexpr int kBytes = kFloats * 4 + kX * static_cast<int>(sizeof(TN));
+  static_assert(kTab % 4 == 0 && kRun % 4 == 0, "16-byte aligned stages");
 };
 
 // FULL: the factors are X, J and det J (fixed at compile time, so that the loop is straight-line
 // code and the waits on the prefetches count the stores exactly); otherwise any subset of them,
-// from p.mask.
-template <int N1, int M, bool SHIFT, bool FULL>
+// from p.mask. TN: the stored node type (double: FP64 nodes, always shifted); tab1d is of TN.
+template <int N1, int M, bool SHIFT, bool FULL, typename TN>
 __global__ void __launch_bounds__(64 * kWaves)
-    factor_sumfact_mfma(const float* __restrict__ nodes, const uint32_t* __restrict__ dofmap,
-                        const float* __restrict__ tab1d, const uint32_t* __restrict__ alex,
+    factor_sumfact_mfma(const TN* __restrict__ nodes, const uint32_t* __restrict__ dofmap,
+                        const TN* __restrict__ tab1d, const uint32_t* __restrict__ alex,
                         const float* __restrict__ tab2d, const double* __restrict__ tab64,
                         FactorParams<float> p, FactorOutputs o) {
   constexpr int K = N1 * N1 * N1;
   constexpr int NQ = M * M * M;
-  using L = StreamLds<N1, M>;
+  constexpr bool N64 = std::is_same<TN, double>::value;
+  static_assert(!N64 || SHIFT, "FP64 nodes are always shifted");
+  using L = StreamLds<N1, M, TN>;
   constexpr int kRun = L::kRun;
   static_assert(N1 >= 2 && N1 <= 4 && M >= 1 && M <= 4, "one 16x16x4 product per (b, kind)");
-  __shared__ __align__(16) float lds[L::kTotal];
-  float* __restrict__ xs = lds + L::kTab;
+  __shared__ __align__(16) unsigned char smem[L::kBytes];
+  float* __restrict__ lds = reinterpret_cast<float*>(smem);
+  TN* __restrict__ xs = reinterpret_cast<TN*>(smem + L::kFloats * 4);
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) & (kWaves - 1);
   // The wavefront's stage: X [M^2 points][3][16], then J [M^2][9][16], then det J [M^2][16].
-  float* __restrict__ stage = lds + L::kTab + L::kX + w * L::kStage;
+  float* __restrict__ stage = lds + L::kTab + w * L::kStage;
   float* __restrict__ sx = stage;
   float* __restrict__ sj = stage + 3 * kRun;
   float* __restrict__ sd = stage + 12 * kRun;
@@ -138,7 +151,7 @@ __global__ void __launch_bounds__(64 * kWaves)
     for (int cc = 0; cc < N1; ++cc) idx[cc] = row[map[cc]];
   };
   struct P3 {
-    float v[3];
+    TN v[3];
   };
   auto load_xyz = [&](const uint32_t (&idx)[N1], P3 (&v)[N1]) {
 #pragma unroll
@@ -163,70 +176,108 @@ __global__ void __launch_bounds__(64 * kWaves)
       for (int g = 0; g < 3; ++g)
 #pragma unroll
         for (int cc = 0; cc < N1; ++cc)
-          xs[((g * N1 + w) * N1 + cc) * 64 + lane] = a_live ? v[cc].v[g] : 0.f;
+          xs[((g * N1 + w) * N1 + cc) * 64 + lane] = a_live ? v[cc].v[g] : TN(0);
     }
   };
   // X, J and det J of tile t at the points with r = w, from xs into the wavefront's stage.
   auto compute = [&](uint64_t t) {
-    float x[3][N1][N1];
-#pragma unroll
-    for (int g = 0; g < 3; ++g)
-#pragma unroll
-      for (int b = 0; b < N1; ++b)
-#pragma unroll
-        for (int cc = 0; cc < N1; ++cc) x[g][b][cc] = xs[((g * N1 + b) * N1 + cc) * 64 + lane];
-    // The cell's vertex mean: vertex k is node (a, b, c) = (k & 1, k >> 1 & 1, k >> 2) (n1 - 1),
-    // at lane ((k & 1) (n1 - 1)) 16 + cell of xs; summed from vertex 0 in order.
-    float sg[3] = {0.f, 0.f, 0.f};
-    if constexpr (SHIFT) {
-      constexpr int E = N1 - 1;
-#pragma unroll
-      for (int g = 0; g < 3; ++g) {
-        float acc = 0.f;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-          const float vk = xs[((g * N1 + ((k >> 1) & 1) * E) * N1 + ((k >> 2) & 1) * E) * 64 +
-                              (k & 1) * E * 16 + cell];
-          acc = k == 0 ? vk : acc + vk;
-        }
-        sg[g] = acc * 0.125f;
-#pragma unroll
-        for (int b = 0; b < N1; ++b)
-#pragma unroll
-          for (int cc = 0; cc < N1; ++cc) x[g][b][cc] = a_live ? x[g][b][cc] - sg[g] : 0.f;
-      }
-    }
-    // First sums (along zeta) for r, from the c = 0 product in ascending c, then the products.
     const v4f zero = {0.f, 0.f, 0.f, 0.f};
     v4f ax[3], jac[3][3];  // jac[d][g] = J(g, d) at the lane's points (p = register index)
-#pragma unroll
-    for (int g = 0; g < 3; ++g) {
-      float s0[N1], s1[N1];
-#pragma unroll
-      for (int b = 0; b < N1; ++b) {
-        float u0 = 0.f, u1 = 0.f;
-#pragma unroll
-        for (int cc = 0; cc < N1; ++cc) {
-          u0 = __builtin_fmaf(x[g][b][cc], tab1d[r * N1 + cc], u0);
-          u1 = __builtin_fmaf(x[g][b][cc], tab1d[(M + r) * N1 + cc], u1);
-        }
-        s0[b] = u0;
-        s1[b] = u1;
-      }
+    // The products of coordinate g from its first sums: X from hi, J from hi, then from lo.
+    auto products = [&](int g, const float (&h0)[N1], const float (&h1)[N1],
+                        const float (&l0)[N1], const float (&l1)[N1]) {
       v4f px = zero, j0 = zero, j1 = zero, j2 = zero;
 #pragma unroll
       for (int b = 0; b < N1; ++b) {
         const float t_val = lds[b * 64 + lane], t_xi = lds[(N1 + b) * 64 + lane],
                     t_eta = lds[(2 * N1 + b) * 64 + lane];
-        if (mask & kFPoints) px = mfma4(t_val, s0[b], px);
-        j0 = mfma4(t_xi, s0[b], j0);
-        j1 = mfma4(t_eta, s0[b], j1);
-        j2 = mfma4(t_val, s1[b], j2);
+        if (mask & kFPoints) px = mfma4(t_val, h0[b], px);
+        j0 = mfma4(t_xi, h0[b], j0);
+        j1 = mfma4(t_eta, h0[b], j1);
+        j2 = mfma4(t_val, h1[b], j2);
+        if constexpr (N64) {
+          j0 = mfma4(t_xi, l0[b], j0);
+          j1 = mfma4(t_eta, l0[b], j1);
+          j2 = mfma4(t_val, l1[b], j2);
+        }
       }
       ax[g] = px;
       jac[0][g] = j0;
       jac[1][g] = j1;
       jac[2][g] = j2;
+    };
+    // The cell's vertex mean: vertex k is node (a, b, c) = (k & 1, k >> 1 & 1, k >> 2) (n1 - 1),
+    // at lane ((k & 1) (n1 - 1)) 16 + cell of xs; summed from vertex 0 in order.
+    auto vertex_mean = [&](int g) {
+      constexpr int E = N1 - 1;
+      TN acc = TN(0);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const TN vk = xs[((g * N1 + ((k >> 1) & 1) * E) * N1 + ((k >> 2) & 1) * E) * 64 +
+                         (k & 1) * E * 16 + cell];
+        acc = k == 0 ? vk : acc + vk;
+      }
+      return acc * TN(0.125);
+    };
+    TN sg[3] = {TN(0), TN(0), TN(0)};
+    if constexpr (!N64) {
+      float x[3][N1][N1];
+#pragma unroll
+      for (int g = 0; g < 3; ++g)
+#pragma unroll
+        for (int b = 0; b < N1; ++b)
+#pragma unroll
+          for (int cc = 0; cc < N1; ++cc) x[g][b][cc] = xs[((g * N1 + b) * N1 + cc) * 64 + lane];
+      if constexpr (SHIFT) {
+#pragma unroll
+        for (int g = 0; g < 3; ++g) {
+          sg[g] = vertex_mean(g);
+#pragma unroll
+          for (int b = 0; b < N1; ++b)
+#pragma unroll
+            for (int cc = 0; cc < N1; ++cc) x[g][b][cc] = a_live ? x[g][b][cc] - sg[g] : 0.f;
+        }
+      }
+      // First sums (along zeta) for r, from the c = 0 product in ascending c, then the products.
+#pragma unroll
+      for (int g = 0; g < 3; ++g) {
+        float s0[N1], s1[N1];
+#pragma unroll
+        for (int b = 0; b < N1; ++b) {
+          float u0 = 0.f, u1 = 0.f;
+#pragma unroll
+          for (int cc = 0; cc < N1; ++cc) {
+            u0 = __builtin_fmaf(x[g][b][cc], tab1d[r * N1 + cc], u0);
+            u1 = __builtin_fmaf(x[g][b][cc], tab1d[(M + r) * N1 + cc], u1);
+          }
+          s0[b] = u0;
+          s1[b] = u1;
+        }
+        products(g, s0, s1, s0, s1);
+      }
+    } else {
+      // One coordinate at a time (16 FP64 values live): mean, offsets and first sums in FP64,
+      // each sum split into hi and lo.
+#pragma unroll
+      for (int g = 0; g < 3; ++g) {
+        sg[g] = vertex_mean(g);
+        float h0[N1], h1[N1], l0[N1], l1[N1];
+#pragma unroll
+        for (int b = 0; b < N1; ++b) {
+          double u0 = 0.0, u1 = 0.0;
+#pragma unroll
+          for (int cc = 0; cc < N1; ++cc) {
+            const double d = a_live ? xs[((g * N1 + b) * N1 + cc) * 64 + lane] - sg[g] : 0.0;
+            u0 = __builtin_fma(d, tab1d[r * N1 + cc], u0);
+            u1 = __builtin_fma(d, tab1d[(M + r) * N1 + cc], u1);
+          }
+          h0[b] = static_cast<float>(u0);
+          h1[b] = static_cast<float>(u1);
+          l0[b] = static_cast<float>(u0 - static_cast<double>(h0[b]));
+          l1[b] = static_cast<float>(u1 - static_cast<double>(h1[b]));
+        }
+        products(g, h0, h1, l0, l1);
+      }
     }
     float det[M];
     uint32_t near = 0;  // bit p: point (p, q, r) needs the FP64 det J
@@ -252,7 +303,9 @@ __global__ void __launch_bounds__(64 * kWaves)
       for (int pp = 0; pp < M; ++pp)
 #pragma unroll
         for (int g = 0; g < 3; ++g)
-          sx[((pp + M * q) * 3 + g) * 16 + cell] = (ax[g][pp] + sg[g]) + p.origin[g];
+          sx[((pp + M * q) * 3 + g) * 16 + cell] =
+              N64 ? static_cast<float>((static_cast<double>(ax[g][pp]) + sg[g]) + p.origin[g])
+                  : static_cast<float>((ax[g][pp] + sg[g]) + p.origin[g]);
     if (mask & kFJac)
 #pragma unroll
       for (int pp = 0; pp < M; ++pp)
@@ -313,13 +366,14 @@ __global__ void __launch_bounds__(64 * kWaves)
   if (r < M) flush_tile(t - nx);
 }
 
-using MfmaFn = void (*)(const float*, const uint32_t*, const float*, const uint32_t*, const float*,
+template <typename TN>
+using MfmaFn = void (*)(const TN*, const uint32_t*, const TN*, const uint32_t*, const float*,
                         const double*, FactorParams<float>, FactorOutputs);
 
-template <bool SHIFT, bool FULL>
-MfmaFn pick(int n1, int m) {
+template <bool SHIFT, bool FULL, typename TN>
+MfmaFn<TN> pick(int n1, int m) {
 #define TG_SFM(N1_, M_) \
-  if (n1 == N1_ && m == M_) return factor_sumfact_mfma<N1_, M_, SHIFT, FULL>;
+  if (n1 == N1_ && m == M_) return factor_sumfact_mfma<N1_, M_, SHIFT, FULL, TN>;
   TG_SFM(2, 2) TG_SFM(2, 3) TG_SFM(2, 4)
   TG_SFM(3, 2) TG_SFM(3, 3) TG_SFM(3, 4)
   TG_SFM(4, 2) TG_SFM(4, 3) TG_SFM(4, 4)
@@ -338,18 +392,10 @@ int groups_per_cu() {
   return g;
 }
 
-}  // namespace
-
-bool has_sumfact_mfma(int n1, int m) { return pick<true, true>(n1, m) != nullptr; }
-
-hipError_t launch_sumfact_mfma(int n1, int m, bool shift, const float* nodes,
-                               const uint32_t* dofmap, const float* tab1d, const uint32_t* alex,
-                               const float* tab2d, const double* tab64,
-                               const FactorParams<float>& p, const FactorOutputs& out,
-                               hipStream_t stream) {
-  const bool full = p.mask == (kFPoints | kFJac | kFDetJ);
-  MfmaFn fn = shift ? (full ? pick<true, true>(n1, m) : pick<true, false>(n1, m))
-                    : (full ? pick<false, true>(n1, m) : pick<false, false>(n1, m));
+template <typename TN>
+hipError_t launch(MfmaFn<TN> fn, const TN* nodes, const uint32_t* dofmap, const TN* tab1d,
+                  const uint32_t* alex, const float* tab2d, const double* tab64,
+                  const FactorParams<float>& p, const FactorOutputs& out, hipStream_t stream) {
   if (!fn || p.tile_log2 != 4) return hipErrorNotSupported;
   int dev = 0, cus = 0;
   hipError_t e = hipGetDevice(&dev);
@@ -364,6 +410,31 @@ hipError_t launch_sumfact_mfma(int n1, int m, bool shift, const float* nodes,
   hipLaunchKernelGGL(fn, dim3(static_cast<uint32_t>(grid)), dim3(64 * kWaves), 0, stream, nodes,
                      dofmap, tab1d, alex, tab2d, tab64, p, out);
   return hipGetLastError();
+}
+
+}  // namespace
+
+bool has_sumfact_mfma(int n1, int m) { return pick<true, true, float>(n1, m) != nullptr; }
+
+hipError_t launch_sumfact_mfma(int n1, int m, bool shift, const float* nodes,
+                               const uint32_t* dofmap, const float* tab1d, const uint32_t* alex,
+                               const float* tab2d, const double* tab64,
+                               const FactorParams<float>& p, const FactorOutputs& out,
+                               hipStream_t stream) {
+  const bool full = p.mask == (kFPoints | kFJac | kFDetJ);
+  MfmaFn<float> fn = shift ? (full ? pick<true, true, float>(n1, m) : pick<true, false, float>(n1, m))
+                           : (full ? pick<false, true, float>(n1, m) : pick<false, false, float>(n1, m));
+  return launch(fn, nodes, dofmap, tab1d, alex, tab2d, tab64, p, out, stream);
+}
+
+hipError_t launch_sumfact_mfma_f64_nodes(int n1, int m, const double* nodes,
+                                         const uint32_t* dofmap, const double* tab1d,
+                                         const uint32_t* alex, const float* tab2d,
+                                         const double* tab64, const FactorParams<float>& p,
+                                         const FactorOutputs& out, hipStream_t stream) {
+  const bool full = p.mask == (kFPoints | kFJac | kFDetJ);
+  MfmaFn<double> fn = full ? pick<true, true, double>(n1, m) : pick<true, false, double>(n1, m);
+  return launch(fn, nodes, dofmap, tab1d, alex, tab2d, tab64, p, out, stream);
 }
 
 }  // namespace tg

@@ -64,10 +64,9 @@ def test_factors_against_the_exact_mesh(cuda, kind, cell, p):
 @pytest.mark.parametrize("p", [1, 2, 3])
 @pytest.mark.parametrize("m", [2, 3, 4])
 def test_sum_factorised_hexahedra_round_the_fp64_kernel(cuda, p, m):
-    """Sum-factorised hexahedra on FP64 nodes take their sums in FP64 (the role of the hi/lo
-    split of J: the FP32 sums over the 64 nodes of Q3 round to ~1.5e-6 of J on curved cells even
-    from exact offsets): X, J and det J are the FP64 kernel's values rounded to FP32, bit for bit
-    (the box's origin, 0.5, is exact in FP32)."""
+    """The vector-ALU sum-factorised hexahedra on FP64 nodes (TG_OPT_NO_MFMA) take their sums in
+    FP64: X, J and det J are the FP64 kernel's values rounded to FP32, bit for bit (the box's
+    origin, 0.5, is exact in FP32)."""
     import torch
 
     mesh = make_case("vol", HEX, p)
@@ -77,9 +76,32 @@ def test_sum_factorised_hexahedra_round_the_fp64_kernel(cuda, p, m):
     assert all(float(o) == float(np.float32(o)) for o in g64.origin)
     want = gpu_factors(g64, 0, ps, names, options=api.OPT_SUMFACT_ALWAYS)
     g = api.geometry_from_host(mesh, dtype=torch.float32, nodes_f64=True)
-    got = gpu_factors(g, 0, ps, names, options=api.OPT_SUMFACT_ALWAYS)
+    got = gpu_factors(g, 0, ps, names, options=api.OPT_SUMFACT_ALWAYS | api.OPT_NO_MFMA)
     for n in names:
         assert np.array_equal(got[n], want[n].astype(np.float32).astype(np.float64)), n
+
+
+@pytest.mark.parametrize("p", [1, 2, 3])
+@pytest.mark.parametrize("m", [2, 3, 4])
+def test_matrix_core_hexahedra_on_fp64_nodes(cuda, p, m):
+    """The matrix-core kernel on FP64 nodes (first sums in FP64, their hi and lo FP32 parts
+    through the products) against the FP64 kernel on the same mesh: X, J and det J within 1e-6;
+    and the FP32 arithmetic is visible, so this is not the vector-ALU FP64 path."""
+    import torch
+
+    mesh = make_case("vol", HEX, p)
+    ps = api.PointSet.gauss(HEX, 2 * m - 1)
+    names = ["points", "jacobians", "detj"]
+    g64 = api.geometry_from_host(mesh, dtype=torch.float64)
+    want = gpu_factors(g64, 0, ps, names, options=api.OPT_SUMFACT_ALWAYS)
+    g = api.geometry_from_host(mesh, dtype=torch.float32, nodes_f64=True)
+    got = gpu_factors(g, 0, ps, names, options=api.OPT_SUMFACT_ALWAYS)
+    valu = gpu_factors(g, 0, ps, names, options=api.OPT_SUMFACT_ALWAYS | api.OPT_NO_MFMA)
+    err = rel_errs(got, want)
+    print(f"p={p} m={m} matrix cores on FP64 nodes {err}")
+    for n in names:
+        assert err[n] <= 1e-6, (n, err[n])
+    assert not np.array_equal(got["jacobians"], valu["jacobians"])
 
 
 @pytest.mark.parametrize("cell,p", [(TET, 2), (HEX, 2), (PRISM, 2)])
