@@ -222,22 +222,22 @@ __global__ void __launch_bounds__(kBlock)
   if (n >= num_nodes) return;
   const uint64_t e0 = offsets[n], e1 = offsets[n + 1];
   double a0 = 0.0, a1 = 0.0, a2 = 0.0;
-  // Four entries' loads are issued before their (in-order) additions.
+  // One entry is one vector load of its three components; four entries' loads are issued
+  // before their (in-order) additions.
+  struct Entry {
+    T v[3];
+  };
+  const Entry* __restrict__ entries = reinterpret_cast<const Entry*>(contrib);
   for (uint64_t e = e0; e < e1; e += 4) {
-    T v[4][3];
+    Entry v[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const T* src = contrib + (e + i < e1 ? e + i : e0) * 3;
-      v[i][0] = src[0];
-      v[i][1] = src[1];
-      v[i][2] = src[2];
-    }
+    for (int i = 0; i < 4; ++i) v[i] = entries[e + i < e1 ? e + i : e0];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       if (e + i < e1) {
-        a0 += static_cast<double>(v[i][0]);
-        a1 += static_cast<double>(v[i][1]);
-        a2 += static_cast<double>(v[i][2]);
+        a0 += static_cast<double>(v[i].v[0]);
+        a1 += static_cast<double>(v[i].v[1]);
+        a2 += static_cast<double>(v[i].v[2]);
       }
     }
   }

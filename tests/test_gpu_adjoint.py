@@ -5,7 +5,8 @@ Config 5: the config-1 mesh as P2 tetrahedra (32^3 x 6 = 196,608 cells, curved b
 jitter of the interior nodes), the 14-point degree-5 rule, beta = 0.1.
 Tolerances (written per test): FP64 kernels against the FP64 CPU adjoint 1e-12 (value) and
 1e-11 (gradient, max-norm relative); FP32 forward 1e-6 relative; FP32-J gradient 1e-5
-max-norm relative; central differences with h = 1e-6 on the FP64 forward 1e-6 relative.
+max-norm relative; central differences with h = 1e-6 on the FP64 forward 1e-6 relative (a
+6^3 x 6 box) and 2e-6 on config 5 itself (the rounding of F = 2e4, derived in the test).
 """
 import numpy as np
 import pytest
@@ -77,6 +78,34 @@ def test_central_differences(cuda):
         worst = max(worst, abs(fd - an) / max(abs(an), 1e-300))
     print(f"central differences: worst relative mismatch {worst:.2e}")
     assert worst <= 1e-6
+
+
+def test_central_differences_config5(cuda, config5):
+    """The same check on config 5 itself (196,608 cells, 274,625 nodes): 64 seeded directions
+    over every node, h = 1e-6, the FP64 gradient against the FP64 forward. Tolerance 2e-6
+    relative, from the differences' own rounding: F = 2.0e4 here (the quality term dominates),
+    each evaluation's fixed-order sum of 2.75 M terms rounds to about 20 eps F = 4e-11, so a
+    difference quotient is uncertain by about 4e-11 / h = 4e-5 against directional derivatives
+    of 10-100 (measured worst 7.8e-7); the h^2 truncation, about (h / (h_mesh / 2))^2 = 1.6e-8,
+    is below that."""
+    import torch
+
+    mesh = config5
+    ps = api.PointSet.tet_14()
+    g = api.geometry_from_host(mesh, dtype=torch.float64)
+    fn = api.MeshFunctional(g, 0, ps, beta=BETA)
+    grad = fn.vjp().cpu().numpy()
+    x0 = g.nodes.clone()
+    h = 1e-6
+    worst = 0.0
+    for i in range(64):
+        d = M.splitmix64_uniform(2000 + i, x0.numel()).reshape(x0.shape) * 2.0 - 1.0
+        dt = torch.from_numpy(d).to(x0.device)
+        fd = (fn.forward(x0 + h * dt).item() - fn.forward(x0 - h * dt).item()) / (2 * h)
+        an = float((grad * d).sum())
+        worst = max(worst, abs(fd - an) / abs(fd))
+    print(f"config 5 central differences: worst relative mismatch {worst:.2e}")
+    assert worst <= 2e-6
 
 
 def test_autograd_semantics(cuda):
