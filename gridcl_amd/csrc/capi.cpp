@@ -400,12 +400,15 @@ int enqueue_block(const tg_geometry* g, const Block& b, tg_pointset* pts, Device
     }
   }
   // The cooperative strategy: forced, or by default for cells of many nodes (56 in FP32, 84 in
-  // FP64: tetrahedra from degree 5 and 6), where it measured faster than the runtime-K kernel
-  // (P5 tetrahedra FP32 1.07 against 1.47 ms, P6 0.62 against 1.65 ms, P8 0.77 against 6.2 ms;
-  // FP64 P6 1.07 against 2.57 ms; P4 and FP64 P5 favour the runtime-K kernel). The reference's
-  // GPU policy takes it for tetrahedra from degree 7 (FP32) and 6 (FP64), degree-policy.toml:77,86.
+  // FP64: tetrahedra from degree 5 and 6) that have no register-resident kernel, where it
+  // measured faster than the runtime-K kernel (P5 tetrahedra FP32 1.07 against 1.47 ms, P6 0.62
+  // against 1.65 ms, P8 0.77 against 6.2 ms; FP64 P6 1.07 against 2.57 ms; P4 and FP64 P5
+  // favour the runtime-K kernel). Q3 hexahedra (64 nodes) keep their register kernel: 4.9 ms
+  // against 17.9 ms cooperative on 96^3 cells at 4^3 points. The reference's GPU policy takes
+  // it for tetrahedra from degree 7 (FP32) and 6 (FP64), degree-policy.toml:77,86.
   if (!n64 && !(options & TG_OPT_FORCE_GENERIC) &&
-      ((options & TG_OPT_COOPERATIVE) || b.k >= (f32 ? 56u : 84u))) {
+      ((options & TG_OPT_COOPERATIVE) ||
+       (b.k >= (f32 ? 56u : 84u) && !tg::has_direct_factor<T>(b.tdim, g->gdim, b.k, shift)))) {
     hipError_t e = tg::launch_factor_coop<T>(
         b.tdim, g->gdim, shift, static_cast<const T*>(g->nodes), b.dofmap, b.flags,
         static_cast<const T*>(tabs->tab), tabs->tab64, static_cast<const T*>(tabs->ref_normals),

@@ -1165,6 +1165,11 @@ bool has_measure_cells(int tdim, int gdim, uint32_t k, bool shift) {
 }
 
 template <typename T>
+bool has_direct_factor(int tdim, int gdim, uint32_t k, bool shift) {
+  return direct_kernel<T, T>(tdim, gdim, k, shift) != nullptr;
+}
+
+template <typename T>
 hipError_t launch_measure_cells(int tdim, int gdim, bool shift, const T* nodes,
                                 const uint32_t* dofmap, const T* tab, const double* tab64,
                                 const double* weights, const FactorParams<T>& p, T* partials,
@@ -1277,6 +1282,7 @@ template hipError_t launch_factor_coop<FT>(int, int, bool, const FT*, const uint
 template hipError_t launch_evector<FT>(const FT*, const uint32_t*, uint64_t, uint64_t, uint32_t,
                                        int, uint32_t, FT*, hipStream_t);
 template bool has_measure_cells<FT>(int, int, uint32_t, bool);
+template bool has_direct_factor<FT>(int, int, uint32_t, bool);
 template hipError_t launch_measure_cells<FT>(int, int, bool, const FT*, const uint32_t*, const FT*,
                                              const double*, const double*,
                                              const FactorParams<FT>&, FT*, hipStream_t);

@@ -181,6 +181,10 @@ hipError_t launch_evector(const T* nodes, const uint32_t* dofmap, uint64_t num_c
                           uint64_t padded, uint32_t k, int gdim, uint32_t last_node, T* out,
                           hipStream_t stream);
 
+// Whether launch_factor has a register-resident kernel for (tdim, gdim, k, shift).
+template <typename T>
+bool has_direct_factor(int tdim, int gdim, uint32_t k, bool shift);
+
 // Whether launch_measure_cells has a kernel for (tdim, gdim, k, shift).
 template <typename T>
 bool has_measure_cells(int tdim, int gdim, uint32_t k, bool shift);
