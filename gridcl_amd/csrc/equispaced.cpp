@@ -85,7 +85,7 @@ int convert(tg_geometry* g, const std::vector<Occurrences>& occ, uint32_t* first
     if (rc) return rc;
     const uint64_t elems = (b.num_cells + 15) / 16 * 16 * pts->nq * g->gdim;
     void* field = nullptr;
-    hipError_t e = hipMallocAsync(&field, elems * sizeof(T), s);
+    hipError_t e = tg::work_alloc(&field, elems * sizeof(T), s);
     if (e != hipSuccess) return tg::hip_fail(e, "equispaced conversion field");
     tg_outputs outs{};
     outs.points = field;
@@ -97,7 +97,7 @@ int convert(tg_geometry* g, const std::vector<Occurrences>& occ, uint32_t* first
                                      static_cast<T*>(out), s);
       if (e != hipSuccess) rc = tg::hip_fail(e, "equispaced conversion kernel launch");
     }
-    const hipError_t ef = hipFreeAsync(field, s);
+    const hipError_t ef = tg::work_free(field, s);
     if (rc) return rc;
     if (ef != hipSuccess) return tg::hip_fail(ef, "equispaced conversion field");
   }

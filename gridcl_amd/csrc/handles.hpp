@@ -48,6 +48,12 @@ struct DeviceTables {
 // Records the message of the last error on this thread and returns `code`.
 int fail(int code, const std::string& msg);
 int hip_fail(hipError_t e, const char* what);
+// Stream-ordered work memory of the library's calls from a per-device pool that keeps freed
+// blocks mapped (the measure's work arrays, the equispaced conversion's lattice field, the VTU
+// writer's converted nodes): the pool holds the largest such set of blocks in use at once until
+// the process ends.
+hipError_t work_alloc(void** p, size_t bytes, hipStream_t s);
+hipError_t work_free(void* p, hipStream_t s);
 const char* cell_name(int c);
 
 }  // namespace tg

@@ -95,7 +95,7 @@ int measure_block_with(const tg_geometry* g, int32_t block, tg_pointset* pts, ui
                             : plan_measure<double>(g, b, pts, tabs, options);
   bool owned = false;
   if (work == nullptr) {
-    const hipError_t e = hipMallocAsync(&work, m.bytes, s);
+    const hipError_t e = work_alloc(&work, m.bytes, s);
     if (e != hipSuccess) return hip_fail(e, "measure work arrays");
     owned = true;
   } else if (m.bytes > work_bytes) {
@@ -107,7 +107,7 @@ int measure_block_with(const tg_geometry* g, int32_t block, tg_pointset* pts, ui
   rc = f32 ? run_measure<float>(g, b, pts, tabs, weights, options, m, work, value, s)
            : run_measure<double>(g, b, pts, tabs, weights, options, m, work, value, s);
   if (owned) {
-    const hipError_t ef = hipFreeAsync(work, s);
+    const hipError_t ef = work_free(work, s);
     if (rc == TG_OK && ef != hipSuccess) rc = hip_fail(ef, "measure work arrays");
   }
   return rc;

@@ -65,7 +65,7 @@ int tg_vtu_write_geometry(const tg_geometry* g, const char* path, uint32_t optio
   void* conv = nullptr;
   int rc = TG_OK;
   if (gll && n) {
-    hipError_t e = hipMallocAsync(&conv, n * elem, s);
+    hipError_t e = tg::work_alloc(&conv, n * elem, s);
     if (e != hipSuccess) return tg::hip_fail(e, "VTU equispaced nodes");
     rc = tg_equispaced_nodes(g, conv, s);
     src = conv;
@@ -92,7 +92,7 @@ int tg_vtu_write_geometry(const tg_geometry* g, const char* path, uint32_t optio
   if (rc == TG_OK) rc = download(point_data, num_point_data, bufs, pd, s);
   if (rc == TG_OK) rc = download(cell_data, num_cell_data, bufs, cd, s);
   if (conv) {
-    const hipError_t e = hipFreeAsync(conv, s);
+    const hipError_t e = tg::work_free(conv, s);
     if (rc == TG_OK && e != hipSuccess) rc = tg::hip_fail(e, "VTU equispaced nodes");
   }
   const hipError_t e = hipStreamSynchronize(s);
