@@ -104,6 +104,27 @@ def test_matrix_core_hexahedra_on_fp64_nodes(cuda, p, m):
     assert not np.array_equal(got["jacobians"], valu["jacobians"])
 
 
+@pytest.mark.parametrize("p,m", [(2, 3), (3, 4)])
+def test_matrix_core_factor_subsets_on_fp64_nodes(cuda, p, m):
+    """The matrix-core kernel on FP64 nodes asked for one factor at a time (det J alone, as the
+    measure asks; X alone, as the equispaced conversion does; J alone) gives the values of the
+    call for all three, bit for bit; and the measure (det J alone through the same kernel) is
+    the FP64 geometry's within 1e-6."""
+    import torch
+
+    mesh = make_case("vol", HEX, p)
+    ps = api.PointSet.gauss(HEX, 2 * m - 1)
+    names = ["points", "jacobians", "detj"]
+    g = api.geometry_from_host(mesh, dtype=torch.float32, nodes_f64=True)
+    full = gpu_factors(g, 0, ps, names)
+    for n in names:
+        one = gpu_factors(g, 0, ps, [n])
+        assert np.array_equal(one[n], full[n]), n
+    vol = api.measure_block(g, 0, ps)
+    vol64 = api.measure_block(api.geometry_from_host(mesh, dtype=torch.float64), 0, ps)
+    assert abs(vol - vol64) <= 1e-6 * abs(vol64), (vol, vol64)
+
+
 @pytest.mark.parametrize("cell,p", [(TET, 2), (HEX, 2), (PRISM, 2)])
 def test_generic_and_dense_kernels(cuda, cell, p):
     """The runtime-K kernel (forced) and the dense hexahedron kernel (sum factorisation off) on
