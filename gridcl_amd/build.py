@@ -27,6 +27,7 @@ HIP_SOURCES = [
     # vector ALU at once, and the accumulation registers would otherwise count twice (4 waves
     # per SIMD instead of 3 for Q3).
     ("sumfact_mfma.hip", ["-mllvm", "-amdgpu-mfma-vgpr-form"]),
+    ("dense_mfma.hip", ["-mllvm", "-amdgpu-mfma-vgpr-form"]),
     ("adjoint_kernels.hip", []),
     ("measure_kernels.hip", []),
     ("measure_noalloc.hip", []),

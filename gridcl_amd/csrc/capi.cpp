@@ -93,6 +93,7 @@ DeviceTables::~DeviceTables() {
   if (sf_tab64) (void)hipFree(sf_tab64);
   if (sf_lex) (void)hipFree(sf_lex);
   if (sf_mfma) (void)hipFree(sf_mfma);
+  if (dense_mfma) (void)hipFree(dense_mfma);
 }
 
 }  // namespace tg
@@ -237,6 +238,26 @@ int device_tables(tg_pointset* p, unsigned degree, int family, int dtype, Device
     rc = dtype == TG_F32 ? upload<float>(p->normals, &t->ref_normals)
                          : upload<double>(p->normals, &t->ref_normals);
     if (rc) return rc;
+  }
+  if (dtype == TG_F32 && p->tdim == 3 && tg::has_dense_mfma(3, 3, static_cast<uint32_t>(k)) &&
+      p->nq <= tg::kDenseMfmaMaxPoints) {
+    // The table in matrix-core operand order (dense_mfma.hip): block qb, kind v, step s, lane l
+    // holds T_v[16 qb + l % 16][4 s + l / 16].
+    const uint32_t kk = static_cast<uint32_t>(k), nq = static_cast<uint32_t>(p->nq);
+    const uint32_t S = (kk + 3) / 4, nqb = (nq + 15) / 16;
+    std::vector<double> a(tg::dense_mfma_table_elems(kk, nq), 0.0);
+    for (uint32_t qb = 0; qb < nqb; ++qb)
+      for (uint32_t v = 0; v < 4; ++v)
+        for (uint32_t st = 0; st < S; ++st)
+          for (uint32_t l = 0; l < 64; ++l) {
+            const uint32_t q = qb * 16 + (l & 15u), node = 4 * st + (l >> 4);
+            if (q < nq && node < kk)
+              a[((qb * 4 + v) * S + st) * 64 + l] = inter[(q * kk + node) * 4 + v];
+          }
+    void* dev = nullptr;
+    rc = upload<float>(a, &dev);
+    if (rc) return rc;
+    t->dense_mfma = static_cast<float*>(dev);
   }
   std::vector<double> tab1d;
   std::vector<uint32_t> lex;
@@ -399,6 +420,21 @@ static bool takes_sumfact(const tg_geometry* g, const DeviceTables* tabs, uint32
          ((options & TG_OPT_SUMFACT_ALWAYS) || sumfact_pays(f32, tabs->sf_n1, tabs->sf_m));
 }
 
+// Whether the factors of a block go to the dense matrix-core kernel: FP32 with the cell shift on
+// FP32 nodes, P3 tetrahedra (the point set's operand-order table exists), tiles of 16, calls
+// that ask for J (X and det J alone keep the register kernels, as the measure does), unless a
+// strategy is forced. Measured on MI355X, X + J + det J, 663,552 cells: 14 points 0.121 against
+// 0.150 ms on the register kernel, 64 points 0.413 against 0.503 ms; P2 tetrahedra were slower
+// on it (0.212 against 0.197 ms at 14 points) and keep the register kernel.
+bool takes_dense_mfma(const tg_geometry* g, const Block& b, const DeviceTables* tabs,
+                      uint32_t factors, uint32_t tile, uint32_t options) {
+  const bool shift = g->dtype == TG_F32 && !(options & TG_OPT_NO_CELL_SHIFT);
+  return shift && !g->nodes64 && tabs->dense_mfma && tile == 16 && b.tdim == 3 &&
+         g->gdim == 3 && (factors & TG_JACOBIAN) &&
+         !(factors & ~(TG_POINTS | TG_JACOBIAN | TG_DETJ)) &&
+         !(options & (TG_OPT_NO_MFMA | TG_OPT_FORCE_GENERIC | TG_OPT_COOPERATIVE));
+}
+
 template <typename T>
 int enqueue_block(const tg_geometry* g, const Block& b, tg_pointset* pts, DeviceTables* tabs,
                   uint32_t factors, uint32_t tile, uint32_t options, const tg_outputs* out,
@@ -445,6 +481,14 @@ int enqueue_block(const tg_geometry* g, const Block& b, tg_pointset* pts, Device
           static_cast<const T*>(tabs->sf_tab), tabs->sf_lex, tabs->tab64, p, o, stream);
       if (e == hipSuccess) return TG_OK;
       if (e != hipErrorNotSupported) return hip_fail(e, "sum-factorised kernel launch");
+    }
+  }
+  if constexpr (sizeof(T) == 4) {
+    if (takes_dense_mfma(g, b, tabs, factors, tile, options)) {
+      hipError_t e = tg::launch_dense_mfma(b.k, static_cast<const float*>(g->nodes), b.dofmap,
+                                           tabs->dense_mfma, tabs->tab64, p, o, stream);
+      if (e == hipSuccess) return TG_OK;
+      if (e != hipErrorNotSupported) return hip_fail(e, "dense matrix-core launch");
     }
   }
   // The cooperative strategy: forced, or by default for cells of many nodes (56 in FP32, 84 in

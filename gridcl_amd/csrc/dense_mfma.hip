@@ -1,0 +1,268 @@
+// X, J and det J of FP32 cells of dense bases (P3 tetrahedra: 20 nodes) with the node sums on
+// the matrix cores (v_mfma_f32_16x16x4f32). Same factors as the register-resident kernels
+// (the reference's factor kernel, tetraloma-geometry/src/layout.rs:138-213: X_g = sum_k x_kg
+// phi_k, J_gd = sum_k x_kg d_d phi_k, det J by first-row cofactors), with the sums taken as
+// matrix products over the cells of a tile:
+//
+//   D_v[q][cell] = sum_k T_v[q][k] x_g[k][cell]      v = 0 (phi), 1..3 (d phi / d xi_d)
+//
+// One wavefront per tile of 16 cells. Its lanes gather the tile's nodes into LDS, four lanes
+// per cell, as offsets from each cell's node 0 (node 0 itself stays absolute and X adds it
+// back, as every FP32 kernel here). The B operand of lane l is the offset of node 4s + l / 16
+// of cell l % 16 (zero for node 0 and past k), held in registers for the whole tile; the A
+// operand is the table value T_v[q = 16 qb + l % 16][k = 4s + l / 16], laid out on the host in
+// operand order (one coalesced load per product from the L1-resident table). Each block of 16
+// points takes 4 kinds x 3 coordinates x ceil(k / 4) products, and lane l receives the points
+// 16 qb + 4 (l / 16) + i (i = 0..3, one per register) of cell l % 16: X and all of J of its
+// points, so it forms det J (with the FP64 recomputation where |det J| < tau |J|_F^3). The
+// block's rows of the tile are contiguous in each field: they are staged in LDS (over the
+// gathered nodes, dead once the operands are in registers) and leave as 16-byte non-temporal
+// stores.
+//
+// The products accumulate in an order of their own (four nodes per step), so the values differ
+// from the register kernels' in the last bits; the parity tests hold them to the oracle.
+// Workgroups are persistent: those of one XCD walk one contiguous range of tiles, four tiles
+// (one per wavefront) at a time.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "geom_internal.hpp"
+#include "sumfact_tile.hpp"
+
+namespace tg {
+namespace {
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+constexpr int kWaves = 4;
+
+__device__ __forceinline__ v4f mfma4(float a, float b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+template <typename T>
+__device__ __forceinline__ gptr<T> gp(T* p) {
+  return (gptr<T>)p;
+}
+
+// Elements of a wavefront's LDS: its gathered nodes [k][3][16 cells], then, over them, the
+// stage of one block of 16 points: X [16][3][16], J [16][9][16], det J [16][16].
+template <int K>
+constexpr int dense_wave_elems() {
+  return K * 3 * 16 > 13 * 256 ? K * 3 * 16 : 13 * 256;
+}
+
+// Copies n values (a multiple of 4) from LDS to global memory with 16-byte non-temporal stores
+// spread over the wavefront's lanes.
+__device__ __forceinline__ void flush(const float* __restrict__ src, gptr<float> dst, uint32_t n,
+                                      uint32_t lane) {
+  for (uint32_t e = lane; e < n / 4u; e += 64u) {
+    const v4f v = *reinterpret_cast<const v4f*>(src + e * 4u);
+    __builtin_nontemporal_store(v, reinterpret_cast<gptr<v4f>>(dst + e * 4u));
+  }
+}
+
+template <int K>
+__global__ void __launch_bounds__(64 * kWaves)
+    dense_mfma(const float* __restrict__ nodes, const uint32_t* __restrict__ dofmap,
+               const float* __restrict__ taba, const double* __restrict__ tab64,
+               FactorParams<float> p, FactorOutputs o) {
+  constexpr int S = (K + 3) / 4;  // node steps of four (and gather rounds of four lanes a cell)
+  constexpr int W = dense_wave_elems<K>();
+  __shared__ __align__(16) float lds[kWaves * W];
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint32_t nqb = (p.nq + 15u) >> 4;
+  float* __restrict__ xs = lds + w * W;
+  float* __restrict__ sx = xs;
+  float* __restrict__ sj = xs + 3 * 256;
+  float* __restrict__ sd = xs + 12 * 256;
+
+  // Tiles: XCD x = blockIdx.x % 8 takes the contiguous range [lo, hi) of the block's tiles; its
+  // grid / 8 workgroups take four consecutive tiles (one per wavefront) at a time, in turn.
+  const uint64_t ntiles = p.padded_cells >> 4;
+  const uint32_t nx = gridDim.x >> 3, xcd = blockIdx.x & 7u, j = blockIdx.x >> 3;
+  const uint64_t per = ntiles >> 3, rem = ntiles & 7u;
+  const uint64_t lo = xcd * per + (xcd < rem ? xcd : rem);
+  const uint64_t hi = lo + per + (xcd < rem ? 1u : 0u);
+
+  const uint32_t cell = lane & 15u, kg = lane >> 4;
+  const uint32_t mask = p.mask;
+  gptr<float> ox = gp(static_cast<float*>(o.x));
+  gptr<float> oj = gp(static_cast<float*>(o.j));
+  gptr<float> od = gp(static_cast<float*>(o.detj));
+  for (uint64_t t = lo + (static_cast<uint64_t>(j) * kWaves + w); t < hi;
+       t += static_cast<uint64_t>(nx) * kWaves) {
+    const uint64_t c0 = t * 16 + cell;
+    const uint64_t c = c0 < p.num_cells ? c0 : p.num_cells - 1;
+    const uint32_t* __restrict__ row = dofmap + c * K;
+    // Gather: lane (cell, kg) loads nodes kg, kg + 4, ... of its cell (past k: node k - 1
+    // again, so the loads go together) and node 0.
+    float x0[3];
+    {
+      const uint32_t r0 = gp(row)[0];
+      const uint32_t n0 = r0 < p.last_node ? r0 : p.last_node;
+#pragma unroll
+      for (int g = 0; g < 3; ++g) x0[g] = gp(nodes)[static_cast<uint64_t>(n0) * 3 + g];
+    }
+    float b[S][3];
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+      const uint32_t kk = kg + 4u * i;
+      const uint32_t r = gp(row)[kk < static_cast<uint32_t>(K) ? kk : K - 1];
+      const uint32_t n = r < p.last_node ? r : p.last_node;
+#pragma unroll
+      for (int g = 0; g < 3; ++g) b[i][g] = gp(nodes)[static_cast<uint64_t>(n) * 3 + g];
+    }
+    sf::wave_sync();  // the previous tile's reads of the stage are done
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+      const uint32_t kk = kg + 4u * i;
+      if (kk < static_cast<uint32_t>(K)) {
+#pragma unroll
+        for (int g = 0; g < 3; ++g) xs[(kk * 3 + g) * 16 + cell] = kk > 0 ? b[i][g] - x0[g] : 0.f;
+      }
+    }
+    sf::wave_sync();
+    // B operands: the offset of node 4s + kg of cell `cell` (zero for node 0 and past k).
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const uint32_t k = 4u * s + kg;
+#pragma unroll
+      for (int g = 0; g < 3; ++g) b[s][g] = k < static_cast<uint32_t>(K) ? xs[(k * 3 + g) * 16 + cell] : 0.f;
+    }
+    sf::wave_sync();  // the nodes are in registers; the stage may overwrite them
+    for (uint32_t qb = 0; qb < nqb; ++qb) {
+      const v4f zero = {0.f, 0.f, 0.f, 0.f};
+      v4f acc[4][3];
+#pragma unroll
+      for (int vv = 0; vv < 4; ++vv)
+#pragma unroll
+        for (int g = 0; g < 3; ++g) acc[vv][g] = zero;
+      gptr<const float> ta = gp(taba) + qb * (4u * S * 64u) + lane;
+#pragma unroll
+      for (int s = 0; s < S; ++s) {
+#pragma unroll
+        for (int vv = 0; vv < 4; ++vv) {
+          if (vv == 0 && !(mask & kFPoints)) continue;
+          const float a = ta[(vv * S + s) * 64];
+#pragma unroll
+          for (int g = 0; g < 3; ++g) acc[vv][g] = mfma4(a, b[s][g], acc[vv][g]);
+        }
+      }
+      // Lane: rows qq = 4 kg + i of the block (points 16 qb + qq) of cell `cell`;
+      // J(g, d) = acc[1 + d][g]. det J of every point first; X and J go to the stage, then the
+      // FP64 recomputation of flagged points runs with the accumulators dead.
+      const uint32_t nv = p.nq - qb * 16u < 16u ? p.nq - qb * 16u : 16u;  // rows in the block
+      float det[4];
+      uint32_t near = 0;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+#pragma clang fp contract(off)
+        float jm[3][3];  // jm[d][g] = J(g, d)
+#pragma unroll
+        for (int d = 0; d < 3; ++d)
+#pragma unroll
+          for (int g = 0; g < 3; ++g) jm[d][g] = acc[1 + d][g][i];
+        det[i] = jm[0][0] * (jm[1][1] * jm[2][2] - jm[2][1] * jm[1][2]) -
+                 jm[1][0] * (jm[0][1] * jm[2][2] - jm[2][1] * jm[0][2]) +
+                 jm[2][0] * (jm[0][1] * jm[1][2] - jm[1][1] * jm[0][2]);
+        if (p.fixup) {
+          float n2 = 0.f;
+#pragma unroll
+          for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int g = 0; g < 3; ++g) n2 = __builtin_fmaf(jm[d][g], jm[d][g], n2);
+          if (__builtin_fabsf(det[i]) < static_cast<float>(p.tau) * n2 * __builtin_sqrtf(n2))
+            near |= 1u << i;
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const uint32_t qq = kg * 4u + static_cast<uint32_t>(i);
+        if (mask & kFPoints) {
+#pragma unroll
+          for (int g = 0; g < 3; ++g) sx[(qq * 3 + g) * 16 + cell] = (x0[g] + acc[0][g][i]) + p.origin[g];
+        }
+        if (mask & kFJac) {
+#pragma unroll
+          for (int d = 0; d < 3; ++d)
+#pragma unroll
+            for (int g = 0; g < 3; ++g) sj[(qq * 9 + d * 3 + g) * 16 + cell] = acc[1 + d][g][i];
+        }
+      }
+      if (mask & kFDetJ) {
+        if (near) {  // rare
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+            if (near & (1u << i))
+              det[i] = static_cast<float>(sf::detj_fp64<K>(nodes, row, p.last_node, tab64,
+                                                           qb * 16u + kg * 4u + i));
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) sd[(kg * 4u + i) * 16 + cell] = det[i];
+      }
+      sf::wave_sync();
+      const uint64_t r0 = t * p.nq + qb * 16u;  // the block's first row of the tile
+      if (mask & kFPoints) flush(sx, ox + ((r0 * 3) << 4), nv * 48u, lane);
+      if (mask & kFJac) flush(sj, oj + ((r0 * 9) << 4), nv * 144u, lane);
+      if (mask & kFDetJ) flush(sd, od + (r0 << 4), nv * 16u, lane);
+      sf::wave_sync();  // the stage is read; the next block may overwrite it
+    }
+  }
+}
+
+using DenseFn = void (*)(const float*, const uint32_t*, const float*, const double*,
+                         FactorParams<float>, FactorOutputs);
+
+DenseFn pick(uint32_t k) {
+  switch (k) {
+    case 20:
+      return dense_mfma<20>;
+    default:
+      return nullptr;
+  }
+}
+
+}  // namespace
+
+bool has_dense_mfma(int tdim, int gdim, uint32_t k) {
+  return tdim == 3 && gdim == 3 && pick(k) != nullptr;
+}
+
+size_t dense_mfma_table_elems(uint32_t k, uint32_t nq) {
+  const uint32_t S = (k + 3) / 4, nqb = (nq + 15) / 16;
+  return static_cast<size_t>(nqb) * 4 * S * 64;
+}
+
+hipError_t launch_dense_mfma(uint32_t k, const float* nodes, const uint32_t* dofmap,
+                             const float* taba, const double* tab64, const FactorParams<float>& p,
+                             const FactorOutputs& out, hipStream_t stream) {
+  DenseFn fn = pick(k);
+  if (!fn || p.tile_log2 != 4 || p.nq == 0 || p.nq > kDenseMfmaMaxPoints)
+    return hipErrorNotSupported;
+  int dev = 0, cus = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e != hipSuccess) return e;
+  // The workgroups resident at once (registers and LDS), a multiple of 8, no more than the
+  // tiles need.
+  int per_cu = 0;
+  e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(fn),
+                                                   64 * kWaves, 0);
+  if (e != hipSuccess) return e;
+  const uint64_t ntiles = p.padded_cells >> 4;
+  uint64_t grid = static_cast<uint64_t>(cus > 0 ? cus : 8) * static_cast<uint64_t>(per_cu > 0 ? per_cu : 1);
+  const uint64_t need = (ntiles + kWaves - 1) / kWaves;
+  const uint64_t need8 = (need + 7) / 8 * 8;
+  grid = (grid < need8 ? grid : need8) / 8 * 8;
+  if (grid < 8) grid = 8;
+  hipLaunchKernelGGL(fn, dim3(static_cast<uint32_t>(grid)), dim3(64 * kWaves), 0, stream, nodes,
+                     dofmap, taba, tab64, p, out);
+  return hipGetLastError();
+}
+
+}  // namespace tg

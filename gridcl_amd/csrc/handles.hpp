@@ -42,6 +42,7 @@ struct DeviceTables {
   uint32_t* sf_lex = nullptr;
   void* sf_mfma = nullptr;
   int sf_n1 = 0, sf_m = 0;
+  float* dense_mfma = nullptr;  // the table in matrix-core operand order (FP32 P3 tetrahedra)
   ~DeviceTables();
 };
 
@@ -106,9 +107,12 @@ int device_weights(tg_pointset* p, const double** out);
 // The same two lookups without the upload: TG_ERR_INVALID_ARGUMENT where nothing is resident.
 int resident_tables(tg_pointset* p, unsigned degree, int family, int dtype, DeviceTables** out);
 int resident_weights(const tg_pointset* p, const double** out);
+// Whether enqueue_block runs the block's factors on the dense matrix-core kernel.
+bool takes_dense_mfma(const tg_geometry* g, const Block& b, const DeviceTables* tabs,
+                      uint32_t factors, uint32_t tile, uint32_t options);
 // Enqueues every factor in `factors` of block b at pts (the dispatch of tg_eval_block: the
-// sum-factorised kernel where it applies, else the register-resident or runtime-K kernel).
-// Returns a TG status.
+// sum-factorised kernel where it applies, the dense matrix-core kernel for FP32 P3 tetrahedra,
+// else the register-resident or runtime-K kernel). Returns a TG status.
 template <typename T>
 int enqueue_block(const tg_geometry* g, const Block& b, tg_pointset* pts, DeviceTables* tabs,
                   uint32_t factors, uint32_t tile, uint32_t options, const struct tg_outputs* out,

@@ -270,9 +270,12 @@ int tg_plan_create(const tg_geometry* g, tg_pointset* const* pts, uint32_t facto
     PlanBlock pb{static_cast<int>(bi), p, nullptr, mask, outs[bi]};
     int rc = tg::device_tables(p, b.degree, b.family, g->dtype, &pb.tabs);
     if (rc) return rc;
-    // The persistent kernel: a shape of its list, tiles of 16, the table within its LDS budget.
+    // The persistent kernel: a shape of its list, tiles of 16, the table within its LDS budget;
+    // blocks that tg_eval_block runs on the dense matrix-core kernel keep to their own launch,
+    // so that the plan's fields equal it bit for bit.
     if (tg::hybrid_shape(b.tdim, g->gdim, b.k) < 0 || tile != 16 ||
-        p->nq * b.k * 4 * (g->dtype == TG_F32 ? 4 : 8) > tg::kMaxLdsTableBytes)
+        p->nq * b.k * 4 * (g->dtype == TG_F32 ? 4 : 8) > tg::kMaxLdsTableBytes ||
+        tg::takes_dense_mfma(g, b, pb.tabs, mask, tile, options))
       all_hybrid = false;
     plan->blocks.push_back(pb);
   }

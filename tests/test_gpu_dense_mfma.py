@@ -1,0 +1,135 @@
+"""P3 tetrahedra (20 nodes) in FP32 on the dense matrix-core kernel (dense_mfma.hip), on the GPU.
+
+The kernel takes every call for J on FP32 P3 tetrahedra with tiles of 16 and at most 128
+points. Each v_mfma_f32_16x16x4f32 step adds its four products to the accumulator in node order,
+each rounded once, so the sums over the 20 nodes are the register kernel's chain of fused
+multiply-adds from the first node: the values equal the register kernel's (TG_OPT_NO_MFMA) bit
+for bit, and both are held to the oracle with the tolerance for 20-node FP32 sums (4e-6, block
+rel_err, tests/test_gpu_parity.py). Calls that fall back (more than 128 points, X or det J
+alone, forced strategies) run the other kernels."""
+import numpy as np
+import pytest
+
+from gridcl_amd import api
+from gridcl_amd import meshes as M
+from tests.helpers import gpu_factors, oracle_factors, pointwise_rel, rel_errs
+from tests.test_gpu_parity import volume_mesh
+
+pytestmark = pytest.mark.gpu
+TET = M.TETRAHEDRON
+NAMES = ["points", "jacobians", "detj"]
+
+
+def _sets():
+    # Keast-4 (4 points), the 14-point rule, Gauss degrees 4, 6 and 8 (27, 64, 125 points: one
+    # to eight blocks of 16 points, the last one partial).
+    return [("keast4", api.PointSet.tet_keast4()), ("tet14", api.PointSet.tet_14()),
+            ("gauss4", api.PointSet.gauss(TET, 4)), ("gauss6", api.PointSet.gauss(TET, 6)),
+            ("gauss8", api.PointSet.gauss(TET, 8))]
+
+
+@pytest.mark.parametrize("idx", range(5))
+def test_against_oracle_and_register_kernel(cuda, idx):
+    import torch
+
+    name, ps = _sets()[idx]
+    mesh = volume_mesh(TET, 3)
+    g = api.geometry_from_host(mesh, dtype=torch.float32)
+    got = gpu_factors(g, 0, ps, NAMES)
+    reg = gpu_factors(g, 0, ps, NAMES, options=api.OPT_NO_MFMA)
+    want = oracle_factors(g, mesh, 0, ps, NAMES)
+    err, err_reg, diff = rel_errs(got, want), rel_errs(reg, want), rel_errs(got, reg)
+    print(f"P3 tet {name} ({len(ps)} points): matrix cores {err}, register kernel {err_reg}, "
+          f"between {diff}")
+    for n in NAMES:
+        assert err[n] <= 4e-6, (n, err[n])
+        assert np.array_equal(got[n], reg[n]), n
+
+
+@pytest.mark.parametrize("cells", [1, 17, 50])
+def test_ragged_blocks(cuda, cells):
+    """Blocks whose cell count is not a multiple of the 16-cell tile (and of the four tiles of a
+    workgroup): padding lanes repeat the last cell; every real cell against the oracle."""
+    import torch
+
+    mesh = volume_mesh(TET, 3)
+    b = mesh.blocks[0]
+    mesh.blocks[0] = M.HostBlock(b.cell, b.degree, b.cells[:cells].copy())
+    g = api.geometry_from_host(mesh, dtype=torch.float32)
+    ps = api.PointSet.gauss(TET, 6)
+    got = gpu_factors(g, 0, ps, NAMES)
+    want = oracle_factors(g, mesh, 0, ps, NAMES)
+    for n, e in rel_errs(got, want).items():
+        assert e <= 4e-6, (cells, n, e)
+
+
+def test_factor_subsets_and_fallbacks(cuda):
+    """J alone and J with X or det J run on the matrix cores and equal the full call bit for bit;
+    X alone, det J alone (the measure's call), more than 128 points (Gauss degree 10, 216) and
+    TG_OPT_COOPERATIVE / TG_OPT_FORCE_GENERIC keep their kernels."""
+    import torch
+
+    mesh = volume_mesh(TET, 3)
+    g = api.geometry_from_host(mesh, dtype=torch.float32)
+    ps = api.PointSet.gauss(TET, 6)
+    full = gpu_factors(g, 0, ps, NAMES)
+    for subset in (["jacobians"], ["points", "jacobians"], ["jacobians", "detj"]):
+        part = gpu_factors(g, 0, ps, subset)
+        for n in subset:
+            assert np.array_equal(part[n], full[n]), (subset, n)
+    reg = gpu_factors(g, 0, ps, NAMES, options=api.OPT_NO_MFMA)
+    for n in ("points", "detj"):
+        alone = gpu_factors(g, 0, ps, [n])[n]
+        assert np.array_equal(alone, reg[n]), n
+    big = api.PointSet.gauss(TET, 10)
+    assert len(big) > 128
+    assert all(np.array_equal(a, b) for a, b in zip(
+        gpu_factors(g, 0, big, NAMES).values(),
+        gpu_factors(g, 0, big, NAMES, options=api.OPT_NO_MFMA).values()))
+    for opt in (api.OPT_COOPERATIVE, api.OPT_FORCE_GENERIC):
+        forced = gpu_factors(g, 0, ps, NAMES, options=opt)
+        for n, e in rel_errs(forced, oracle_factors(g, mesh, 0, ps, NAMES)).items():
+            assert e <= 1e-5, (opt, n, e)
+
+
+def test_fp64_fixup_on_near_degenerate_cells(cuda):
+    """Near-degenerate points (|det J| < tau |J|_F^3) get det J in FP64 on the matrix-core path
+    too: on a box flattened in z by 1e-4 the pointwise det J matches the oracle to 1e-6 where the
+    FP32 value alone does not."""
+    import torch
+
+    mesh = volume_mesh(TET, 3, n=2)
+    x = mesh.nodes.copy()
+    x[:, 2] = 0.3 + 1e-4 * x[:, 2]
+    mesh.nodes = x
+    g = api.geometry_from_host(mesh, dtype=torch.float32)
+    ps = api.PointSet.tet_14()
+    want = oracle_factors(g, mesh, 0, ps, ["detj"])["detj"]
+    g.set_fixup_threshold(1e-2)
+    fixed = gpu_factors(g, 0, ps, NAMES)["detj"]
+    plain = gpu_factors(g, 0, ps, NAMES, options=api.OPT_NO_FP64_FIXUP)["detj"]
+    e_fixed, e_plain = pointwise_rel(fixed, want), pointwise_rel(plain, want)
+    print("P3 near-degenerate det J pointwise rel err: fixed", e_fixed, "fp32 only", e_plain)
+    assert e_fixed <= 1e-6
+    assert e_plain > e_fixed
+
+
+def test_plans_keep_the_block_on_its_kernel(cuda):
+    """A mesh plan over P3 tetrahedra runs the block's own launch (the persistent kernel is not
+    taken even when asked for), eagerly and from its graph, equal to tg_eval_block bit for bit."""
+    import torch
+
+    mesh = volume_mesh(TET, 3)
+    g = api.geometry_from_host(mesh, dtype=torch.float32)
+    rule = api.QuadRule.gauss([TET], 6)
+    single = gpu_factors(g, 0, rule.get(TET), NAMES)
+    for opt in (api.OPT_PLAN_PERSISTENT, api.OPT_PLAN_PER_BLOCK):
+        plan = api.MeshPlan(g, rule, NAMES, options=opt)
+        assert not plan.persistent
+        for launch in (plan.launch, plan.graph_launch):
+            for f in plan.fields[0].values():
+                f.data.fill_(float("nan"))
+            launch()
+            torch.cuda.synchronize()
+            for n in NAMES:
+                assert np.array_equal(plan.fields[0][n].to_host_cell_major(), single[n]), (opt, n)
