@@ -558,8 +558,54 @@ def bench_q3(api, M, steps, warmup, n=96):
     return rec
 
 
+def bench_p3tet(api, M, steps, warmup, n=48):
+    """Curved P3 tetrahedra (48^3 x 6 = 663,552 cells, 20 nodes per cell), X + J + det J in
+    FP32 at the 14-point rule and at Gauss degree 6 (64 points): the dense matrix-core kernel
+    (the default) and the register kernel (TG_OPT_NO_MFMA), whose results are equal bit for
+    bit, and the FP64 kernel; the FP32 result's error against the FP64 kernel over the mesh."""
+    import torch
+
+    h = M.p3tet_mesh(n=n)
+    names = ["points", "jacobians", "detj"]
+    C, N = h.blocks[0].num_cells, h.num_nodes
+    lib = api._lib.load()
+    rec = {"cells": C, "nodes": N}
+    g64 = api.geometry_from_host(h, dtype=torch.float64)
+    g = api.geometry_from_host(h, dtype=torch.float32)
+    for rule, ps in (("tet14", api.PointSet.tet_14()), ("gauss6", api.PointSet.gauss(4, 6))):
+        nq = len(ps)
+
+        def buffers(tdt):
+            return {nm: torch.empty(int(lib.tg_field_elems(C, nq, api.factor_len(nm, 3, 3), 16)),
+                                    dtype=tdt, device="cuda") for nm in names}
+
+        want = buffers(torch.float64)
+        ms64 = time_steps(lambda: api.geometric_factors_block(g64, 0, ps, names, out=want),
+                          max(2, steps // 2), 1)
+        out, reg = buffers(torch.float32), buffers(torch.float32)
+        nbytes = algorithmic_bytes(20, C, N, nq, 4)
+        r = {"points": nq}
+        for key, opt, dst in (("mfma", 0, out), ("valu", api.OPT_NO_MFMA, reg)):
+            step = lambda opt=opt, dst=dst: api.geometric_factors_block(g, 0, ps, names, out=dst,
+                                                                        options=opt)
+            spin_up(step, 0.3)
+            ms = time_steps(step, steps, warmup)
+            r[key] = {"ms": ms, "evals_per_s": C * nq / (ms * 1e-3),
+                      "achieved_gbs": nbytes / (ms * 1e-3) / 1e9,
+                      "frac": nbytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS}
+        torch.cuda.synchronize()
+        r["rel_err_vs_fp64_kernel"] = {nm: device_rel_err(out[nm], want[nm]) for nm in names}
+        r["mfma_equals_valu"] = all(torch.equal(out[nm], reg[nm]) for nm in names)
+        r["fp64"] = {"ms": ms64, "evals_per_s": C * nq / (ms64 * 1e-3)}
+        rec[rule] = r
+        del want, out, reg
+    del g64, g
+    return rec
+
+
 HEADLINE_SOURCES = ("factor_kernels.hip",)
-EXTRAS = ("config1", "adjoint", "hybrid", "q3", "pullback", "measure", "equispaced", "gmsh")
+EXTRAS = ("config1", "adjoint", "hybrid", "q3", "p3tet", "pullback", "measure", "equispaced",
+          "gmsh")
 
 
 def main() -> int:
@@ -673,6 +719,10 @@ def main() -> int:
     if "q3" in legs:
         extras["q3_hex_96"] = bench_q3(api, M, max(5, args.steps // 5), 2)
         log(f"q3: {extras['q3_hex_96']}")
+        torch.cuda.empty_cache()
+    if "p3tet" in legs:
+        extras["p3tet_48"] = bench_p3tet(api, M, args.steps, args.warmup)
+        log(f"p3 tetrahedra: {extras['p3tet_48']}")
         torch.cuda.empty_cache()
     if "pullback" in legs:
         extras["pull_back_p2tet_64"] = bench_pull_back(api, M, args.steps, args.warmup)

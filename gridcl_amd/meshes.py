@@ -265,6 +265,18 @@ def config5_mesh(seed: int = 1, n: int = 32) -> HostMesh:
     return jitter(m, 0.02 / n, seed + 100, interior_only=True)
 
 
+def p3tet_mesh(seed: int = 6, n: int = 48) -> HostMesh:
+    """Curved P3 tetrahedra: n^3 x 6, config 2's displacement field (amplitude 0.05h) and
+    uniform jitter +-0.02h of every node, at degree 3 (20 nodes per cell)."""
+    m = cube_box(TETRAHEDRON, n, 3)
+    h = 1.0 / n
+    x = m.nodes
+    s = np.sin(2.0 * np.pi * x)
+    disp = 0.05 * h * np.stack([s[:, 0] * s[:, 1], s[:, 1] * s[:, 2], s[:, 2] * s[:, 0]], axis=1)
+    m = HostMesh(gdim=3, nodes=x + disp, blocks=m.blocks)
+    return jitter(m, 0.02 * h, seed, interior_only=False)
+
+
 HYBRID_NOMINAL = [
     # (cell, degree, nominal cells)
     (TETRAHEDRON, 2, 600_000),

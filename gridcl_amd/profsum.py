@@ -1,15 +1,19 @@
 """Short text summaries of rocprofv3 CSV output, for the committed profile summaries.
 
     python -m gridcl_amd.profsum stats  <prefix>_kernel_stats.csv [--top N]
-    python -m gridcl_amd.profsum pmc    <prefix>_counter_collection.csv [--match SUBSTRING]
+    python -m gridcl_amd.profsum pmc    <prefix>_counter_collection.csv [--match SUBSTRING] [--runs]
     python -m gridcl_amd.profsum trace  <prefix>_kernel_trace.csv [--match SUBSTRING] [--last N]
+                                        [--runs]
     python -m gridcl_amd.profsum digest <kernel source> [...]
 
 ``stats`` prints calls and average/min/max duration per kernel; ``pmc`` prints, per kernel,
 counter and grid size, the number of dispatches and the mean, min and max counter value per
 dispatch (FETCH_SIZE and WRITE_SIZE are in KiB); ``trace`` prints, per kernel and grid size, the
 dispatch count and the average, min and max duration, optionally of the last N dispatches only
-(the timed steps of a run); ``digest`` prints the content digest of kernel sources and the local
+(the timed steps of a run); with ``--runs`` both split each kernel and grid size into runs of
+back-to-back dispatches (a dispatch of any other kernel or grid ends a run), numbered from 0 in
+dispatch order, so that legs of one program that launch the same kernel at the same grid (one
+point set after another) are told apart; ``digest`` prints the content digest of kernel sources and the local
 headers they include, which each profile summary records so that a reader (and the benchmark,
 for the traffic figure) can tell whether the profiled kernel is still the one in the tree.
 """
@@ -61,12 +65,31 @@ def stats(path: str, top: int) -> None:
               f"{float(r['MaxNs']) / 1e3:>9.1f} {float(r['Percentage']):>6.2f}")
 
 
-def pmc(path: str, match: str | None, drop: int = 0) -> None:
+def run_numbers(rows, key) -> dict:
+    """Dispatch id -> the number of its run among the runs of its key (kernel and grid): the
+    rows in dispatch order, a run ending where a dispatch of another key comes between."""
+    order = sorted({(int(r["Dispatch_Id"]), key(r)) for r in rows})
+    runs, out, prev = defaultdict(int), {}, None
+    for did, k in order:
+        if prev is not None and k != prev and k in runs:
+            runs[k] += 1
+        runs.setdefault(k, 0)
+        out[did] = runs[k]
+        prev = k
+    return out
+
+
+def pmc(path: str, match: str | None, drop: int = 0, by_run: bool = False) -> None:
+    rows = list(csv.DictReader(open(path)))
+    run = run_numbers(rows, lambda r: (r["Kernel_Name"], r["Grid_Size"])) if by_run else {}
     acc = defaultdict(list)  # (kernel, counter, grid size in work-items) -> values
-    for r in csv.DictReader(open(path)):
+    for r in rows:
         if match and match not in r["Kernel_Name"]:
             continue
-        key = (short_name(r["Kernel_Name"], 72), r["Counter_Name"], r["Grid_Size"])
+        name = short_name(r["Kernel_Name"], 72 - 6 * by_run)
+        if by_run:
+            name = f"{name} run {run[int(r['Dispatch_Id'])]}"
+        key = (name, r["Counter_Name"], r["Grid_Size"])
         acc[key].append(float(r["Counter_Value"]))
     print(f"{'kernel':<72} {'counter':<11} {'n':>5} {'grid':>9} {'mean':>14} {'min':>14} "
           f"{'max':>14}")
@@ -77,14 +100,19 @@ def pmc(path: str, match: str | None, drop: int = 0) -> None:
               f"{sum(v) / len(v):>14.1f} {min(v):>14.1f} {max(v):>14.1f}")
 
 
-def trace(path: str, match: str | None, last: int, drop: int = 0) -> None:
+def trace(path: str, match: str | None, last: int, drop: int = 0, by_run: bool = False) -> None:
+    rows = list(csv.DictReader(open(path)))
+    grid_of = lambda r: int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) * int(r["Grid_Size_Z"])
+    run = run_numbers(rows, lambda r: (r["Kernel_Name"], grid_of(r))) if by_run else {}
+    rows.sort(key=lambda r: int(r["Dispatch_Id"]))
     acc = defaultdict(list)  # (kernel, grid size in work-items) -> durations in ns
-    for r in csv.DictReader(open(path)):
+    for r in rows:
         if match and match not in r["Kernel_Name"]:
             continue
-        grid = int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) * int(r["Grid_Size_Z"])
-        acc[(short_name(r["Kernel_Name"], 72), grid)].append(
-            int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+        name = short_name(r["Kernel_Name"], 72 - 6 * by_run)
+        if by_run:
+            name = f"{name} run {run[int(r['Dispatch_Id'])]}"
+        acc[(name, grid_of(r))].append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
     print(f"{'kernel':<72} {'grid':>10} {'n':>6} {'avg us':>9} {'min us':>9} {'max us':>9}")
     for (name, grid), v in sorted(acc.items(), key=lambda kv: -sum(kv[1])):
         if drop:
@@ -106,12 +134,16 @@ def main(argv=None) -> int:
     b.add_argument("--match")
     b.add_argument("--drop", type=int, default=0,
                    help="leave out the last DROP dispatches of each (kernel, counter, grid)")
+    b.add_argument("--runs", action="store_true",
+                   help="split each (kernel, grid) into runs of back-to-back dispatches")
     c = sub.add_parser("trace")
     c.add_argument("csv")
     c.add_argument("--match")
     c.add_argument("--last", type=int, default=0)
     c.add_argument("--drop", type=int, default=0,
                    help="leave out the last DROP dispatches of each (kernel, grid) first")
+    c.add_argument("--runs", action="store_true",
+                   help="split each (kernel, grid) into runs of back-to-back dispatches")
     d = sub.add_parser("digest")
     d.add_argument("sources", nargs="+")
     args = ap.parse_args(argv)
@@ -120,9 +152,9 @@ def main(argv=None) -> int:
     elif args.cmd == "stats":
         stats(args.csv, args.top)
     elif args.cmd == "pmc":
-        pmc(args.csv, args.match, args.drop)
+        pmc(args.csv, args.match, args.drop, args.runs)
     else:
-        trace(args.csv, args.match, args.last, args.drop)
+        trace(args.csv, args.match, args.last, args.drop, args.runs)
     return 0
 
 

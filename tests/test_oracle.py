@@ -44,6 +44,20 @@ def test_unit_volume(cell, p, deg):
     assert abs((d @ ps.weights).sum() - 1.0) <= 1e-13
 
 
+def test_p3_tetrahedron_mesh_is_valid():
+    """The curved P3 mesh of the P3 benchmark leg: 6 n^3 cells of 20 nodes, det J > 0 at every
+    Gauss point of degree 6, and a volume within what the jitter of the boundary nodes (+-0.02h
+    on each of the six faces) can move from the unit cube's."""
+    mesh = M.p3tet_mesh(n=3)
+    blk = mesh.blocks[0]
+    assert blk.cells.shape == (6 * 27, 20) and blk.degree == 3
+    ps = PointSet.gauss(TET, 6)
+    d = O.eval_block(TET, 3, 0, 3, mesh.nodes, blk.cells.reshape(-1), ps.points,
+                     factors=("detj",))["detj"][:, :, 0]
+    assert d.min() > 0
+    assert abs((d @ ps.weights).sum() - 1.0) <= 6 * 0.02 / 3
+
+
 def test_tiled_cpu_path_matches_cell_major():
     mesh = M.config2_mesh(n=3)
     blk = mesh.blocks[0]
