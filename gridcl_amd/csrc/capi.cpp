@@ -422,15 +422,15 @@ static bool takes_sumfact(const tg_geometry* g, const DeviceTables* tabs, uint32
 
 // Whether the factors of a block go to the dense matrix-core kernel: FP32 with the cell shift on
 // FP32 nodes, P3 tetrahedra (the point set's operand-order table exists), tiles of 16, calls
-// that ask for J (X and det J alone keep the register kernels, as the measure does), unless a
-// strategy is forced. Measured on MI355X, X + J + det J, 663,552 cells: 14 points 0.121 against
+// that ask for J (X alone and det J alone keep the register kernels' lane-per-cell mappings, and
+// the measure its register kernel), unless a strategy is forced. Measured on MI355X, X + J + det J, 663,552 cells: 14 points 0.121 against
 // 0.150 ms on the register kernel, 64 points 0.413 against 0.503 ms; P2 tetrahedra were slower
 // on it (0.212 against 0.197 ms at 14 points) and keep the register kernel.
 bool takes_dense_mfma(const tg_geometry* g, const Block& b, const DeviceTables* tabs,
                       uint32_t factors, uint32_t tile, uint32_t options) {
   const bool shift = g->dtype == TG_F32 && !(options & TG_OPT_NO_CELL_SHIFT);
-  return shift && !g->nodes64 && tabs->dense_mfma && tile == 16 && b.tdim == 3 &&
-         g->gdim == 3 && (factors & TG_JACOBIAN) &&
+  return shift && !g->nodes64 && tabs->dense_mfma && tile == 16 &&
+         tg::has_dense_mfma(b.tdim, g->gdim, b.k) && (factors & TG_JACOBIAN) &&
          !(factors & ~(TG_POINTS | TG_JACOBIAN | TG_DETJ)) &&
          !(options & (TG_OPT_NO_MFMA | TG_OPT_FORCE_GENERIC | TG_OPT_COOPERATIVE));
 }
