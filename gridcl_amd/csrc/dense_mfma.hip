@@ -6,18 +6,18 @@
 //
 //   D_v[q][cell] = sum_k T_v[q][k] x_g[k][cell]      v = 0 (phi), 1..3 (d phi / d xi_d)
 //
-// One wavefront per tile of 16 cells. Its lanes gather the tile's nodes into LDS, four lanes
-// per cell, as offsets from each cell's node 0 (node 0 itself stays absolute and X adds it
-// back, as every FP32 kernel here). The B operand of lane l is the offset of node 4s + l / 16
-// of cell l % 16 (zero for node 0 and past k), held in registers for the whole tile; the A
+// One wavefront per tile of 16 cells. Its lanes gather the tile's nodes, four lanes per cell:
+// lane l loads nodes l / 16, l / 16 + 4, ... of cell l % 16, which are its B operands (node
+// 4s + l / 16 at step s), formed in registers as offsets from each cell's node 0 (zero for node
+// 0 and past k; node 0 itself stays absolute and X adds it back, as every FP32 kernel here) and
+// held for the whole tile; the A
 // operand is the table value T_v[q = 16 qb + l % 16][k = 4s + l / 16], laid out on the host in
 // operand order (one coalesced load per product from the L1-resident table). Each block of 16
 // points takes 4 kinds x 3 coordinates x ceil(k / 4) products, and lane l receives the points
 // 16 qb + 4 (l / 16) + i (i = 0..3, one per register) of cell l % 16: X and all of J of its
 // points, so it forms det J (with the FP64 recomputation where |det J| < tau |J|_F^3). The
-// block's rows of the tile are contiguous in each field: they are staged in LDS (over the
-// gathered nodes, dead once the operands are in registers) and leave as 16-byte non-temporal
-// stores.
+// block's rows of the tile are contiguous in each field: they are staged in LDS and leave as
+// 16-byte non-temporal stores.
 //
 // Each step adds its four products to the accumulator in node order, each rounded once, as the
 // register kernels' chain of fused multiply-adds from node 0 does: the values equal theirs bit
@@ -50,12 +50,9 @@ __device__ __forceinline__ gptr<T> gp(T* p) {
   return (gptr<T>)p;
 }
 
-// Elements of a wavefront's LDS: its gathered nodes [k][3][16 cells], then, over them, the
-// stage of one block of 16 points: X [16][3][16], J [16][9][16], det J [16][16].
-template <int K>
-constexpr int dense_wave_elems() {
-  return K * 3 * 16 > 13 * 256 ? K * 3 * 16 : 13 * 256;
-}
+// Elements of a wavefront's LDS: the stage of one block of 16 points, X [16][3][16],
+// J [16][9][16], det J [16][16].
+constexpr int kStageElems = 13 * 256;
 
 // Copies n values (a multiple of 4) from LDS to global memory with 16-byte non-temporal stores
 // spread over the wavefront's lanes.
@@ -73,15 +70,13 @@ __global__ void __launch_bounds__(64 * kWaves)
                const float* __restrict__ taba, const double* __restrict__ tab64,
                FactorParams<float> p, FactorOutputs o) {
   constexpr int S = (K + 3) / 4;  // node steps of four (and gather rounds of four lanes a cell)
-  constexpr int W = dense_wave_elems<K>();
-  __shared__ __align__(16) float lds[kWaves * W];
+  __shared__ __align__(16) float lds[kWaves * kStageElems];
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint32_t nqb = (p.nq + 15u) >> 4;
-  float* __restrict__ xs = lds + w * W;
-  float* __restrict__ sx = xs;
-  float* __restrict__ sj = xs + 3 * 256;
-  float* __restrict__ sd = xs + 12 * 256;
+  float* __restrict__ sx = lds + w * kStageElems;
+  float* __restrict__ sj = sx + 3 * 256;
+  float* __restrict__ sd = sx + 12 * 256;
 
   // Tiles: XCD x = blockIdx.x % 8 takes the contiguous range [lo, hi) of the block's tiles; its
   // grid / 8 workgroups take four consecutive tiles (one per wavefront) at a time, in turn.
@@ -110,33 +105,24 @@ __global__ void __launch_bounds__(64 * kWaves)
 #pragma unroll
       for (int g = 0; g < 3; ++g) x0[g] = gp(nodes)[static_cast<uint64_t>(n0) * 3 + g];
     }
+    // B operands, formed in registers: the node lane (cell, kg) loads at step s is node
+    // 4 s + kg, whose offset from node 0 is the operand (zero for node 0 and past k).
     float b[S][3];
 #pragma unroll
-    for (int i = 0; i < S; ++i) {
-      const uint32_t kk = kg + 4u * i;
-      const uint32_t r = gp(row)[kk < static_cast<uint32_t>(K) ? kk : K - 1];
+    for (int s = 0; s < S; ++s) {
+      const uint32_t k = 4u * s + kg;
+      const uint32_t r = gp(row)[k < static_cast<uint32_t>(K) ? k : K - 1];
       const uint32_t n = r < p.last_node ? r : p.last_node;
 #pragma unroll
-      for (int g = 0; g < 3; ++g) b[i][g] = gp(nodes)[static_cast<uint64_t>(n) * 3 + g];
+      for (int g = 0; g < 3; ++g) b[s][g] = gp(nodes)[static_cast<uint64_t>(n) * 3 + g];
     }
-    sf::wave_sync();  // the previous tile's reads of the stage are done
-#pragma unroll
-    for (int i = 0; i < S; ++i) {
-      const uint32_t kk = kg + 4u * i;
-      if (kk < static_cast<uint32_t>(K)) {
-#pragma unroll
-        for (int g = 0; g < 3; ++g) xs[(kk * 3 + g) * 16 + cell] = kk > 0 ? b[i][g] - x0[g] : 0.f;
-      }
-    }
-    sf::wave_sync();
-    // B operands: the offset of node 4s + kg of cell `cell` (zero for node 0 and past k).
 #pragma unroll
     for (int s = 0; s < S; ++s) {
       const uint32_t k = 4u * s + kg;
 #pragma unroll
-      for (int g = 0; g < 3; ++g) b[s][g] = k < static_cast<uint32_t>(K) ? xs[(k * 3 + g) * 16 + cell] : 0.f;
+      for (int g = 0; g < 3; ++g)
+        b[s][g] = (k > 0 && k < static_cast<uint32_t>(K)) ? b[s][g] - x0[g] : 0.f;
     }
-    sf::wave_sync();  // the nodes are in registers; the stage may overwrite them
     for (uint32_t qb = 0; qb < nqb; ++qb) {
       const v4f zero = {0.f, 0.f, 0.f, 0.f};
       v4f acc[4][3];
