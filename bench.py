@@ -562,7 +562,8 @@ def bench_p3tet(api, M, steps, warmup, n=48):
     """Curved P3 tetrahedra (48^3 x 6 = 663,552 cells, 20 nodes per cell), X + J + det J in
     FP32 at the 14-point rule and at Gauss degree 6 (64 points): the dense matrix-core kernel
     (the default) and the register kernel (TG_OPT_NO_MFMA), whose results are equal bit for
-    bit, and the FP64 kernel; the FP32 result's error against the FP64 kernel over the mesh."""
+    bit, on FP32 nodes and on FP64-stored nodes, and the FP64 kernel; the FP32 results' error
+    against the FP64 kernel over the mesh."""
     import torch
 
     h = M.p3tet_mesh(n=n)
@@ -572,6 +573,7 @@ def bench_p3tet(api, M, steps, warmup, n=48):
     rec = {"cells": C, "nodes": N}
     g64 = api.geometry_from_host(h, dtype=torch.float64)
     g = api.geometry_from_host(h, dtype=torch.float32)
+    gn = api.geometry_from_host(h, dtype=torch.float32, nodes_f64=True)
     for rule, ps in (("tet14", api.PointSet.tet_14()), ("gauss6", api.PointSet.gauss(4, 6))):
         nq = len(ps)
 
@@ -596,10 +598,25 @@ def bench_p3tet(api, M, steps, warmup, n=48):
         torch.cuda.synchronize()
         r["rel_err_vs_fp64_kernel"] = {nm: device_rel_err(out[nm], want[nm]) for nm in names}
         r["mfma_equals_valu"] = all(torch.equal(out[nm], reg[nm]) for nm in names)
+        # FP32 on FP64-stored nodes (offsets from node 0 in FP64, rounded once): the matrix
+        # cores and the register kernel, against the FP64 kernel on the same mesh.
+        nb64 = nbytes + 3 * 4 * N
+        for key, opt, dst in (("fp32_nodes64", 0, out), ("fp32_nodes64_valu", api.OPT_NO_MFMA,
+                                                          reg)):
+            step = lambda opt=opt, dst=dst: api.geometric_factors_block(gn, 0, ps, names,
+                                                                        out=dst, options=opt)
+            spin_up(step, 0.3)
+            ms = time_steps(step, steps, warmup)
+            r[key] = {"ms": ms, "evals_per_s": C * nq / (ms * 1e-3),
+                      "frac": nb64 / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS}
+        torch.cuda.synchronize()
+        r["fp32_nodes64"]["rel_err_vs_fp64_kernel"] = {nm: device_rel_err(out[nm], want[nm])
+                                                       for nm in names}
+        r["fp32_nodes64"]["equals_valu"] = all(torch.equal(out[nm], reg[nm]) for nm in names)
         r["fp64"] = {"ms": ms64, "evals_per_s": C * nq / (ms64 * 1e-3)}
         rec[rule] = r
         del want, out, reg
-    del g64, g
+    del g64, g, gn
     return rec
 
 

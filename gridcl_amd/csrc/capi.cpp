@@ -420,8 +420,8 @@ static bool takes_sumfact(const tg_geometry* g, const DeviceTables* tabs, uint32
          ((options & TG_OPT_SUMFACT_ALWAYS) || sumfact_pays(f32, tabs->sf_n1, tabs->sf_m));
 }
 
-// Whether the factors of a block go to the dense matrix-core kernel: FP32 with the cell shift on
-// FP32 nodes, P3 tetrahedra (the point set's operand-order table exists), tiles of 16, calls
+// Whether the factors of a block go to the dense matrix-core kernel: FP32 with the cell shift (on
+// FP32 nodes or the FP64 copy), P3 tetrahedra (the point set's operand-order table exists), tiles of 16, calls
 // that ask for J (X alone and det J alone keep the register kernels' lane-per-cell mappings, and
 // the measure its register kernel), unless a strategy is forced. Measured on MI355X, X + J + det J, 663,552 cells: 14 points 0.121 against
 // 0.150 ms on the register kernel, 64 points 0.413 against 0.503 ms; P2 tetrahedra were slower
@@ -429,7 +429,7 @@ static bool takes_sumfact(const tg_geometry* g, const DeviceTables* tabs, uint32
 bool takes_dense_mfma(const tg_geometry* g, const Block& b, const DeviceTables* tabs,
                       uint32_t factors, uint32_t tile, uint32_t options) {
   const bool shift = g->dtype == TG_F32 && !(options & TG_OPT_NO_CELL_SHIFT);
-  return shift && !g->nodes64 && tabs->dense_mfma && tile == 16 &&
+  return shift && tabs->dense_mfma && tile == 16 &&
          tg::has_dense_mfma(b.tdim, g->gdim, b.k) && (factors & TG_JACOBIAN) &&
          !(factors & ~(TG_POINTS | TG_JACOBIAN | TG_DETJ)) &&
          !(options & (TG_OPT_NO_MFMA | TG_OPT_FORCE_GENERIC | TG_OPT_COOPERATIVE));
@@ -485,8 +485,11 @@ int enqueue_block(const tg_geometry* g, const Block& b, tg_pointset* pts, Device
   }
   if constexpr (sizeof(T) == 4) {
     if (takes_dense_mfma(g, b, tabs, factors, tile, options)) {
-      hipError_t e = tg::launch_dense_mfma(b.k, static_cast<const float*>(g->nodes), b.dofmap,
-                                           tabs->dense_mfma, tabs->tab64, p, o, stream);
+      hipError_t e = n64 ? tg::launch_dense_mfma_f64_nodes(b.k, n64, b.dofmap, tabs->dense_mfma,
+                                                           tabs->tab64, p, o, stream)
+                         : tg::launch_dense_mfma(b.k, static_cast<const float*>(g->nodes),
+                                                 b.dofmap, tabs->dense_mfma, tabs->tab64, p, o,
+                                                 stream);
       if (e == hipSuccess) return TG_OK;
       if (e != hipErrorNotSupported) return hip_fail(e, "dense matrix-core launch");
     }

@@ -24,7 +24,9 @@
 // for bit (measured on every point set of the parity tests and on 663,552 cells at 14 and 64
 // points), and both are held to the oracle.
 // Workgroups are persistent: those of one XCD walk one contiguous range of tiles, four tiles
-// (one per wavefront) at a time.
+// (one per wavefront) at a time. On FP64-stored nodes (TN = double) the offsets are taken in
+// FP64 and rounded once to FP32, and the FP64 recomputation reads the FP64 nodes, as the
+// register kernels on FP64 nodes do; the products are the same FP32 products.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -64,9 +66,9 @@ __device__ __forceinline__ void flush(const float* __restrict__ src, gptr<float>
   }
 }
 
-template <int K>
+template <int K, typename TN>
 __global__ void __launch_bounds__(64 * kWaves)
-    dense_mfma(const float* __restrict__ nodes, const uint32_t* __restrict__ dofmap,
+    dense_mfma(const TN* __restrict__ nodes, const uint32_t* __restrict__ dofmap,
                const float* __restrict__ taba, const double* __restrict__ tab64,
                FactorParams<float> p, FactorOutputs o) {
   constexpr int S = (K + 3) / 4;  // node steps of four (and gather rounds of four lanes a cell)
@@ -98,30 +100,37 @@ __global__ void __launch_bounds__(64 * kWaves)
     const uint32_t* __restrict__ row = dofmap + c * K;
     // Gather: lane (cell, kg) loads nodes kg, kg + 4, ... of its cell (past k: node k - 1
     // again, so the loads go together) and node 0.
+    TN x0n[3];
     float x0[3];
     {
       const uint32_t r0 = gp(row)[0];
       const uint32_t n0 = r0 < p.last_node ? r0 : p.last_node;
 #pragma unroll
-      for (int g = 0; g < 3; ++g) x0[g] = gp(nodes)[static_cast<uint64_t>(n0) * 3 + g];
+      for (int g = 0; g < 3; ++g) {
+        x0n[g] = gp(nodes)[static_cast<uint64_t>(n0) * 3 + g];
+        x0[g] = static_cast<float>(x0n[g]);
+      }
     }
     // B operands, formed in registers: the node lane (cell, kg) loads at step s is node
-    // 4 s + kg, whose offset from node 0 is the operand (zero for node 0 and past k).
-    float b[S][3];
+    // 4 s + kg, whose offset from node 0 (taken in the nodes' type, then rounded to FP32) is the
+    // operand (zero for node 0 and past k).
+    TN v[S][3];
 #pragma unroll
     for (int s = 0; s < S; ++s) {
       const uint32_t k = 4u * s + kg;
       const uint32_t r = gp(row)[k < static_cast<uint32_t>(K) ? k : K - 1];
       const uint32_t n = r < p.last_node ? r : p.last_node;
 #pragma unroll
-      for (int g = 0; g < 3; ++g) b[s][g] = gp(nodes)[static_cast<uint64_t>(n) * 3 + g];
+      for (int g = 0; g < 3; ++g) v[s][g] = gp(nodes)[static_cast<uint64_t>(n) * 3 + g];
     }
+    float b[S][3];
 #pragma unroll
     for (int s = 0; s < S; ++s) {
       const uint32_t k = 4u * s + kg;
 #pragma unroll
       for (int g = 0; g < 3; ++g)
-        b[s][g] = (k > 0 && k < static_cast<uint32_t>(K)) ? b[s][g] - x0[g] : 0.f;
+        b[s][g] = (k > 0 && k < static_cast<uint32_t>(K)) ? static_cast<float>(v[s][g] - x0n[g])
+                                                          : 0.f;
     }
     for (uint32_t qb = 0; qb < nqb; ++qb) {
       const v4f zero = {0.f, 0.f, 0.f, 0.f};
@@ -203,33 +212,25 @@ __global__ void __launch_bounds__(64 * kWaves)
   }
 }
 
-using DenseFn = void (*)(const float*, const uint32_t*, const float*, const double*,
+template <typename TN>
+using DenseFn = void (*)(const TN*, const uint32_t*, const float*, const double*,
                          FactorParams<float>, FactorOutputs);
 
-DenseFn pick(uint32_t k) {
+template <typename TN>
+DenseFn<TN> pick(uint32_t k) {
   switch (k) {
     case 20:
-      return dense_mfma<20>;
+      return dense_mfma<20, TN>;
     default:
       return nullptr;
   }
 }
 
-}  // namespace
-
-bool has_dense_mfma(int tdim, int gdim, uint32_t k) {
-  return tdim == 3 && gdim == 3 && pick(k) != nullptr;
-}
-
-size_t dense_mfma_table_elems(uint32_t k, uint32_t nq) {
-  const uint32_t S = (k + 3) / 4, nqb = (nq + 15) / 16;
-  return static_cast<size_t>(nqb) * 4 * S * 64;
-}
-
-hipError_t launch_dense_mfma(uint32_t k, const float* nodes, const uint32_t* dofmap,
-                             const float* taba, const double* tab64, const FactorParams<float>& p,
-                             const FactorOutputs& out, hipStream_t stream) {
-  DenseFn fn = pick(k);
+template <typename TN>
+hipError_t launch(uint32_t k, const TN* nodes, const uint32_t* dofmap, const float* taba,
+                  const double* tab64, const FactorParams<float>& p, const FactorOutputs& out,
+                  hipStream_t stream) {
+  DenseFn<TN> fn = pick<TN>(k);
   if (!fn || p.tile_log2 != 4 || p.nq == 0 || p.nq > kDenseMfmaMaxPoints)
     return hipErrorNotSupported;
   int dev = 0, cus = 0;
@@ -251,6 +252,30 @@ hipError_t launch_dense_mfma(uint32_t k, const float* nodes, const uint32_t* dof
   hipLaunchKernelGGL(fn, dim3(static_cast<uint32_t>(grid)), dim3(64 * kWaves), 0, stream, nodes,
                      dofmap, taba, tab64, p, out);
   return hipGetLastError();
+}
+
+}  // namespace
+
+bool has_dense_mfma(int tdim, int gdim, uint32_t k) {
+  return tdim == 3 && gdim == 3 && pick<float>(k) != nullptr;
+}
+
+size_t dense_mfma_table_elems(uint32_t k, uint32_t nq) {
+  const uint32_t S = (k + 3) / 4, nqb = (nq + 15) / 16;
+  return static_cast<size_t>(nqb) * 4 * S * 64;
+}
+
+hipError_t launch_dense_mfma(uint32_t k, const float* nodes, const uint32_t* dofmap,
+                             const float* taba, const double* tab64, const FactorParams<float>& p,
+                             const FactorOutputs& out, hipStream_t stream) {
+  return launch<float>(k, nodes, dofmap, taba, tab64, p, out, stream);
+}
+
+hipError_t launch_dense_mfma_f64_nodes(uint32_t k, const double* nodes, const uint32_t* dofmap,
+                                       const float* taba, const double* tab64,
+                                       const FactorParams<float>& p, const FactorOutputs& out,
+                                       hipStream_t stream) {
+  return launch<double>(k, nodes, dofmap, taba, tab64, p, out, stream);
 }
 
 }  // namespace tg

@@ -179,6 +179,12 @@ size_t dense_mfma_table_elems(uint32_t k, uint32_t nq);
 hipError_t launch_dense_mfma(uint32_t k, const float* nodes, const uint32_t* dofmap,
                              const float* taba, const double* tab64, const FactorParams<float>& p,
                              const FactorOutputs& out, hipStream_t stream);
+// The same on FP64-stored nodes: each offset from node 0 taken in FP64 and rounded to FP32 (as
+// the register kernels on FP64 nodes take it), the FP64 recomputation from the FP64 nodes.
+hipError_t launch_dense_mfma_f64_nodes(uint32_t k, const double* nodes, const uint32_t* dofmap,
+                                       const float* taba, const double* tab64,
+                                       const FactorParams<float>& p, const FactorOutputs& out,
+                                       hipStream_t stream);
 
 // The cooperative strategy (k up to 256 nodes): L lanes per cell, L the largest power of two not
 // above min(32, k), leaves added by a pairwise tree. hipErrorNotSupported for other shapes.

@@ -169,3 +169,29 @@ def test_full_size_mesh(cuda):
         for n, e in err.items():
             assert e <= 4e-6, (name, n, e)
         assert float(np.min(want["detj"])) > 0
+
+
+@pytest.mark.parametrize("idx", [1, 3])
+def test_fp64_stored_nodes(cuda, idx):
+    """FP32 P3 tetrahedra on FP64-stored nodes take the matrix-core kernel too (each offset from
+    node 0 taken in FP64 and rounded once to FP32): equal bit for bit to the register kernel on
+    the same nodes (TG_OPT_NO_MFMA), within 4e-6 of the exact mesh (the oracle on the FP64 nodes;
+    the tolerance of the 20-term FP32 sums on FP64 nodes, tests/test_gpu_nodes64.py), and not the
+    values of the FP32-node geometry."""
+    import torch
+
+    from tests.test_gpu_nodes64 import exact_oracle
+
+    name, ps = _sets()[idx]
+    mesh = volume_mesh(TET, 3)
+    g = api.geometry_from_host(mesh, dtype=torch.float32, nodes_f64=True)
+    got = gpu_factors(g, 0, ps, NAMES)
+    reg = gpu_factors(g, 0, ps, NAMES, options=api.OPT_NO_MFMA)
+    for n in NAMES:
+        assert np.array_equal(got[n], reg[n]), n
+    err = rel_errs(got, exact_oracle(mesh, ps, NAMES))
+    print(f"P3 tet {name} on FP64 nodes: {err}")
+    for n in NAMES:
+        assert err[n] <= 4e-6, (n, err[n])
+    f32 = gpu_factors(api.geometry_from_host(mesh, dtype=torch.float32), 0, ps, NAMES)
+    assert not np.array_equal(got["jacobians"], f32["jacobians"])
