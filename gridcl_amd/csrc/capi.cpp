@@ -420,19 +420,25 @@ static bool takes_sumfact(const tg_geometry* g, const DeviceTables* tabs, uint32
          ((options & TG_OPT_SUMFACT_ALWAYS) || sumfact_pays(f32, tabs->sf_n1, tabs->sf_m));
 }
 
-// Whether the factors of a block go to the dense matrix-core kernel: FP32 with the cell shift (on
-// FP32 nodes or the FP64 copy), P3 tetrahedra (the point set's operand-order table exists), tiles of 16, calls
-// that ask for J (X alone and det J alone keep the register kernels' lane-per-cell mappings, and
-// the measure its register kernel), unless a strategy is forced. Measured on MI355X, X + J + det J, 663,552 cells: 14 points 0.121 against
-// 0.150 ms on the register kernel, 64 points 0.413 against 0.503 ms; P2 tetrahedra were slower
-// on it (0.212 against 0.197 ms at 14 points) and keep the register kernel.
+// Whether the factors of a block go to the dense matrix-core kernel (dense_mfma.hip): FP32 with
+// the cell shift (on FP32 nodes or the FP64 copy), a 3D cell of a node count it has (20 to 84;
+// the point set's operand-order table exists), tiles of 16, X, J and det J only, a block the
+// sum-factorised kernels do not take, unless a strategy is forced. Calls without J (X alone,
+// det J alone: the measure's and the equispaced conversion's) take it only where the cell has no
+// register-resident kernel, whose lane-per-cell mappings serve them. Measured on MI355X, X + J +
+// det J: P3 tetrahedra on 663,552 cells 0.121 against 0.150 ms at 14 points, 0.413 against 0.503
+// ms at 64; the cells without register kernels 7-51x faster than the runtime-K and cooperative
+// kernels (dense_mfma.hip lists them).
 bool takes_dense_mfma(const tg_geometry* g, const Block& b, const DeviceTables* tabs,
                       uint32_t factors, uint32_t tile, uint32_t options) {
   const bool shift = g->dtype == TG_F32 && !(options & TG_OPT_NO_CELL_SHIFT);
-  return shift && tabs->dense_mfma && tile == 16 &&
-         tg::has_dense_mfma(b.tdim, g->gdim, b.k) && (factors & TG_JACOBIAN) &&
-         !(factors & ~(TG_POINTS | TG_JACOBIAN | TG_DETJ)) &&
-         !(options & (TG_OPT_NO_MFMA | TG_OPT_FORCE_GENERIC | TG_OPT_COOPERATIVE));
+  const uint32_t xjd = TG_POINTS | TG_JACOBIAN | TG_DETJ;
+  if (!shift || !tabs->dense_mfma || tile != 16 || !tg::has_dense_mfma(b.tdim, g->gdim, b.k) ||
+      !(factors & xjd) || (factors & ~xjd) ||
+      (options & (TG_OPT_NO_MFMA | TG_OPT_FORCE_GENERIC | TG_OPT_COOPERATIVE)))
+    return false;
+  if (takes_sumfact(g, tabs, factors, tile, options, true)) return false;
+  return (factors & TG_JACOBIAN) || !tg::has_direct_factor<float>(b.tdim, g->gdim, b.k, true);
 }
 
 template <typename T>

@@ -1,5 +1,7 @@
-// X, J and det J of FP32 cells of dense bases (P3 tetrahedra: 20 nodes) with the node sums on
-// the matrix cores (v_mfma_f32_16x16x4f32). Same factors as the register-resident kernels
+// X, J and det J of FP32 3D cells of dense bases (20 to 84 nodes: P3 to P6 tetrahedra, P3
+// pyramids, P3 and P4 prisms, and Q2 and Q3 hexahedra at point sets the sum-factorised kernels
+// do not take) with the node sums on the matrix cores (v_mfma_f32_16x16x4f32). Same factors as
+// the register-resident and runtime-K kernels
 // (the reference's factor kernel, tetraloma-geometry/src/layout.rs:138-213: X_g = sum_k x_kg
 // phi_k, J_gd = sum_k x_kg d_d phi_k, det J by first-row cofactors), with the sums taken as
 // matrix products over the cells of a tile:
@@ -20,9 +22,10 @@
 // 16-byte non-temporal stores.
 //
 // Each step adds its four products to the accumulator in node order, each rounded once, as the
-// register kernels' chain of fused multiply-adds from node 0 does: the values equal theirs bit
-// for bit (measured on every point set of the parity tests and on 663,552 cells at 14 and 64
-// points), and both are held to the oracle.
+// register and runtime-K kernels' chain of fused multiply-adds from node 0 does: the values
+// equal theirs bit for bit (measured on every shape and point set of the parity tests and on
+// 663,552 P3 tetrahedra at 14 and 64 points), and all are held to the oracle. (The cooperative
+// kernel adds its leaves by a tree and differs in the last bits.)
 // Workgroups are persistent: those of one XCD walk one contiguous range of tiles, four tiles
 // (one per wavefront) at a time. On FP64-stored nodes (TN = double) the offsets are taken in
 // FP64 and rounded once to FP32, and the FP64 recomputation reads the FP64 nodes, as the
@@ -145,6 +148,7 @@ __global__ void __launch_bounds__(64 * kWaves)
 #pragma unroll
         for (int vv = 0; vv < 4; ++vv) {
           if (vv == 0 && !(mask & kFPoints)) continue;
+          if (vv > 0 && !(mask & (kFJac | kFDetJ))) continue;
           const float a = ta[(vv * S + s) * 64];
 #pragma unroll
           for (int g = 0; g < 3; ++g) acc[vv][g] = mfma4(a, b[s][g], acc[vv][g]);
@@ -216,11 +220,33 @@ template <typename TN>
 using DenseFn = void (*)(const TN*, const uint32_t*, const float*, const double*,
                          FactorParams<float>, FactorOutputs);
 
+// 3D cells of 20 to 84 nodes. Measured on MI355X against the kernels they replace (X + J + det
+// J, FP32): P3 tetrahedra 1.25x (the register kernel), Q2 hexahedra 1.11x and Q3 hexahedra 2.9x
+// (the register kernels, where the sum-factorised ones do not apply), P3 pyramids 10x, P4
+// tetrahedra 12x, P3 prisms 51x (the runtime-K kernel), P5 and P6 tetrahedra 14x and 9x, P4
+// prisms 7.7x (the cooperative kernel). P2 tetrahedra (10 nodes), P2 pyramids (14) and P2
+// prisms (18) measured slower here than on their register kernels and keep them.
 template <typename TN>
 DenseFn<TN> pick(uint32_t k) {
   switch (k) {
-    case 20:
+    case 20:  // P3 tetrahedra
       return dense_mfma<20, TN>;
+    case 27:  // Q2 hexahedra (point sets the sum-factorised kernels do not take)
+      return dense_mfma<27, TN>;
+    case 30:  // P3 pyramids
+      return dense_mfma<30, TN>;
+    case 35:  // P4 tetrahedra
+      return dense_mfma<35, TN>;
+    case 40:  // P3 prisms
+      return dense_mfma<40, TN>;
+    case 56:  // P5 tetrahedra
+      return dense_mfma<56, TN>;
+    case 64:  // Q3 hexahedra (as for Q2)
+      return dense_mfma<64, TN>;
+    case 75:  // P4 prisms
+      return dense_mfma<75, TN>;
+    case 84:  // P6 tetrahedra
+      return dense_mfma<84, TN>;
     default:
       return nullptr;
   }

@@ -195,3 +195,63 @@ def test_fp64_stored_nodes(cuda, idx):
         assert err[n] <= 4e-6, (n, err[n])
     f32 = gpu_factors(api.geometry_from_host(mesh, dtype=torch.float32), 0, ps, NAMES)
     assert not np.array_equal(got["jacobians"], f32["jacobians"])
+
+
+PYR, PRISM, HEX = M.PYRAMID, M.PRISM, M.HEXAHEDRON
+
+
+def _scattered(cell, n, seed=3):
+    """n seeded points in the unit cube (no tensor structure, so the sum-factorised kernels do
+    not take them), equal weights."""
+    pts = M.splitmix64_uniform(seed, 3 * n).reshape(n, 3)
+    return api.PointSet.custom(cell, pts, np.full(n, 1.0 / n))
+
+
+# (label, cell, degree, point set, whether a register-resident kernel exists)
+SHAPES = [("P3 tet", TET, 3, lambda: api.PointSet.gauss(TET, 4), True),
+          ("Q2 hex", HEX, 2, lambda: _scattered(HEX, 40), True),
+          ("P3 pyramid", PYR, 3, lambda: api.PointSet.gauss(PYR, 4), False),
+          ("P4 tet", TET, 4, lambda: api.PointSet.gauss(TET, 5), False),
+          ("P3 prism", PRISM, 3, lambda: api.PointSet.gauss(PRISM, 5), False),
+          ("P5 tet", TET, 5, lambda: api.PointSet.gauss(TET, 6), False),
+          ("Q3 hex", HEX, 3, lambda: _scattered(HEX, 100), True),
+          ("P4 prism", PRISM, 4, lambda: api.PointSet.gauss(PRISM, 5), False),
+          ("P6 tet", TET, 6, lambda: api.PointSet.gauss(TET, 7), False)]
+
+
+@pytest.mark.parametrize("idx", range(len(SHAPES)))
+def test_every_shape_against_oracle_and_node_order_kernels(cuda, idx):
+    """Every cell the kernel takes (20 to 84 nodes): X, J and det J equal the runtime-K kernel's
+    (TG_OPT_FORCE_GENERIC: node order from node 0) bit for bit, and the register kernel's where
+    the cell has one (TG_OPT_NO_MFMA); against the oracle within 4e-6 where a register kernel
+    exists (the dense FP32 sums above), the reference's f32 base tolerance 1e-5 otherwise, and
+    its scaled tolerances for the tetrahedra of degree 5 and 6 (tests/test_gpu_coop.py). X alone
+    and det J alone equal the full call."""
+    import torch
+
+    from tests.test_gpu_coop import scaled_tolerances
+
+    label, cell, p, make, has_reg = SHAPES[idx]
+    ps = make()
+    assert len(ps) <= 128
+    mesh = volume_mesh(cell, p, n=2)
+    g = api.geometry_from_host(mesh, dtype=torch.float32)
+    got = gpu_factors(g, 0, ps, NAMES)
+    gen = gpu_factors(g, 0, ps, NAMES, options=api.OPT_FORCE_GENERIC)
+    for n in NAMES:
+        assert np.array_equal(got[n], gen[n]), (label, n)
+    if has_reg:
+        reg = gpu_factors(g, 0, ps, NAMES, options=api.OPT_NO_MFMA)
+        for n in NAMES:
+            assert np.array_equal(got[n], reg[n]), (label, n)
+    want = oracle_factors(g, mesh, 0, ps, NAMES)
+    err = rel_errs(got, want)
+    if cell == TET and p >= 5:
+        tol = scaled_tolerances(TET, p, ps.points, 1e-5, want["jacobians"])
+    else:
+        tol = {n: 4e-6 if has_reg else 1e-5 for n in NAMES}
+    print(f"{label} ({mesh.blocks[0].nodes_per_cell} nodes, {len(ps)} points): {err}")
+    for n in NAMES:
+        assert err[n] <= tol[n], (label, n, err[n], tol[n])
+    for n in ("points", "detj"):
+        assert np.array_equal(gpu_factors(g, 0, ps, [n])[n], got[n]), (label, n)
