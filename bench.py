@@ -620,9 +620,56 @@ def bench_p3tet(api, M, steps, warmup, n=48):
     return rec
 
 
+def bench_dense_shapes(api, M, steps, warmup):
+    """The other 3D cells the dense matrix-core kernel takes (X + J + det J, FP32, jittered unit
+    boxes of 65,536-524,288 cells): its time per launch against the kernel the cell runs on
+    without it (TG_OPT_NO_MFMA: the register kernel of Q2/Q3 hexahedra at a point set without
+    tensor structure, the runtime-K kernel of P3 pyramids, P4 tetrahedra and P3 prisms, the
+    cooperative kernel of P5/P6 tetrahedra and P4 prisms), and whether its values equal the
+    runtime-K kernel's (TG_OPT_FORCE_GENERIC) bit for bit."""
+    import torch
+
+    names = ["points", "jacobians", "detj"]
+    hexpts = lambda n: api.PointSet.custom(
+        M.HEXAHEDRON, M.splitmix64_uniform(3, 3 * n).reshape(n, 3), np.full(n, 1.0 / n))
+    cases = [("q2hex_80", M.HEXAHEDRON, 2, 80, lambda: hexpts(40)),
+             ("p3pyr_24", M.PYRAMID, 3, 24, lambda: api.PointSet.gauss(M.PYRAMID, 4)),
+             ("p4tet_24", M.TETRAHEDRON, 4, 24, lambda: api.PointSet.gauss(M.TETRAHEDRON, 5)),
+             ("p3prism_48", M.PRISM, 3, 48, lambda: api.PointSet.gauss(M.PRISM, 5)),
+             ("p5tet_20", M.TETRAHEDRON, 5, 20, lambda: api.PointSet.gauss(M.TETRAHEDRON, 6)),
+             ("q3hex_64", M.HEXAHEDRON, 3, 64, lambda: hexpts(100)),
+             ("p4prism_32", M.PRISM, 4, 32, lambda: api.PointSet.gauss(M.PRISM, 5)),
+             ("p6tet_16", M.TETRAHEDRON, 6, 16, lambda: api.PointSet.gauss(M.TETRAHEDRON, 7))]
+    out = {}
+    for key, cell, p, n, make in cases:
+        mesh = M.jitter(M.cube_box(cell, n, p), 0.02 / (n * p), 5, interior_only=True)
+        g = api.geometry_from_host(mesh, dtype=torch.float32)
+        ps = make()
+        C, N, k, nq = (mesh.blocks[0].num_cells, mesh.num_nodes, mesh.blocks[0].nodes_per_cell,
+                       len(ps))
+        a = {nm: f.data for nm, f in api.geometric_factors_block(g, 0, ps, names).items()}
+        b = {nm: f.data for nm, f in api.geometric_factors_block(
+            g, 0, ps, names, options=api.OPT_FORCE_GENERIC).items()}
+        torch.cuda.synchronize()
+        same = all(torch.equal(a[nm], b[nm]) for nm in names)
+        nbytes = algorithmic_bytes(k, C, N, nq, 4)
+        rec = {"cells": C, "nodes_per_cell": k, "points": nq, "equals_runtime_k": same}
+        for label, opt, dst in (("mfma", 0, a), ("without", api.OPT_NO_MFMA, b)):
+            step = lambda opt=opt, dst=dst: api.geometric_factors_block(g, 0, ps, names, out=dst,
+                                                                        options=opt)
+            spin_up(step, 0.2)
+            ms = time_steps(step, steps, warmup)
+            rec[label] = {"ms": ms, "frac": nbytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS}
+        rec["speedup"] = rec["without"]["ms"] / rec["mfma"]["ms"]
+        out[key] = rec
+        del g, a, b
+        torch.cuda.empty_cache()
+    return out
+
+
 HEADLINE_SOURCES = ("factor_kernels.hip",)
-EXTRAS = ("config1", "adjoint", "hybrid", "q3", "p3tet", "pullback", "measure", "equispaced",
-          "gmsh")
+EXTRAS = ("config1", "adjoint", "hybrid", "q3", "p3tet", "dense", "pullback", "measure",
+          "equispaced", "gmsh")
 
 
 def main() -> int:
@@ -740,6 +787,10 @@ def main() -> int:
     if "p3tet" in legs:
         extras["p3tet_48"] = bench_p3tet(api, M, args.steps, args.warmup)
         log(f"p3 tetrahedra: {extras['p3tet_48']}")
+        torch.cuda.empty_cache()
+    if "dense" in legs:
+        extras["dense_shapes"] = bench_dense_shapes(api, M, max(5, args.steps // 5), 2)
+        log(f"dense shapes: {extras['dense_shapes']}")
         torch.cuda.empty_cache()
     if "pullback" in legs:
         extras["pull_back_p2tet_64"] = bench_pull_back(api, M, args.steps, args.warmup)
