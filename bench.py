@@ -624,9 +624,9 @@ def bench_dense_shapes(api, M, steps, warmup):
     """The other 3D cells the dense matrix-core kernel takes (X + J + det J, FP32, jittered unit
     boxes of 65,536-524,288 cells): its time per launch against the kernel the cell runs on
     without it (TG_OPT_NO_MFMA: the register kernel of Q2/Q3 hexahedra at a point set without
-    tensor structure, the runtime-K kernel of P3 pyramids, P4 tetrahedra and P3 prisms, the
-    cooperative kernel of P5/P6 tetrahedra and P4 prisms), and whether its values equal the
-    runtime-K kernel's (TG_OPT_FORCE_GENERIC) bit for bit."""
+    tensor structure, the runtime-K kernel of P3/P4 pyramids, P4 tetrahedra, P3 prisms and Q4
+    hexahedra, the cooperative kernel of P5-P8 tetrahedra and P4/P5 prisms), and whether its
+    values equal the runtime-K kernel's (TG_OPT_FORCE_GENERIC) bit for bit."""
     import torch
 
     names = ["points", "jacobians", "detj"]
@@ -639,7 +639,12 @@ def bench_dense_shapes(api, M, steps, warmup):
              ("p5tet_20", M.TETRAHEDRON, 5, 20, lambda: api.PointSet.gauss(M.TETRAHEDRON, 6)),
              ("q3hex_64", M.HEXAHEDRON, 3, 64, lambda: hexpts(100)),
              ("p4prism_32", M.PRISM, 4, 32, lambda: api.PointSet.gauss(M.PRISM, 5)),
-             ("p6tet_16", M.TETRAHEDRON, 6, 16, lambda: api.PointSet.gauss(M.TETRAHEDRON, 7))]
+             ("p6tet_16", M.TETRAHEDRON, 6, 16, lambda: api.PointSet.gauss(M.TETRAHEDRON, 7)),
+             ("p4pyr_16", M.PYRAMID, 4, 16, lambda: api.PointSet.gauss(M.PYRAMID, 5)),
+             ("p7tet_14", M.TETRAHEDRON, 7, 14, lambda: api.PointSet.gauss(M.TETRAHEDRON, 8)),
+             ("q4hex_32", M.HEXAHEDRON, 4, 32, lambda: hexpts(64)),
+             ("p5prism_20", M.PRISM, 5, 20, lambda: api.PointSet.gauss(M.PRISM, 6)),
+             ("p8tet_12", M.TETRAHEDRON, 8, 12, lambda: api.PointSet.gauss(M.TETRAHEDRON, 9))]
     out = {}
     for key, cell, p, n, make in cases:
         mesh = M.jitter(M.cube_box(cell, n, p), 0.02 / (n * p), 5, interior_only=True)

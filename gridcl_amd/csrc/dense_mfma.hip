@@ -69,6 +69,108 @@ __device__ __forceinline__ void flush(const float* __restrict__ src, gptr<float>
   }
 }
 
+// FP64 det J of a cell of k nodes at point q from the stored coordinates (of type TN; offsets
+// from node 0 in FP64), for near-degenerate points: the register and runtime-K kernels' sums, in
+// node order.
+template <typename TN>
+__device__ __noinline__ double detj_fp64_k(const TN* __restrict__ nodes,
+                                           const uint32_t* __restrict__ row, uint32_t k,
+                                           uint32_t last_node, const double* __restrict__ tab64,
+                                           uint32_t q) {
+  double x0[3], j[9];
+  const uint32_t n0 = row[0] < last_node ? row[0] : last_node;
+#pragma unroll
+  for (int g = 0; g < 3; ++g) x0[g] = static_cast<double>(nodes[static_cast<uint64_t>(n0) * 3 + g]);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) j[i] = 0.0;
+  for (uint32_t kk = 1; kk < k; ++kk) {
+    const uint32_t n = row[kk] < last_node ? row[kk] : last_node;
+    const double* t = tab64 + (static_cast<uint64_t>(q) * k + kk) * 4;
+#pragma unroll
+    for (int g = 0; g < 3; ++g) {
+      const double d = static_cast<double>(nodes[static_cast<uint64_t>(n) * 3 + g]) - x0[g];
+#pragma unroll
+      for (int dd = 0; dd < 3; ++dd) j[dd * 3 + g] = __builtin_fma(d, t[1 + dd], j[dd * 3 + g]);
+    }
+  }
+  return j[0] * (j[4] * j[8] - j[7] * j[5]) - j[3] * (j[1] * j[8] - j[7] * j[2]) +
+         j[6] * (j[1] * j[5] - j[4] * j[2]);
+}
+
+// The end of one block of 16 points of a tile, from the products acc[v][g] (v = 0: X offsets,
+// 1..3: J(g, d = v - 1)); lane (cell, kg) = (l % 16, l / 16) holds the rows qq = 4 kg + i of
+// the block (points 16 qb + qq) of its cell. det J of every point first; X and J go to the
+// stage, then the FP64 recomputation of flagged points runs with the accumulators dead; the
+// block's rows leave as contiguous runs of each field.
+template <typename TN>
+__device__ __forceinline__ void finish_block(const v4f (&acc)[4][3], const float (&x0)[3],
+                                             const TN* __restrict__ nodes,
+                                             const uint32_t* __restrict__ row, uint32_t k,
+                                             const double* __restrict__ tab64,
+                                             const FactorParams<float>& p, uint64_t t,
+                                             uint32_t qb, float* __restrict__ sx,
+                                             float* __restrict__ sj, float* __restrict__ sd,
+                                             gptr<float> ox, gptr<float> oj, gptr<float> od,
+                                             uint32_t lane) {
+  const uint32_t cell = lane & 15u, kg = lane >> 4;
+  const uint32_t mask = p.mask;
+  const uint32_t nv = p.nq - qb * 16u < 16u ? p.nq - qb * 16u : 16u;  // rows in the block
+  float det[4];
+  uint32_t near = 0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+#pragma clang fp contract(off)
+    float jm[3][3];  // jm[d][g] = J(g, d)
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int g = 0; g < 3; ++g) jm[d][g] = acc[1 + d][g][i];
+    det[i] = jm[0][0] * (jm[1][1] * jm[2][2] - jm[2][1] * jm[1][2]) -
+             jm[1][0] * (jm[0][1] * jm[2][2] - jm[2][1] * jm[0][2]) +
+             jm[2][0] * (jm[0][1] * jm[1][2] - jm[1][1] * jm[0][2]);
+    if (p.fixup) {
+      float n2 = 0.f;
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int g = 0; g < 3; ++g) n2 = __builtin_fmaf(jm[d][g], jm[d][g], n2);
+      if (__builtin_fabsf(det[i]) < static_cast<float>(p.tau) * n2 * __builtin_sqrtf(n2))
+        near |= 1u << i;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const uint32_t qq = kg * 4u + static_cast<uint32_t>(i);
+    if (mask & kFPoints) {
+#pragma unroll
+      for (int g = 0; g < 3; ++g) sx[(qq * 3 + g) * 16 + cell] = (x0[g] + acc[0][g][i]) + p.origin[g];
+    }
+    if (mask & kFJac) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+#pragma unroll
+        for (int g = 0; g < 3; ++g) sj[(qq * 9 + d * 3 + g) * 16 + cell] = acc[1 + d][g][i];
+    }
+  }
+  if (mask & kFDetJ) {
+    if (near) {  // rare
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        if (near & (1u << i))
+          det[i] = static_cast<float>(
+              detj_fp64_k(nodes, row, k, p.last_node, tab64, qb * 16u + kg * 4u + i));
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) sd[(kg * 4u + i) * 16 + cell] = det[i];
+  }
+  sf::wave_sync();
+  const uint64_t r0 = t * p.nq + qb * 16u;  // the block's first row of the tile
+  if (mask & kFPoints) flush(sx, ox + ((r0 * 3) << 4), nv * 48u, lane);
+  if (mask & kFJac) flush(sj, oj + ((r0 * 9) << 4), nv * 144u, lane);
+  if (mask & kFDetJ) flush(sd, od + (r0 << 4), nv * 16u, lane);
+  sf::wave_sync();  // the stage is read; the next block may overwrite it
+}
+
 template <int K, typename TN>
 __global__ void __launch_bounds__(64 * kWaves)
     dense_mfma(const TN* __restrict__ nodes, const uint32_t* __restrict__ dofmap,
@@ -154,65 +256,98 @@ __global__ void __launch_bounds__(64 * kWaves)
           for (int g = 0; g < 3; ++g) acc[vv][g] = mfma4(a, b[s][g], acc[vv][g]);
         }
       }
-      // Lane: rows qq = 4 kg + i of the block (points 16 qb + qq) of cell `cell`;
-      // J(g, d) = acc[1 + d][g]. det J of every point first; X and J go to the stage, then the
-      // FP64 recomputation of flagged points runs with the accumulators dead.
-      const uint32_t nv = p.nq - qb * 16u < 16u ? p.nq - qb * 16u : 16u;  // rows in the block
-      float det[4];
-      uint32_t near = 0;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-#pragma clang fp contract(off)
-        float jm[3][3];  // jm[d][g] = J(g, d)
-#pragma unroll
-        for (int d = 0; d < 3; ++d)
-#pragma unroll
-          for (int g = 0; g < 3; ++g) jm[d][g] = acc[1 + d][g][i];
-        det[i] = jm[0][0] * (jm[1][1] * jm[2][2] - jm[2][1] * jm[1][2]) -
-                 jm[1][0] * (jm[0][1] * jm[2][2] - jm[2][1] * jm[0][2]) +
-                 jm[2][0] * (jm[0][1] * jm[1][2] - jm[1][1] * jm[0][2]);
-        if (p.fixup) {
-          float n2 = 0.f;
-#pragma unroll
-          for (int d = 0; d < 3; ++d)
-#pragma unroll
-            for (int g = 0; g < 3; ++g) n2 = __builtin_fmaf(jm[d][g], jm[d][g], n2);
-          if (__builtin_fabsf(det[i]) < static_cast<float>(p.tau) * n2 * __builtin_sqrtf(n2))
-            near |= 1u << i;
-        }
-      }
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const uint32_t qq = kg * 4u + static_cast<uint32_t>(i);
-        if (mask & kFPoints) {
-#pragma unroll
-          for (int g = 0; g < 3; ++g) sx[(qq * 3 + g) * 16 + cell] = (x0[g] + acc[0][g][i]) + p.origin[g];
-        }
-        if (mask & kFJac) {
-#pragma unroll
-          for (int d = 0; d < 3; ++d)
-#pragma unroll
-            for (int g = 0; g < 3; ++g) sj[(qq * 9 + d * 3 + g) * 16 + cell] = acc[1 + d][g][i];
-        }
-      }
-      if (mask & kFDetJ) {
-        if (near) {  // rare
-#pragma unroll
-          for (int i = 0; i < 4; ++i)
-            if (near & (1u << i))
-              det[i] = static_cast<float>(sf::detj_fp64<K>(nodes, row, p.last_node, tab64,
-                                                           qb * 16u + kg * 4u + i));
-        }
-#pragma unroll
-        for (int i = 0; i < 4; ++i) sd[(kg * 4u + i) * 16 + cell] = det[i];
-      }
-      sf::wave_sync();
-      const uint64_t r0 = t * p.nq + qb * 16u;  // the block's first row of the tile
-      if (mask & kFPoints) flush(sx, ox + ((r0 * 3) << 4), nv * 48u, lane);
-      if (mask & kFJac) flush(sj, oj + ((r0 * 9) << 4), nv * 144u, lane);
-      if (mask & kFDetJ) flush(sd, od + (r0 << 4), nv * 16u, lane);
-      sf::wave_sync();  // the stage is read; the next block may overwrite it
+      finish_block(acc, x0, nodes, row, K, tab64, p, t, qb, sx, sj, sd, ox, oj, od, lane);
     }
+  }
+}
+
+// Cells of more nodes than the register operands above hold (runtime k): the same products, with
+// the B operands gathered in chunks of kChunk steps for each block of 16 points, the
+// accumulators carried across the chunks (the same node order). The work items are (tile, block
+// of 16 points), so that the few tiles of high-degree meshes spread over more wavefronts; the
+// XCDs take contiguous ranges of them, a tile's blocks in turn.
+constexpr int kChunk = 8;
+
+template <typename TN>
+__global__ void __launch_bounds__(64 * kWaves)
+    dense_mfma_rt(const TN* __restrict__ nodes, const uint32_t* __restrict__ dofmap,
+                  const float* __restrict__ taba, const double* __restrict__ tab64,
+                  FactorParams<float> p, FactorOutputs o) {
+  __shared__ __align__(16) float lds[kWaves * kStageElems];
+  const uint32_t K = p.k, S = (K + 3) / 4;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const uint32_t nqb = (p.nq + 15u) >> 4;
+  float* __restrict__ sx = lds + w * kStageElems;
+  float* __restrict__ sj = sx + 3 * 256;
+  float* __restrict__ sd = sx + 12 * 256;
+  const uint64_t nitems = (p.padded_cells >> 4) * nqb;
+  const uint32_t nx = gridDim.x >> 3, xcd = blockIdx.x & 7u, j = blockIdx.x >> 3;
+  const uint64_t per = nitems >> 3, rem = nitems & 7u;
+  const uint64_t lo = xcd * per + (xcd < rem ? xcd : rem);
+  const uint64_t hi = lo + per + (xcd < rem ? 1u : 0u);
+  const uint32_t cell = lane & 15u, kg = lane >> 4;
+  const uint32_t mask = p.mask;
+  gptr<float> ox = gp(static_cast<float*>(o.x));
+  gptr<float> oj = gp(static_cast<float*>(o.j));
+  gptr<float> od = gp(static_cast<float*>(o.detj));
+  for (uint64_t it = lo + (static_cast<uint64_t>(j) * kWaves + w); it < hi;
+       it += static_cast<uint64_t>(nx) * kWaves) {
+    const uint64_t t = it / nqb;
+    const uint32_t qb = static_cast<uint32_t>(it - t * nqb);
+    const uint64_t c0 = t * 16 + cell;
+    const uint64_t c = c0 < p.num_cells ? c0 : p.num_cells - 1;
+    const uint32_t* __restrict__ row = dofmap + c * K;
+    TN x0n[3];
+    float x0[3];
+    {
+      const uint32_t r0 = gp(row)[0];
+      const uint32_t n0 = r0 < p.last_node ? r0 : p.last_node;
+#pragma unroll
+      for (int g = 0; g < 3; ++g) {
+        x0n[g] = gp(nodes)[static_cast<uint64_t>(n0) * 3 + g];
+        x0[g] = static_cast<float>(x0n[g]);
+      }
+    }
+    const v4f zero = {0.f, 0.f, 0.f, 0.f};
+    v4f acc[4][3];
+#pragma unroll
+    for (int vv = 0; vv < 4; ++vv)
+#pragma unroll
+      for (int g = 0; g < 3; ++g) acc[vv][g] = zero;
+    gptr<const float> ta = gp(taba) + qb * (4u * S * 64u) + lane;
+    for (uint32_t s0 = 0; s0 < S; s0 += kChunk) {
+      TN v[kChunk][3];
+#pragma unroll
+      for (int i = 0; i < kChunk; ++i) {
+        const uint32_t k = 4u * (s0 + i) + kg;
+        const uint32_t r = gp(row)[k < K ? k : K - 1];
+        const uint32_t n = r < p.last_node ? r : p.last_node;
+#pragma unroll
+        for (int g = 0; g < 3; ++g) v[i][g] = gp(nodes)[static_cast<uint64_t>(n) * 3 + g];
+      }
+      float b[kChunk][3];
+#pragma unroll
+      for (int i = 0; i < kChunk; ++i) {
+        const uint32_t k = 4u * (s0 + i) + kg;
+#pragma unroll
+        for (int g = 0; g < 3; ++g)
+          b[i][g] = (k > 0 && k < K) ? static_cast<float>(v[i][g] - x0n[g]) : 0.f;
+      }
+#pragma unroll
+      for (int i = 0; i < kChunk; ++i) {
+        if (s0 + i >= S) break;  // uniform
+#pragma unroll
+        for (int vv = 0; vv < 4; ++vv) {
+          if (vv == 0 && !(mask & kFPoints)) continue;
+          if (vv > 0 && !(mask & (kFJac | kFDetJ))) continue;
+          const float a = ta[(vv * S + s0 + i) * 64];
+#pragma unroll
+          for (int g = 0; g < 3; ++g) acc[vv][g] = mfma4(a, b[i][g], acc[vv][g]);
+        }
+      }
+    }
+    finish_block(acc, x0, nodes, row, K, tab64, p, t, qb, sx, sj, sd, ox, oj, od, lane);
   }
 }
 
@@ -252,12 +387,18 @@ DenseFn<TN> pick(uint32_t k) {
   }
 }
 
+// Node counts from 30 not listed above (P4 pyramids and the cells of more than 84 nodes) take
+// the runtime-k form.
+constexpr uint32_t kRtMinNodes = 30;
+
 template <typename TN>
 hipError_t launch(uint32_t k, const TN* nodes, const uint32_t* dofmap, const float* taba,
                   const double* tab64, const FactorParams<float>& p, const FactorOutputs& out,
                   hipStream_t stream) {
   DenseFn<TN> fn = pick<TN>(k);
-  if (!fn || p.tile_log2 != 4 || p.nq == 0 || p.nq > kDenseMfmaMaxPoints)
+  const bool rt = fn == nullptr && k >= kRtMinNodes;
+  if (rt) fn = dense_mfma_rt<TN>;
+  if (!fn || p.tile_log2 != 4 || p.nq == 0 || p.nq > kDenseMfmaMaxPoints || p.k != k)
     return hipErrorNotSupported;
   int dev = 0, cus = 0;
   hipError_t e = hipGetDevice(&dev);
@@ -269,9 +410,10 @@ hipError_t launch(uint32_t k, const TN* nodes, const uint32_t* dofmap, const flo
   e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(fn),
                                                    64 * kWaves, 0);
   if (e != hipSuccess) return e;
-  const uint64_t ntiles = p.padded_cells >> 4;
+  // Work items: tiles, or (tile, block of 16 points) for the runtime-k form.
+  const uint64_t items = (p.padded_cells >> 4) * (rt ? (p.nq + 15) / 16 : 1);
   uint64_t grid = static_cast<uint64_t>(cus > 0 ? cus : 8) * static_cast<uint64_t>(per_cu > 0 ? per_cu : 1);
-  const uint64_t need = (ntiles + kWaves - 1) / kWaves;
+  const uint64_t need = (items + kWaves - 1) / kWaves;
   const uint64_t need8 = (need + 7) / 8 * 8;
   grid = (grid < need8 ? grid : need8) / 8 * 8;
   if (grid < 8) grid = 8;
@@ -283,7 +425,7 @@ hipError_t launch(uint32_t k, const TN* nodes, const uint32_t* dofmap, const flo
 }  // namespace
 
 bool has_dense_mfma(int tdim, int gdim, uint32_t k) {
-  return tdim == 3 && gdim == 3 && pick<float>(k) != nullptr;
+  return tdim == 3 && gdim == 3 && (pick<float>(k) != nullptr || k >= kRtMinNodes);
 }
 
 size_t dense_mfma_table_elems(uint32_t k, uint32_t nq) {

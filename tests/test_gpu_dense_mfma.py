@@ -216,17 +216,24 @@ SHAPES = [("P3 tet", TET, 3, lambda: api.PointSet.gauss(TET, 4), True),
           ("P5 tet", TET, 5, lambda: api.PointSet.gauss(TET, 6), False),
           ("Q3 hex", HEX, 3, lambda: _scattered(HEX, 100), True),
           ("P4 prism", PRISM, 4, lambda: api.PointSet.gauss(PRISM, 5), False),
-          ("P6 tet", TET, 6, lambda: api.PointSet.gauss(TET, 7), False)]
+          ("P6 tet", TET, 6, lambda: api.PointSet.gauss(TET, 7), False),
+          # the runtime-k form: more than 84 nodes, or a count the register-operand form lacks
+          ("P4 pyramid", PYR, 4, lambda: api.PointSet.gauss(PYR, 5), False),
+          ("P7 tet", TET, 7, lambda: api.PointSet.gauss(TET, 8), False),
+          ("Q4 hex", HEX, 4, lambda: _scattered(HEX, 64), False),
+          ("P5 prism", PRISM, 5, lambda: api.PointSet.gauss(PRISM, 6), False),
+          ("P8 tet", TET, 8, lambda: api.PointSet.gauss(TET, 9), False)]
 
 
 @pytest.mark.parametrize("idx", range(len(SHAPES)))
 def test_every_shape_against_oracle_and_node_order_kernels(cuda, idx):
-    """Every cell the kernel takes (20 to 84 nodes): X, J and det J equal the runtime-K kernel's
-    (TG_OPT_FORCE_GENERIC: node order from node 0) bit for bit, and the register kernel's where
-    the cell has one (TG_OPT_NO_MFMA); against the oracle within 4e-6 where a register kernel
-    exists (the dense FP32 sums above), the reference's f32 base tolerance 1e-5 otherwise, and
-    its scaled tolerances for the tetrahedra of degree 5 and 6 (tests/test_gpu_coop.py). X alone
-    and det J alone equal the full call."""
+    """Every cell the kernel takes (20 to 165 nodes; the runtime-k form from P4 pyramids and
+    above 84 nodes): X, J and det J equal the runtime-K kernel's (TG_OPT_FORCE_GENERIC: node
+    order from node 0) bit for bit, and the register kernel's where the cell has one
+    (TG_OPT_NO_MFMA); against the oracle within 4e-6 where a register kernel exists (the dense
+    FP32 sums above), otherwise within the reference's scaled tolerances on its f32 base 1e-5
+    (tests/test_gpu_coop.py; the base itself up to the degree-4 element). X alone and det J alone
+    equal the full call."""
     import torch
 
     from tests.test_gpu_coop import scaled_tolerances
@@ -246,10 +253,10 @@ def test_every_shape_against_oracle_and_node_order_kernels(cuda, idx):
             assert np.array_equal(got[n], reg[n]), (label, n)
     want = oracle_factors(g, mesh, 0, ps, NAMES)
     err = rel_errs(got, want)
-    if cell == TET and p >= 5:
-        tol = scaled_tolerances(TET, p, ps.points, 1e-5, want["jacobians"])
+    if has_reg:
+        tol = {n: 4e-6 for n in NAMES}
     else:
-        tol = {n: 4e-6 if has_reg else 1e-5 for n in NAMES}
+        tol = scaled_tolerances(cell, p, ps.points, 1e-5, want["jacobians"])
     print(f"{label} ({mesh.blocks[0].nodes_per_cell} nodes, {len(ps)} points): {err}")
     for n in NAMES:
         assert err[n] <= tol[n], (label, n, err[n], tol[n])
