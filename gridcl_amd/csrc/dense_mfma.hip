@@ -55,9 +55,9 @@ __device__ __forceinline__ gptr<T> gp(T* p) {
   return (gptr<T>)p;
 }
 
-// Elements of a wavefront's LDS: the stage of one block of 16 points, X [16][3][16],
-// J [16][9][16], det J [16][16].
-constexpr int kStageElems = 13 * 256;
+// Elements of a wavefront's LDS: the stage of one block of 16 points, J [16][9][16], then, over
+// it once J is stored, X [16][3][16] and det J [16][16] (36 KiB a workgroup: four per CU).
+constexpr int kStageElems = 9 * 256;
 
 // Copies n values (a multiple of 4) from LDS to global memory with 16-byte non-temporal stores
 // spread over the wavefront's lanes.
@@ -138,18 +138,27 @@ __device__ __forceinline__ void finish_block(const v4f (&acc)[4][3], const float
         near |= 1u << i;
     }
   }
+  const uint64_t r0 = t * p.nq + qb * 16u;  // the block's first row of the tile
+  // J through the whole stage first, then X and det J over it.
+  if (mask & kFJac) {
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const uint32_t qq = kg * 4u + static_cast<uint32_t>(i);
-    if (mask & kFPoints) {
-#pragma unroll
-      for (int g = 0; g < 3; ++g) sx[(qq * 3 + g) * 16 + cell] = (x0[g] + acc[0][g][i]) + p.origin[g];
-    }
-    if (mask & kFJac) {
+    for (int i = 0; i < 4; ++i) {
+      const uint32_t qq = kg * 4u + static_cast<uint32_t>(i);
 #pragma unroll
       for (int d = 0; d < 3; ++d)
 #pragma unroll
         for (int g = 0; g < 3; ++g) sj[(qq * 9 + d * 3 + g) * 16 + cell] = acc[1 + d][g][i];
+    }
+    sf::wave_sync();
+    flush(sj, oj + ((r0 * 9) << 4), nv * 144u, lane);
+    sf::wave_sync();  // J is read; X and det J may overwrite it
+  }
+  if (mask & kFPoints) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const uint32_t qq = kg * 4u + static_cast<uint32_t>(i);
+#pragma unroll
+      for (int g = 0; g < 3; ++g) sx[(qq * 3 + g) * 16 + cell] = (x0[g] + acc[0][g][i]) + p.origin[g];
     }
   }
   if (mask & kFDetJ) {
@@ -164,9 +173,7 @@ __device__ __forceinline__ void finish_block(const v4f (&acc)[4][3], const float
     for (int i = 0; i < 4; ++i) sd[(kg * 4u + i) * 16 + cell] = det[i];
   }
   sf::wave_sync();
-  const uint64_t r0 = t * p.nq + qb * 16u;  // the block's first row of the tile
   if (mask & kFPoints) flush(sx, ox + ((r0 * 3) << 4), nv * 48u, lane);
-  if (mask & kFJac) flush(sj, oj + ((r0 * 9) << 4), nv * 144u, lane);
   if (mask & kFDetJ) flush(sd, od + (r0 << 4), nv * 16u, lane);
   sf::wave_sync();  // the stage is read; the next block may overwrite it
 }
@@ -181,9 +188,9 @@ __global__ void __launch_bounds__(64 * kWaves)
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint32_t nqb = (p.nq + 15u) >> 4;
-  float* __restrict__ sx = lds + w * kStageElems;
-  float* __restrict__ sj = sx + 3 * 256;
-  float* __restrict__ sd = sx + 12 * 256;
+  float* __restrict__ sj = lds + w * kStageElems;
+  float* __restrict__ sx = sj;
+  float* __restrict__ sd = sj + 3 * 256;
 
   // Tiles: XCD x = blockIdx.x % 8 takes the contiguous range [lo, hi) of the block's tiles; its
   // grid / 8 workgroups take four consecutive tiles (one per wavefront) at a time, in turn.
@@ -278,9 +285,9 @@ __global__ void __launch_bounds__(64 * kWaves)
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint32_t nqb = (p.nq + 15u) >> 4;
-  float* __restrict__ sx = lds + w * kStageElems;
-  float* __restrict__ sj = sx + 3 * 256;
-  float* __restrict__ sd = sx + 12 * 256;
+  float* __restrict__ sj = lds + w * kStageElems;
+  float* __restrict__ sx = sj;
+  float* __restrict__ sd = sj + 3 * 256;
   const uint64_t nitems = (p.padded_cells >> 4) * nqb;
   const uint32_t nx = gridDim.x >> 3, xcd = blockIdx.x & 7u, j = blockIdx.x >> 3;
   const uint64_t per = nitems >> 3, rem = nitems & 7u;
@@ -410,6 +417,11 @@ hipError_t launch(uint32_t k, const TN* nodes, const uint32_t* dofmap, const flo
   e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(fn),
                                                    64 * kWaves, 0);
   if (e != hipSuccess) return e;
+  // Tiles of more than three blocks of points are bound by their write stream, which a
+  // narrower band of tiles in flight writes faster: three workgroups a CU there (P3 tetrahedra
+  // at 64 points 413.8 against 425.5 us with four; at 14 points four measured faster, 120.8
+  // against 125.0 us, and Q2 hexahedra at 40 points 272.6 against 285.1 us).
+  if (!rt && (p.nq + 15) / 16 > 3 && per_cu > 3) per_cu = 3;
   // Work items: tiles, or (tile, block of 16 points) for the runtime-k form.
   const uint64_t items = (p.padded_cells >> 4) * (rt ? (p.nq + 15) / 16 : 1);
   uint64_t grid = static_cast<uint64_t>(cus > 0 ? cus : 8) * static_cast<uint64_t>(per_cu > 0 ? per_cu : 1);
