@@ -621,12 +621,13 @@ def bench_p3tet(api, M, steps, warmup, n=48):
 
 
 def bench_dense_shapes(api, M, steps, warmup):
-    """The other 3D cells the dense matrix-core kernel takes (X + J + det J, FP32, jittered unit
-    boxes of 65,536-524,288 cells): its time per launch against the kernel the cell runs on
-    without it (TG_OPT_NO_MFMA: the register kernel of Q2/Q3 hexahedra at a point set without
-    tensor structure, the runtime-K kernel of P3/P4 pyramids, P4 tetrahedra, P3 prisms and Q4
-    hexahedra, the cooperative kernel of P5-P8 tetrahedra and P4/P5 prisms), and whether its
-    values equal the runtime-K kernel's (TG_OPT_FORCE_GENERIC) bit for bit."""
+    """The other 3D cells the dense matrix-core kernel takes (X + J + det J, FP32 and FP64,
+    jittered unit boxes of 10,368-524,288 cells): its time per launch against the kernel the
+    cell runs on without it (TG_OPT_NO_MFMA; in FP64 the register kernels up to 27 nodes and the
+    runtime-K or cooperative kernels above; in FP32 the register kernel of Q2/Q3 hexahedra at a
+    point set without tensor structure, the runtime-K kernel of P3/P4 pyramids, P4 tetrahedra,
+    P3 prisms and Q4 hexahedra, the cooperative kernel of P5-P8 tetrahedra and P4/P5 prisms),
+    and whether its values equal the runtime-K kernel's (TG_OPT_FORCE_GENERIC) bit for bit."""
     import torch
 
     names = ["points", "jacobians", "detj"]
@@ -648,27 +649,28 @@ def bench_dense_shapes(api, M, steps, warmup):
     out = {}
     for key, cell, p, n, make in cases:
         mesh = M.jitter(M.cube_box(cell, n, p), 0.02 / (n * p), 5, interior_only=True)
-        g = api.geometry_from_host(mesh, dtype=torch.float32)
         ps = make()
         C, N, k, nq = (mesh.blocks[0].num_cells, mesh.num_nodes, mesh.blocks[0].nodes_per_cell,
                        len(ps))
-        a = {nm: f.data for nm, f in api.geometric_factors_block(g, 0, ps, names).items()}
-        b = {nm: f.data for nm, f in api.geometric_factors_block(
-            g, 0, ps, names, options=api.OPT_FORCE_GENERIC).items()}
-        torch.cuda.synchronize()
-        same = all(torch.equal(a[nm], b[nm]) for nm in names)
-        nbytes = algorithmic_bytes(k, C, N, nq, 4)
-        rec = {"cells": C, "nodes_per_cell": k, "points": nq, "equals_runtime_k": same}
-        for label, opt, dst in (("mfma", 0, a), ("without", api.OPT_NO_MFMA, b)):
-            step = lambda opt=opt, dst=dst: api.geometric_factors_block(g, 0, ps, names, out=dst,
-                                                                        options=opt)
-            spin_up(step, 0.2)
-            ms = time_steps(step, steps, warmup)
-            rec[label] = {"ms": ms, "frac": nbytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS}
-        rec["speedup"] = rec["without"]["ms"] / rec["mfma"]["ms"]
-        out[key] = rec
-        del g, a, b
-        torch.cuda.empty_cache()
+        out[key] = {"cells": C, "nodes_per_cell": k, "points": nq}
+        for dt, tdt, size in (("f32", torch.float32, 4), ("f64", torch.float64, 8)):
+            g = api.geometry_from_host(mesh, dtype=tdt)
+            a = {nm: f.data for nm, f in api.geometric_factors_block(g, 0, ps, names).items()}
+            b = {nm: f.data for nm, f in api.geometric_factors_block(
+                g, 0, ps, names, options=api.OPT_FORCE_GENERIC).items()}
+            torch.cuda.synchronize()
+            rec = {"equals_runtime_k": all(torch.equal(a[nm], b[nm]) for nm in names)}
+            nbytes = algorithmic_bytes(k, C, N, nq, size)
+            for label, opt, dst in (("mfma", 0, a), ("without", api.OPT_NO_MFMA, b)):
+                step = lambda opt=opt, dst=dst: api.geometric_factors_block(
+                    g, 0, ps, names, out=dst, options=opt)
+                spin_up(step, 0.2)
+                ms = time_steps(step, steps, warmup)
+                rec[label] = {"ms": ms, "frac": nbytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS}
+            rec["speedup"] = rec["without"]["ms"] / rec["mfma"]["ms"]
+            out[key][dt] = rec
+            del g, a, b
+            torch.cuda.empty_cache()
     return out
 
 

@@ -239,8 +239,9 @@ int device_tables(tg_pointset* p, unsigned degree, int family, int dtype, Device
                          : upload<double>(p->normals, &t->ref_normals);
     if (rc) return rc;
   }
-  if (dtype == TG_F32 && p->tdim == 3 && tg::has_dense_mfma(3, 3, static_cast<uint32_t>(k)) &&
-      p->nq <= tg::kDenseMfmaMaxPoints) {
+  const uint32_t ku = static_cast<uint32_t>(k);
+  if (p->tdim == 3 && p->nq <= tg::kDenseMfmaMaxPoints &&
+      (dtype == TG_F32 ? tg::has_dense_mfma(3, 3, ku) : tg::has_dense_mfma_fp64(3, 3, ku))) {
     // The table in matrix-core operand order (dense_mfma.hip): block qb, kind v, step s, lane l
     // holds T_v[16 qb + l % 16][4 s + l / 16].
     const uint32_t kk = static_cast<uint32_t>(k), nq = static_cast<uint32_t>(p->nq);
@@ -254,10 +255,8 @@ int device_tables(tg_pointset* p, unsigned degree, int family, int dtype, Device
             if (q < nq && node < kk)
               a[((qb * 4 + v) * S + st) * 64 + l] = inter[(q * kk + node) * 4 + v];
           }
-    void* dev = nullptr;
-    rc = upload<float>(a, &dev);
+    rc = dtype == TG_F32 ? upload<float>(a, &t->dense_mfma) : upload<double>(a, &t->dense_mfma);
     if (rc) return rc;
-    t->dense_mfma = static_cast<float*>(dev);
   }
   std::vector<double> tab1d;
   std::vector<uint32_t> lex;
@@ -420,25 +419,30 @@ static bool takes_sumfact(const tg_geometry* g, const DeviceTables* tabs, uint32
          ((options & TG_OPT_SUMFACT_ALWAYS) || sumfact_pays(f32, tabs->sf_n1, tabs->sf_m));
 }
 
-// Whether the factors of a block go to the dense matrix-core kernel (dense_mfma.hip): FP32 with
-// the cell shift (on FP32 nodes or the FP64 copy), a 3D cell of a node count it has (20 to 84;
-// the point set's operand-order table exists), tiles of 16, X, J and det J only, a block the
+// Whether the factors of a block go to the dense matrix-core kernel (dense_mfma.hip): a 3D cell
+// of a node count it has (FP32 with the cell shift, on FP32 nodes or the FP64 copy: 20 to 84
+// nodes in registers and more in the runtime-k form; FP64: 20 nodes and more), the point set's
+// operand-order table (at most 128 points), tiles of 16, X, J and det J only, a block the
 // sum-factorised kernels do not take, unless a strategy is forced. Calls without J (X alone,
 // det J alone: the measure's and the equispaced conversion's) take it only where the cell has no
 // register-resident kernel, whose lane-per-cell mappings serve them. Measured on MI355X, X + J +
 // det J: P3 tetrahedra on 663,552 cells 0.121 against 0.150 ms at 14 points, 0.413 against 0.503
-// ms at 64; the cells without register kernels 7-51x faster than the runtime-K and cooperative
+// ms at 64; the cells without register kernels 2.4-45x faster than the runtime-K and cooperative
 // kernels (dense_mfma.hip lists them).
 bool takes_dense_mfma(const tg_geometry* g, const Block& b, const DeviceTables* tabs,
                       uint32_t factors, uint32_t tile, uint32_t options) {
-  const bool shift = g->dtype == TG_F32 && !(options & TG_OPT_NO_CELL_SHIFT);
+  const bool f32 = g->dtype == TG_F32;
   const uint32_t xjd = TG_POINTS | TG_JACOBIAN | TG_DETJ;
-  if (!shift || !tabs->dense_mfma || tile != 16 || !tg::has_dense_mfma(b.tdim, g->gdim, b.k) ||
+  if ((f32 && (options & TG_OPT_NO_CELL_SHIFT)) || !tabs->dense_mfma || tile != 16 ||
+      !(f32 ? tg::has_dense_mfma(b.tdim, g->gdim, b.k)
+            : tg::has_dense_mfma_fp64(b.tdim, g->gdim, b.k)) ||
       !(factors & xjd) || (factors & ~xjd) ||
       (options & (TG_OPT_NO_MFMA | TG_OPT_FORCE_GENERIC | TG_OPT_COOPERATIVE)))
     return false;
-  if (takes_sumfact(g, tabs, factors, tile, options, true)) return false;
-  return (factors & TG_JACOBIAN) || !tg::has_direct_factor<float>(b.tdim, g->gdim, b.k, true);
+  if (takes_sumfact(g, tabs, factors, tile, options, f32)) return false;
+  if (factors & TG_JACOBIAN) return true;
+  return f32 ? !tg::has_direct_factor<float>(b.tdim, g->gdim, b.k, true)
+             : !tg::has_direct_factor<double>(b.tdim, g->gdim, b.k, false);
 }
 
 template <typename T>
@@ -491,11 +495,19 @@ int enqueue_block(const tg_geometry* g, const Block& b, tg_pointset* pts, Device
   }
   if constexpr (sizeof(T) == 4) {
     if (takes_dense_mfma(g, b, tabs, factors, tile, options)) {
-      hipError_t e = n64 ? tg::launch_dense_mfma_f64_nodes(b.k, n64, b.dofmap, tabs->dense_mfma,
-                                                           tabs->tab64, p, o, stream)
+      const float* ta = static_cast<const float*>(tabs->dense_mfma);
+      hipError_t e = n64 ? tg::launch_dense_mfma_f64_nodes(b.k, n64, b.dofmap, ta, tabs->tab64, p,
+                                                           o, stream)
                          : tg::launch_dense_mfma(b.k, static_cast<const float*>(g->nodes),
-                                                 b.dofmap, tabs->dense_mfma, tabs->tab64, p, o,
-                                                 stream);
+                                                 b.dofmap, ta, tabs->tab64, p, o, stream);
+      if (e == hipSuccess) return TG_OK;
+      if (e != hipErrorNotSupported) return hip_fail(e, "dense matrix-core launch");
+    }
+  } else {
+    if (takes_dense_mfma(g, b, tabs, factors, tile, options)) {
+      hipError_t e = tg::launch_dense_mfma_fp64(
+          b.k, static_cast<const double*>(g->nodes), b.dofmap,
+          static_cast<const double*>(tabs->dense_mfma), tabs->tab64, p, o, stream);
       if (e == hipSuccess) return TG_OK;
       if (e != hipErrorNotSupported) return hip_fail(e, "dense matrix-core launch");
     }

@@ -41,12 +41,29 @@ namespace tg {
 namespace {
 
 typedef float v4f __attribute__((ext_vector_type(4)));
+typedef double v4d __attribute__((ext_vector_type(4)));
 
 constexpr int kWaves = 4;
 
 __device__ __forceinline__ v4f mfma4(float a, float b, v4f c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
+
+// FP64: the same 16 x 16 x 4 shape and lane layout, one product of doubles per lane.
+__device__ __forceinline__ v4d mfma4(double a, double b, v4d c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+template <typename T>
+struct Acc;
+template <>
+struct Acc<float> {
+  typedef v4f type;
+};
+template <>
+struct Acc<double> {
+  typedef v4d type;
+};
 
 template <typename T>
 using gptr = __attribute__((address_space(1))) T*;
@@ -59,13 +76,16 @@ __device__ __forceinline__ gptr<T> gp(T* p) {
 // it once J is stored, X [16][3][16] and det J [16][16] (36 KiB a workgroup: four per CU).
 constexpr int kStageElems = 9 * 256;
 
-// Copies n values (a multiple of 4) from LDS to global memory with 16-byte non-temporal stores
-// spread over the wavefront's lanes.
-__device__ __forceinline__ void flush(const float* __restrict__ src, gptr<float> dst, uint32_t n,
+// Copies n values (16 bytes of them a multiple) from LDS to global memory with 16-byte
+// non-temporal stores spread over the wavefront's lanes.
+template <typename T>
+__device__ __forceinline__ void flush(const T* __restrict__ src, gptr<T> dst, uint32_t n,
                                       uint32_t lane) {
-  for (uint32_t e = lane; e < n / 4u; e += 64u) {
-    const v4f v = *reinterpret_cast<const v4f*>(src + e * 4u);
-    __builtin_nontemporal_store(v, reinterpret_cast<gptr<v4f>>(dst + e * 4u));
+  constexpr uint32_t E = 16 / sizeof(T);
+  typedef T v16 __attribute__((ext_vector_type(E)));
+  for (uint32_t e = lane; e < n / E; e += 64u) {
+    const v16 v = *reinterpret_cast<const v16*>(src + e * E);
+    __builtin_nontemporal_store(v, reinterpret_cast<gptr<v16>>(dst + e * E));
   }
 }
 
@@ -97,30 +117,38 @@ __device__ __noinline__ double detj_fp64_k(const TN* __restrict__ nodes,
          j[6] * (j[1] * j[5] - j[4] * j[2]);
 }
 
-// The end of one block of 16 points of a tile, from the products acc[v][g] (v = 0: X offsets,
-// 1..3: J(g, d = v - 1)); lane (cell, kg) = (l % 16, l / 16) holds the rows qq = 4 kg + i of
-// the block (points 16 qb + qq) of its cell. det J of every point first; X and J go to the
-// stage, then the FP64 recomputation of flagged points runs with the accumulators dead; the
-// block's rows leave as contiguous runs of each field.
-template <typename TN>
-__device__ __forceinline__ void finish_block(const v4f (&acc)[4][3], const float (&x0)[3],
-                                             const TN* __restrict__ nodes,
+// The end of one block of 16 points of a tile, from the products acc[v][g] (v = 0: X, as
+// offsets from node 0 in FP32 and absolute in FP64; 1..3: J(g, d = v - 1)); lane (cell, kg) =
+// (l % 16, l / 16) holds four rows qq of the block (points 16 qb + qq) of its cell: 4 kg + i in
+// FP32, kg + 4 i in FP64 (the layouts of the two instructions' results).
+// det J of every point first (in FP32 with the test for the FP64 recomputation); J goes through
+// the whole stage first, then X and det J over it, the recomputation of flagged points running
+// with the accumulators dead; the block's rows leave as contiguous runs of each field.
+template <typename T, typename TN>
+__device__ __forceinline__ void finish_block(const typename Acc<T>::type (&acc)[4][3],
+                                             const T (&x0)[3], const TN* __restrict__ nodes,
                                              const uint32_t* __restrict__ row, uint32_t k,
                                              const double* __restrict__ tab64,
-                                             const FactorParams<float>& p, uint64_t t,
-                                             uint32_t qb, float* __restrict__ sx,
-                                             float* __restrict__ sj, float* __restrict__ sd,
-                                             gptr<float> ox, gptr<float> oj, gptr<float> od,
-                                             uint32_t lane) {
+                                             const FactorParams<T>& p, uint64_t t, uint32_t qb,
+                                             T* __restrict__ stage, gptr<T> ox, gptr<T> oj,
+                                             gptr<T> od, uint32_t lane) {
+  constexpr bool kShift = sizeof(T) == 4;
+  T* __restrict__ sj = stage;
+  T* __restrict__ sx = stage;
+  T* __restrict__ sd = stage + 3 * 256;
   const uint32_t cell = lane & 15u, kg = lane >> 4;
   const uint32_t mask = p.mask;
   const uint32_t nv = p.nq - qb * 16u < 16u ? p.nq - qb * 16u : 16u;  // rows in the block
-  float det[4];
+  // Register i of the product holds row 4 kg + i (FP32) or kg + 4 i (FP64) of the block.
+  const auto qrow = [kg](int i) -> uint32_t {
+    return sizeof(T) == 4 ? kg * 4u + static_cast<uint32_t>(i) : kg + 4u * static_cast<uint32_t>(i);
+  };
+  T det[4];
   uint32_t near = 0;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
 #pragma clang fp contract(off)
-    float jm[3][3];  // jm[d][g] = J(g, d)
+    T jm[3][3];  // jm[d][g] = J(g, d)
 #pragma unroll
     for (int d = 0; d < 3; ++d)
 #pragma unroll
@@ -128,22 +156,23 @@ __device__ __forceinline__ void finish_block(const v4f (&acc)[4][3], const float
     det[i] = jm[0][0] * (jm[1][1] * jm[2][2] - jm[2][1] * jm[1][2]) -
              jm[1][0] * (jm[0][1] * jm[2][2] - jm[2][1] * jm[0][2]) +
              jm[2][0] * (jm[0][1] * jm[1][2] - jm[1][1] * jm[0][2]);
-    if (p.fixup) {
-      float n2 = 0.f;
+    if constexpr (sizeof(T) == 4) {
+      if (p.fixup) {
+        float n2 = 0.f;
 #pragma unroll
-      for (int d = 0; d < 3; ++d)
+        for (int d = 0; d < 3; ++d)
 #pragma unroll
-        for (int g = 0; g < 3; ++g) n2 = __builtin_fmaf(jm[d][g], jm[d][g], n2);
-      if (__builtin_fabsf(det[i]) < static_cast<float>(p.tau) * n2 * __builtin_sqrtf(n2))
-        near |= 1u << i;
+          for (int g = 0; g < 3; ++g) n2 = __builtin_fmaf(jm[d][g], jm[d][g], n2);
+        if (__builtin_fabsf(det[i]) < static_cast<float>(p.tau) * n2 * __builtin_sqrtf(n2))
+          near |= 1u << i;
+      }
     }
   }
   const uint64_t r0 = t * p.nq + qb * 16u;  // the block's first row of the tile
-  // J through the whole stage first, then X and det J over it.
   if (mask & kFJac) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const uint32_t qq = kg * 4u + static_cast<uint32_t>(i);
+      const uint32_t qq = qrow(i);
 #pragma unroll
       for (int d = 0; d < 3; ++d)
 #pragma unroll
@@ -156,9 +185,11 @@ __device__ __forceinline__ void finish_block(const v4f (&acc)[4][3], const float
   if (mask & kFPoints) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const uint32_t qq = kg * 4u + static_cast<uint32_t>(i);
+      const uint32_t qq = qrow(i);
 #pragma unroll
-      for (int g = 0; g < 3; ++g) sx[(qq * 3 + g) * 16 + cell] = (x0[g] + acc[0][g][i]) + p.origin[g];
+      for (int g = 0; g < 3; ++g)
+        sx[(qq * 3 + g) * 16 + cell] =
+            kShift ? (x0[g] + acc[0][g][i]) + p.origin[g] : acc[0][g][i] + p.origin[g];
     }
   }
   if (mask & kFDetJ) {
@@ -166,11 +197,11 @@ __device__ __forceinline__ void finish_block(const v4f (&acc)[4][3], const float
 #pragma unroll
       for (int i = 0; i < 4; ++i)
         if (near & (1u << i))
-          det[i] = static_cast<float>(
-              detj_fp64_k(nodes, row, k, p.last_node, tab64, qb * 16u + kg * 4u + i));
+          det[i] = static_cast<T>(
+              detj_fp64_k(nodes, row, k, p.last_node, tab64, qb * 16u + qrow(i)));
     }
 #pragma unroll
-    for (int i = 0; i < 4; ++i) sd[(kg * 4u + i) * 16 + cell] = det[i];
+    for (int i = 0; i < 4; ++i) sd[qrow(i) * 16 + cell] = det[i];
   }
   sf::wave_sync();
   if (mask & kFPoints) flush(sx, ox + ((r0 * 3) << 4), nv * 48u, lane);
@@ -188,9 +219,7 @@ __global__ void __launch_bounds__(64 * kWaves)
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint32_t nqb = (p.nq + 15u) >> 4;
-  float* __restrict__ sj = lds + w * kStageElems;
-  float* __restrict__ sx = sj;
-  float* __restrict__ sd = sj + 3 * 256;
+  float* __restrict__ stage = lds + w * kStageElems;
 
   // Tiles: XCD x = blockIdx.x % 8 takes the contiguous range [lo, hi) of the block's tiles; its
   // grid / 8 workgroups take four consecutive tiles (one per wavefront) at a time, in turn.
@@ -263,31 +292,41 @@ __global__ void __launch_bounds__(64 * kWaves)
           for (int g = 0; g < 3; ++g) acc[vv][g] = mfma4(a, b[s][g], acc[vv][g]);
         }
       }
-      finish_block(acc, x0, nodes, row, K, tab64, p, t, qb, sx, sj, sd, ox, oj, od, lane);
+      finish_block<float>(acc, x0, nodes, row, K, tab64, p, t, qb, stage, ox, oj, od, lane);
     }
   }
 }
 
-// Cells of more nodes than the register operands above hold (runtime k): the same products, with
-// the B operands gathered in chunks of kChunk steps for each block of 16 points, the
-// accumulators carried across the chunks (the same node order). The work items are (tile, block
-// of 16 points), so that the few tiles of high-degree meshes spread over more wavefronts; the
-// XCDs take contiguous ranges of them, a tile's blocks in turn.
-constexpr int kChunk = 8;
+// Cells of more nodes than the register operands above hold (runtime k), and every FP64 cell the
+// matrix cores take: the same products, with the B operands gathered in chunks of steps for each
+// block of 16 points, the accumulators carried across the chunks (the same node order). The work
+// items are (tile, block of 16 points), so that the few tiles of high-degree meshes spread over
+// more wavefronts; the XCDs take contiguous ranges of them, a tile's blocks in turn. In FP64
+// (v_mfma_f64_16x16x4f64) the operands are the absolute coordinates from node 0, as the FP64
+// kernels sum them; its stage of doubles takes two wavefronts a workgroup.
+template <typename T>
+constexpr int rt_waves() {
+  return sizeof(T) == 4 ? kWaves : 2;
+}
+template <typename T>
+constexpr int rt_chunk() {
+  return sizeof(T) == 4 ? 8 : 4;  // steps of four nodes per chunk of B operands
+}
 
-template <typename TN>
-__global__ void __launch_bounds__(64 * kWaves)
+template <typename T, typename TN>
+__global__ void __launch_bounds__(64 * rt_waves<T>())
     dense_mfma_rt(const TN* __restrict__ nodes, const uint32_t* __restrict__ dofmap,
-                  const float* __restrict__ taba, const double* __restrict__ tab64,
-                  FactorParams<float> p, FactorOutputs o) {
-  __shared__ __align__(16) float lds[kWaves * kStageElems];
+                  const T* __restrict__ taba, const double* __restrict__ tab64,
+                  FactorParams<T> p, FactorOutputs o) {
+  constexpr int kW = rt_waves<T>(), kC = rt_chunk<T>();
+  constexpr bool kShift = sizeof(T) == 4;
+  typedef typename Acc<T>::type V;
+  __shared__ __align__(16) T lds[kW * kStageElems];
   const uint32_t K = p.k, S = (K + 3) / 4;
   const uint32_t lane = threadIdx.x & 63u;
   const uint32_t w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const uint32_t nqb = (p.nq + 15u) >> 4;
-  float* __restrict__ sj = lds + w * kStageElems;
-  float* __restrict__ sx = sj;
-  float* __restrict__ sd = sj + 3 * 256;
+  T* __restrict__ stage = lds + w * kStageElems;
   const uint64_t nitems = (p.padded_cells >> 4) * nqb;
   const uint32_t nx = gridDim.x >> 3, xcd = blockIdx.x & 7u, j = blockIdx.x >> 3;
   const uint64_t per = nitems >> 3, rem = nitems & 7u;
@@ -295,72 +334,76 @@ __global__ void __launch_bounds__(64 * kWaves)
   const uint64_t hi = lo + per + (xcd < rem ? 1u : 0u);
   const uint32_t cell = lane & 15u, kg = lane >> 4;
   const uint32_t mask = p.mask;
-  gptr<float> ox = gp(static_cast<float*>(o.x));
-  gptr<float> oj = gp(static_cast<float*>(o.j));
-  gptr<float> od = gp(static_cast<float*>(o.detj));
-  for (uint64_t it = lo + (static_cast<uint64_t>(j) * kWaves + w); it < hi;
-       it += static_cast<uint64_t>(nx) * kWaves) {
+  gptr<T> ox = gp(static_cast<T*>(o.x));
+  gptr<T> oj = gp(static_cast<T*>(o.j));
+  gptr<T> od = gp(static_cast<T*>(o.detj));
+  for (uint64_t it = lo + (static_cast<uint64_t>(j) * kW + w); it < hi;
+       it += static_cast<uint64_t>(nx) * kW) {
     const uint64_t t = it / nqb;
     const uint32_t qb = static_cast<uint32_t>(it - t * nqb);
     const uint64_t c0 = t * 16 + cell;
     const uint64_t c = c0 < p.num_cells ? c0 : p.num_cells - 1;
     const uint32_t* __restrict__ row = dofmap + c * K;
     TN x0n[3];
-    float x0[3];
+    T x0[3];
     {
       const uint32_t r0 = gp(row)[0];
       const uint32_t n0 = r0 < p.last_node ? r0 : p.last_node;
 #pragma unroll
       for (int g = 0; g < 3; ++g) {
         x0n[g] = gp(nodes)[static_cast<uint64_t>(n0) * 3 + g];
-        x0[g] = static_cast<float>(x0n[g]);
+        x0[g] = static_cast<T>(x0n[g]);
       }
     }
-    const v4f zero = {0.f, 0.f, 0.f, 0.f};
-    v4f acc[4][3];
+    const V zero = {0, 0, 0, 0};
+    V acc[4][3];
 #pragma unroll
     for (int vv = 0; vv < 4; ++vv)
 #pragma unroll
       for (int g = 0; g < 3; ++g) acc[vv][g] = zero;
-    gptr<const float> ta = gp(taba) + qb * (4u * S * 64u) + lane;
-    for (uint32_t s0 = 0; s0 < S; s0 += kChunk) {
-      TN v[kChunk][3];
+    gptr<const T> ta = gp(taba) + qb * (4u * S * 64u) + lane;
+    for (uint32_t s0 = 0; s0 < S; s0 += kC) {
+      TN v[kC][3];
 #pragma unroll
-      for (int i = 0; i < kChunk; ++i) {
+      for (int i = 0; i < kC; ++i) {
         const uint32_t k = 4u * (s0 + i) + kg;
         const uint32_t r = gp(row)[k < K ? k : K - 1];
         const uint32_t n = r < p.last_node ? r : p.last_node;
 #pragma unroll
         for (int g = 0; g < 3; ++g) v[i][g] = gp(nodes)[static_cast<uint64_t>(n) * 3 + g];
       }
-      float b[kChunk][3];
+      T b[kC][3];
 #pragma unroll
-      for (int i = 0; i < kChunk; ++i) {
+      for (int i = 0; i < kC; ++i) {
         const uint32_t k = 4u * (s0 + i) + kg;
 #pragma unroll
-        for (int g = 0; g < 3; ++g)
-          b[i][g] = (k > 0 && k < K) ? static_cast<float>(v[i][g] - x0n[g]) : 0.f;
+        for (int g = 0; g < 3; ++g) {
+          if constexpr (kShift)
+            b[i][g] = (k > 0 && k < K) ? static_cast<T>(v[i][g] - x0n[g]) : T(0);
+          else
+            b[i][g] = k < K ? static_cast<T>(v[i][g]) : T(0);
+        }
       }
 #pragma unroll
-      for (int i = 0; i < kChunk; ++i) {
+      for (int i = 0; i < kC; ++i) {
         if (s0 + i >= S) break;  // uniform
 #pragma unroll
         for (int vv = 0; vv < 4; ++vv) {
           if (vv == 0 && !(mask & kFPoints)) continue;
           if (vv > 0 && !(mask & (kFJac | kFDetJ))) continue;
-          const float a = ta[(vv * S + s0 + i) * 64];
+          const T a = ta[(vv * S + s0 + i) * 64];
 #pragma unroll
           for (int g = 0; g < 3; ++g) acc[vv][g] = mfma4(a, b[i][g], acc[vv][g]);
         }
       }
     }
-    finish_block(acc, x0, nodes, row, K, tab64, p, t, qb, sx, sj, sd, ox, oj, od, lane);
+    finish_block<T>(acc, x0, nodes, row, K, tab64, p, t, qb, stage, ox, oj, od, lane);
   }
 }
 
-template <typename TN>
-using DenseFn = void (*)(const TN*, const uint32_t*, const float*, const double*,
-                         FactorParams<float>, FactorOutputs);
+template <typename T, typename TN>
+using DenseFn = void (*)(const TN*, const uint32_t*, const T*, const double*, FactorParams<T>,
+                         FactorOutputs);
 
 // 3D cells of 20 to 84 nodes. Measured on MI355X against the kernels they replace (X + J + det
 // J, FP32): P3 tetrahedra 1.25x (the register kernel), Q2 hexahedra 1.11x and Q3 hexahedra 2.9x
@@ -369,7 +412,7 @@ using DenseFn = void (*)(const TN*, const uint32_t*, const float*, const double*
 // prisms 7.7x (the cooperative kernel). P2 tetrahedra (10 nodes), P2 pyramids (14) and P2
 // prisms (18) measured slower here than on their register kernels and keep them.
 template <typename TN>
-DenseFn<TN> pick(uint32_t k) {
+DenseFn<float, TN> pick(uint32_t k) {
   switch (k) {
     case 20:  // P3 tetrahedra
       return dense_mfma<20, TN>;
@@ -394,17 +437,20 @@ DenseFn<TN> pick(uint32_t k) {
   }
 }
 
-// Node counts from 30 not listed above (P4 pyramids and the cells of more than 84 nodes) take
-// the runtime-k form.
+// FP32 node counts from 30 not listed above (P4 pyramids and the cells of more than 84 nodes)
+// take the runtime-k form, as FP64 cells of 20 nodes and more do.
 constexpr uint32_t kRtMinNodes = 30;
+constexpr uint32_t kF64MinNodes = 20;
 
-template <typename TN>
-hipError_t launch(uint32_t k, const TN* nodes, const uint32_t* dofmap, const float* taba,
-                  const double* tab64, const FactorParams<float>& p, const FactorOutputs& out,
+template <typename T, typename TN>
+hipError_t launch(uint32_t k, const TN* nodes, const uint32_t* dofmap, const T* taba,
+                  const double* tab64, const FactorParams<T>& p, const FactorOutputs& out,
                   hipStream_t stream) {
-  DenseFn<TN> fn = pick<TN>(k);
-  const bool rt = fn == nullptr && k >= kRtMinNodes;
-  if (rt) fn = dense_mfma_rt<TN>;
+  DenseFn<T, TN> fn = nullptr;
+  if constexpr (sizeof(T) == 4) fn = pick<TN>(k);
+  const bool rt = fn == nullptr && k >= (sizeof(T) == 4 ? kRtMinNodes : kF64MinNodes);
+  if (rt) fn = dense_mfma_rt<T, TN>;
+  const int waves = rt ? rt_waves<T>() : kWaves;
   if (!fn || p.tile_log2 != 4 || p.nq == 0 || p.nq > kDenseMfmaMaxPoints || p.k != k)
     return hipErrorNotSupported;
   int dev = 0, cus = 0;
@@ -415,7 +461,7 @@ hipError_t launch(uint32_t k, const TN* nodes, const uint32_t* dofmap, const flo
   // tiles need.
   int per_cu = 0;
   e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(fn),
-                                                   64 * kWaves, 0);
+                                                   64 * waves, 0);
   if (e != hipSuccess) return e;
   // Tiles of more than three blocks of points are bound by their write stream, which a
   // narrower band of tiles in flight writes faster: three workgroups a CU there (P3 tetrahedra
@@ -425,11 +471,11 @@ hipError_t launch(uint32_t k, const TN* nodes, const uint32_t* dofmap, const flo
   // Work items: tiles, or (tile, block of 16 points) for the runtime-k form.
   const uint64_t items = (p.padded_cells >> 4) * (rt ? (p.nq + 15) / 16 : 1);
   uint64_t grid = static_cast<uint64_t>(cus > 0 ? cus : 8) * static_cast<uint64_t>(per_cu > 0 ? per_cu : 1);
-  const uint64_t need = (items + kWaves - 1) / kWaves;
+  const uint64_t need = (items + waves - 1) / waves;
   const uint64_t need8 = (need + 7) / 8 * 8;
   grid = (grid < need8 ? grid : need8) / 8 * 8;
   if (grid < 8) grid = 8;
-  hipLaunchKernelGGL(fn, dim3(static_cast<uint32_t>(grid)), dim3(64 * kWaves), 0, stream, nodes,
+  hipLaunchKernelGGL(fn, dim3(static_cast<uint32_t>(grid)), dim3(64 * waves), 0, stream, nodes,
                      dofmap, taba, tab64, p, out);
   return hipGetLastError();
 }
@@ -440,6 +486,10 @@ bool has_dense_mfma(int tdim, int gdim, uint32_t k) {
   return tdim == 3 && gdim == 3 && (pick<float>(k) != nullptr || k >= kRtMinNodes);
 }
 
+bool has_dense_mfma_fp64(int tdim, int gdim, uint32_t k) {
+  return tdim == 3 && gdim == 3 && k >= kF64MinNodes;
+}
+
 size_t dense_mfma_table_elems(uint32_t k, uint32_t nq) {
   const uint32_t S = (k + 3) / 4, nqb = (nq + 15) / 16;
   return static_cast<size_t>(nqb) * 4 * S * 64;
@@ -448,14 +498,21 @@ size_t dense_mfma_table_elems(uint32_t k, uint32_t nq) {
 hipError_t launch_dense_mfma(uint32_t k, const float* nodes, const uint32_t* dofmap,
                              const float* taba, const double* tab64, const FactorParams<float>& p,
                              const FactorOutputs& out, hipStream_t stream) {
-  return launch<float>(k, nodes, dofmap, taba, tab64, p, out, stream);
+  return launch<float, float>(k, nodes, dofmap, taba, tab64, p, out, stream);
 }
 
 hipError_t launch_dense_mfma_f64_nodes(uint32_t k, const double* nodes, const uint32_t* dofmap,
                                        const float* taba, const double* tab64,
                                        const FactorParams<float>& p, const FactorOutputs& out,
                                        hipStream_t stream) {
-  return launch<double>(k, nodes, dofmap, taba, tab64, p, out, stream);
+  return launch<float, double>(k, nodes, dofmap, taba, tab64, p, out, stream);
+}
+
+hipError_t launch_dense_mfma_fp64(uint32_t k, const double* nodes, const uint32_t* dofmap,
+                                  const double* taba, const double* tab64,
+                                  const FactorParams<double>& p, const FactorOutputs& out,
+                                  hipStream_t stream) {
+  return launch<double, double>(k, nodes, dofmap, taba, tab64, p, out, stream);
 }
 
 }  // namespace tg

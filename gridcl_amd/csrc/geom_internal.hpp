@@ -185,6 +185,14 @@ hipError_t launch_dense_mfma_f64_nodes(uint32_t k, const double* nodes, const ui
                                        const float* taba, const double* tab64,
                                        const FactorParams<float>& p, const FactorOutputs& out,
                                        hipStream_t stream);
+// FP64 geometries (v_mfma_f64_16x16x4f64): 3D cells of k >= 20 nodes, the table in the same
+// operand order in FP64, sums of the absolute coordinates from node 0 as the FP64 kernels take
+// them.
+bool has_dense_mfma_fp64(int tdim, int gdim, uint32_t k);
+hipError_t launch_dense_mfma_fp64(uint32_t k, const double* nodes, const uint32_t* dofmap,
+                                  const double* taba, const double* tab64,
+                                  const FactorParams<double>& p, const FactorOutputs& out,
+                                  hipStream_t stream);
 
 // The cooperative strategy (k up to 256 nodes): L lanes per cell, L the largest power of two not
 // above min(32, k), leaves added by a pairwise tree. hipErrorNotSupported for other shapes.

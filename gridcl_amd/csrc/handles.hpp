@@ -42,7 +42,7 @@ struct DeviceTables {
   uint32_t* sf_lex = nullptr;
   void* sf_mfma = nullptr;
   int sf_n1 = 0, sf_m = 0;
-  float* dense_mfma = nullptr;  // the table in matrix-core operand order (FP32 P3 tetrahedra)
+  void* dense_mfma = nullptr;  // the table in matrix-core operand order (3D cells, FP32 or FP64)
   ~DeviceTables();
 };
 

@@ -226,14 +226,17 @@ SHAPES = [("P3 tet", TET, 3, lambda: api.PointSet.gauss(TET, 4), True),
 
 
 @pytest.mark.parametrize("idx", range(len(SHAPES)))
-def test_every_shape_against_oracle_and_node_order_kernels(cuda, idx):
+@pytest.mark.parametrize("dtype", ["f32", "f64"])
+def test_every_shape_against_oracle_and_node_order_kernels(cuda, idx, dtype):
     """Every cell the kernel takes (20 to 165 nodes; the runtime-k form from P4 pyramids and
     above 84 nodes): X, J and det J equal the runtime-K kernel's (TG_OPT_FORCE_GENERIC: node
     order from node 0) bit for bit, and the register kernel's where the cell has one
     (TG_OPT_NO_MFMA); against the oracle within 4e-6 where a register kernel exists (the dense
     FP32 sums above), otherwise within the reference's scaled tolerances on its f32 base 1e-5
-    (tests/test_gpu_coop.py; the base itself up to the degree-4 element). X alone and det J alone
-    equal the full call."""
+    (tests/test_gpu_coop.py; the base itself up to the degree-4 element). FP64 geometries take
+    the FP64 matrix cores (v_mfma_f64_16x16x4f64) from 20 nodes, with the same checks against
+    the FP64 kernels (node order from node 0, no shift) and the FP64 base 1e-12. X alone and det J
+    alone equal the full call."""
     import torch
 
     from tests.test_gpu_coop import scaled_tolerances
@@ -242,7 +245,8 @@ def test_every_shape_against_oracle_and_node_order_kernels(cuda, idx):
     ps = make()
     assert len(ps) <= 128
     mesh = volume_mesh(cell, p, n=2)
-    g = api.geometry_from_host(mesh, dtype=torch.float32)
+    f64 = dtype == "f64"
+    g = api.geometry_from_host(mesh, dtype=torch.float64 if f64 else torch.float32)
     got = gpu_factors(g, 0, ps, NAMES)
     gen = gpu_factors(g, 0, ps, NAMES, options=api.OPT_FORCE_GENERIC)
     for n in NAMES:
@@ -253,11 +257,18 @@ def test_every_shape_against_oracle_and_node_order_kernels(cuda, idx):
             assert np.array_equal(got[n], reg[n]), (label, n)
     want = oracle_factors(g, mesh, 0, ps, NAMES)
     err = rel_errs(got, want)
-    if has_reg:
+    base = 1e-12 if f64 else 1e-5
+    if has_reg and not f64:
         tol = {n: 4e-6 for n in NAMES}
+    elif f64 and (cell, p) == (PYR, 4):
+        # The library's and the oracle's pyramid tables both come from a Vandermonde solve whose
+        # conditioning costs a digit at degree 4 (tests/test_refcell.py allows 1e-11 for the
+        # values and 1e-10 for the derivatives there); the sums' own rounding is 8e-15 of max |J|
+        # on this mesh. Measured: J 6.2e-12, det J 2.2e-11.
+        tol = {"points": 1e-11, "jacobians": 1e-10, "detj": 1e-10}
     else:
-        tol = scaled_tolerances(cell, p, ps.points, 1e-5, want["jacobians"])
-    print(f"{label} ({mesh.blocks[0].nodes_per_cell} nodes, {len(ps)} points): {err}")
+        tol = scaled_tolerances(cell, p, ps.points, base, want["jacobians"])
+    print(f"{label} {dtype} ({mesh.blocks[0].nodes_per_cell} nodes, {len(ps)} points): {err}")
     for n in NAMES:
         assert err[n] <= tol[n], (label, n, err[n], tol[n])
     for n in ("points", "detj"):
