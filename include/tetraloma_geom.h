@@ -122,10 +122,14 @@ enum {
   /* Plans only: run every block in the one persistent kernel over the work list, where every
    * block's element has a kernel of that form (otherwise the plan stays per block). */
   TG_OPT_PLAN_PERSISTENT = 1u << 11,
-  /* FP32 sum-factorised hexahedra: keep the second and third sums on the vector ALU instead of
-   * the matrix cores (v_mfma_f32_16x16x4f32), which run them by default where they measure
-   * faster (degree 2 at 3^3 points, degree 3 at 4^3 points) and, under TG_OPT_SUMFACT_ALWAYS,
-   * wherever they apply (2-4 points per direction). */
+  /* Keep the node sums off the matrix cores: FP32 sum-factorised hexahedra keep their second
+   * and third sums on the vector ALU (the matrix cores run them by default where they measure
+   * faster: degree 2 at 3^3 points, degree 3 at 4^3 points; under TG_OPT_SUMFACT_ALWAYS
+   * wherever they apply), and the 3D cells of 20 nodes and more (P3 and higher tetrahedra,
+   * pyramids and prisms, Q2 and higher hexahedra off the sum-factorised point sets; FP32 and
+   * FP64, at most 128 points, X, J and det J) run the register-resident, runtime-K or
+   * cooperative kernel instead of the dense matrix-core kernel, whose values equal the
+   * register and runtime-K kernels' bit for bit. */
   TG_OPT_NO_MFMA = 1u << 12,
   /* The cooperative strategy (tetraloma-geometry/src/cooperative/factors.rs:100-183) for any
    * shape: each cell's node sums split among L lanes (L the largest power of two not above
