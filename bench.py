@@ -503,8 +503,8 @@ def bench_q3(api, M, steps, warmup, n=96):
     """Q3 hexahedra (96^3 cells, 64 nodes and 64 Gauss points per cell), X + J + det J: the FP32
     sum-factorised kernel with its second and third sums on the matrix cores (the default), the
     same on the vector ALU, FP32 on FP64-stored nodes (matrix cores with FP64 first sums split
-    hi/lo, and the vector ALU with every sum in FP64), the dense register kernel, and the FP64
-    kernel; each FP32 result's error against the FP64 kernel over the whole mesh."""
+    hi/lo, and the vector ALU with every sum in FP64), the dense matrix-core and the dense
+    register kernels (sum factorisation off), and the FP64 kernel; each FP32 result's error against the FP64 kernel over the whole mesh."""
     import torch
 
     h = M.config3_mesh(n=n)
@@ -537,8 +537,12 @@ def bench_q3(api, M, steps, warmup, n=96):
                     "rel_err_vs_fp64_kernel": {nm: device_rel_err(out[nm], want[nm])
                                                for nm in names}}
     rec["ms"], rec["frac"] = rec["mfma"]["ms"], rec["mfma"]["frac"]
-    rec["ms_dense_register_kernel"] = time_steps(lambda: api.geometric_factors_block(
+    # Sum factorisation off: the dense matrix-core kernel, and the dense register kernel.
+    rec["ms_dense_matrix_cores"] = time_steps(lambda: api.geometric_factors_block(
         g, 0, ps, names, out=out, options=api.OPT_NO_SUMFACT), max(2, steps // 5), 1)
+    rec["ms_dense_register_kernel"] = time_steps(lambda: api.geometric_factors_block(
+        g, 0, ps, names, out=out, options=api.OPT_NO_SUMFACT | api.OPT_NO_MFMA),
+        max(2, steps // 5), 1)
     g.set_nodes_f64(torch.tensor(h.nodes - np.asarray(g.origin), dtype=torch.float64,
                                  device="cuda"))
     nb64 = nbytes + 3 * 4 * N
