@@ -422,13 +422,13 @@ static bool takes_sumfact(const tg_geometry* g, const DeviceTables* tabs, uint32
 // Whether the factors of a block go to the dense matrix-core kernel (dense_mfma.hip): a 3D cell
 // of a node count it has (FP32 with the cell shift, on FP32 nodes or the FP64 copy: 20 to 84
 // nodes in registers and more in the runtime-k form; FP64: 20 nodes and more), the point set's
-// operand-order table (at most 128 points), tiles of 16, X, J and det J only, a block the
-// sum-factorised kernels do not take, unless a strategy is forced. Calls without J (X alone,
-// det J alone: the measure's and the equispaced conversion's) take it only where the cell has no
-// register-resident kernel, whose lane-per-cell mappings serve them. Measured on MI355X, X + J +
-// det J: P3 tetrahedra on 663,552 cells 0.121 against 0.150 ms at 14 points, 0.413 against 0.503
-// ms at 64; the cells without register kernels 2.4-45x faster than the runtime-K and cooperative
-// kernels (dense_mfma.hip lists them).
+// operand-order table (at most 128 points), tiles of 16, any of X, J and det J and nothing else,
+// a block the sum-factorised kernels do not take, unless a strategy is forced. Measured on
+// MI355X, X + J + det J: P3 tetrahedra on 663,552 cells 0.120 against 0.152 ms at 14 points,
+// 0.414 against 0.513 ms at 64; the cells without register kernels 2.4-45x faster than the
+// runtime-K and cooperative kernels (dense_mfma.hip lists them). X alone and det J alone too: P3
+// tetrahedra at 14 points 55 / 73 us against 90 / 112 us on the register kernel's lane-per-cell
+// mappings (FP32; FP64 95 / 143 against 265 / 280 us), Q2 hexahedra 1.2-2.4x.
 bool takes_dense_mfma(const tg_geometry* g, const Block& b, const DeviceTables* tabs,
                       uint32_t factors, uint32_t tile, uint32_t options) {
   const bool f32 = g->dtype == TG_F32;
@@ -439,10 +439,7 @@ bool takes_dense_mfma(const tg_geometry* g, const Block& b, const DeviceTables* 
       !(factors & xjd) || (factors & ~xjd) ||
       (options & (TG_OPT_NO_MFMA | TG_OPT_FORCE_GENERIC | TG_OPT_COOPERATIVE)))
     return false;
-  if (takes_sumfact(g, tabs, factors, tile, options, f32)) return false;
-  if (factors & TG_JACOBIAN) return true;
-  return f32 ? !tg::has_direct_factor<float>(b.tdim, g->gdim, b.k, true)
-             : !tg::has_direct_factor<double>(b.tdim, g->gdim, b.k, false);
+  return !takes_sumfact(g, tabs, factors, tile, options, f32);
 }
 
 template <typename T>

@@ -64,16 +64,18 @@ def test_ragged_blocks(cuda, cells):
 
 
 def test_factor_subsets_and_fallbacks(cuda):
-    """J alone and J with X or det J run on the matrix cores and equal the full call bit for bit;
-    X alone, det J alone (the measure's call), more than 128 points (Gauss degree 10, 216) and
-    TG_OPT_COOPERATIVE / TG_OPT_FORCE_GENERIC keep their kernels."""
+    """Every subset of X, J and det J runs on the matrix cores and equals the full call bit for
+    bit, and X alone and det J alone (the measure's call) equal the register kernel's
+    (TG_OPT_NO_MFMA); more than 128 points (Gauss degree 10, 216) and TG_OPT_COOPERATIVE /
+    TG_OPT_FORCE_GENERIC keep their kernels."""
     import torch
 
     mesh = volume_mesh(TET, 3)
     g = api.geometry_from_host(mesh, dtype=torch.float32)
     ps = api.PointSet.gauss(TET, 6)
     full = gpu_factors(g, 0, ps, NAMES)
-    for subset in (["jacobians"], ["points", "jacobians"], ["jacobians", "detj"]):
+    for subset in (["jacobians"], ["points", "jacobians"], ["jacobians", "detj"], ["points"],
+                   ["detj"], ["points", "detj"]):
         part = gpu_factors(g, 0, ps, subset)
         for n in subset:
             assert np.array_equal(part[n], full[n]), (subset, n)
