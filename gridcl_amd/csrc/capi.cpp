@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/tetraloma_geom.h"
+#include "dense_mfma.hpp"
 #include "geom_internal.hpp"
 #include "handles.hpp"
 #include "refcell.hpp"
@@ -422,7 +423,8 @@ static bool takes_sumfact(const tg_geometry* g, const DeviceTables* tabs, uint32
 // Whether the factors of a block go to the dense matrix-core kernel (dense_mfma.hip): a 3D cell
 // of a node count it has (FP32 with the cell shift, on FP32 nodes or the FP64 copy: 20 to 84
 // nodes in registers and more in the runtime-k form; FP64: 20 nodes and more), the point set's
-// operand-order table (at most 128 points), tiles of 16, any of X, J and det J and nothing else,
+// operand-order table (at most 128 points), tiles of 16, any of X, J, det J, K and the facet
+// normals (these with the point set's reference normals) and nothing else,
 // a block the sum-factorised kernels do not take, unless a strategy is forced. Measured on
 // MI355X, X + J + det J: P3 tetrahedra on 663,552 cells 0.120 against 0.152 ms at 14 points,
 // 0.414 against 0.513 ms at 64; the cells without register kernels 2.4-45x faster than the
@@ -432,11 +434,12 @@ static bool takes_sumfact(const tg_geometry* g, const DeviceTables* tabs, uint32
 bool takes_dense_mfma(const tg_geometry* g, const Block& b, const DeviceTables* tabs,
                       uint32_t factors, uint32_t tile, uint32_t options) {
   const bool f32 = g->dtype == TG_F32;
-  const uint32_t xjd = TG_POINTS | TG_JACOBIAN | TG_DETJ;
+  const uint32_t taken = TG_POINTS | TG_JACOBIAN | TG_DETJ | TG_PINV | TG_FACET_NORMALS;
   if ((f32 && (options & TG_OPT_NO_CELL_SHIFT)) || !tabs->dense_mfma || tile != 16 ||
       !(f32 ? tg::has_dense_mfma(b.tdim, g->gdim, b.k)
             : tg::has_dense_mfma_fp64(b.tdim, g->gdim, b.k)) ||
-      !(factors & xjd) || (factors & ~xjd) ||
+      !(factors & taken) || (factors & ~taken) ||
+      ((factors & TG_FACET_NORMALS) && !tabs->ref_normals) ||
       (options & (TG_OPT_NO_MFMA | TG_OPT_FORCE_GENERIC | TG_OPT_COOPERATIVE)))
     return false;
   return !takes_sumfact(g, tabs, factors, tile, options, f32);
@@ -493,10 +496,11 @@ int enqueue_block(const tg_geometry* g, const Block& b, tg_pointset* pts, Device
   if constexpr (sizeof(T) == 4) {
     if (takes_dense_mfma(g, b, tabs, factors, tile, options)) {
       const float* ta = static_cast<const float*>(tabs->dense_mfma);
-      hipError_t e = n64 ? tg::launch_dense_mfma_f64_nodes(b.k, n64, b.dofmap, ta, tabs->tab64, p,
-                                                           o, stream)
+      const float* rn = static_cast<const float*>(tabs->ref_normals);
+      hipError_t e = n64 ? tg::launch_dense_mfma_f64_nodes(b.k, n64, b.dofmap, ta, tabs->tab64,
+                                                           rn, p, o, stream)
                          : tg::launch_dense_mfma(b.k, static_cast<const float*>(g->nodes),
-                                                 b.dofmap, ta, tabs->tab64, p, o, stream);
+                                                 b.dofmap, ta, tabs->tab64, rn, p, o, stream);
       if (e == hipSuccess) return TG_OK;
       if (e != hipErrorNotSupported) return hip_fail(e, "dense matrix-core launch");
     }
@@ -504,7 +508,8 @@ int enqueue_block(const tg_geometry* g, const Block& b, tg_pointset* pts, Device
     if (takes_dense_mfma(g, b, tabs, factors, tile, options)) {
       hipError_t e = tg::launch_dense_mfma_fp64(
           b.k, static_cast<const double*>(g->nodes), b.dofmap,
-          static_cast<const double*>(tabs->dense_mfma), tabs->tab64, p, o, stream);
+          static_cast<const double*>(tabs->dense_mfma), tabs->tab64,
+          static_cast<const double*>(tabs->ref_normals), p, o, stream);
       if (e == hipSuccess) return TG_OK;
       if (e != hipErrorNotSupported) return hip_fail(e, "dense matrix-core launch");
     }

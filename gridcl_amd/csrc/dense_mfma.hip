@@ -1,7 +1,8 @@
-// X, J and det J of FP32 3D cells of dense bases (20 to 84 nodes: P3 to P6 tetrahedra, P3
-// pyramids, P3 and P4 prisms, and Q2 and Q3 hexahedra at point sets the sum-factorised kernels
-// do not take) with the node sums on the matrix cores (v_mfma_f32_16x16x4f32). Same factors as
-// the register-resident and runtime-K kernels
+// X, J, det J, K and facet normals of 3D cells of dense bases (20 nodes and more: P3 and higher
+// tetrahedra, pyramids and prisms, and Q2 and higher hexahedra at point sets the sum-factorised
+// kernels do not take), FP32 and FP64, with the node sums on the matrix cores
+// (v_mfma_f32_16x16x4f32, v_mfma_f64_16x16x4f64). Same factors as the register-resident and
+// runtime-K kernels
 // (the reference's factor kernel, tetraloma-geometry/src/layout.rs:138-213: X_g = sum_k x_kg
 // phi_k, J_gd = sum_k x_kg d_d phi_k, det J by first-row cofactors), with the sums taken as
 // matrix products over the cells of a tile:
@@ -34,7 +35,9 @@
 
 #include <cstdint>
 
+#include "dense_mfma.hpp"
 #include "geom_internal.hpp"
+#include "jacobian.hpp"
 #include "sumfact_tile.hpp"
 
 namespace tg {
@@ -44,6 +47,8 @@ typedef float v4f __attribute__((ext_vector_type(4)));
 typedef double v4d __attribute__((ext_vector_type(4)));
 
 constexpr int kWaves = 4;
+// The factors formed from J: the J products run only for calls that ask for one of them.
+constexpr uint32_t kNeedJ = kFJac | kFDetJ | kFPinv | kFFacetNormals;
 
 __device__ __forceinline__ v4f mfma4(float a, float b, v4f c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
@@ -120,22 +125,26 @@ __device__ __noinline__ double detj_fp64_k(const TN* __restrict__ nodes,
 // The end of one block of 16 points of a tile, from the products acc[v][g] (v = 0: X, as
 // offsets from node 0 in FP32 and absolute in FP64; 1..3: J(g, d = v - 1)); lane (cell, kg) =
 // (l % 16, l / 16) holds four rows qq of the block (points 16 qb + qq) of its cell: 4 kg + i in
-// FP32, kg + 4 i in FP64 (the layouts of the two instructions' results).
-// det J of every point first (in FP32 with the test for the FP64 recomputation); J goes through
-// the whole stage first, then X and det J over it, the recomputation of flagged points running
-// with the accumulators dead; the block's rows leave as contiguous runs of each field.
-template <typename T, typename TN>
+// FP32, kg + 4 i in FP64 (the layouts of the two instructions' results). det J of every point
+// first (in FP32 with the test for the FP64 recomputation); J, then K, go through the whole
+// stage, the facet normals (from K and the point's reference normal) wait in registers; then X,
+// det J (the recomputation of flagged points running with the accumulators dead) and the facet
+// normals share the stage; each field's rows of the block leave as one contiguous run. K and the
+// facet normals are the register kernels' formulas (jacobian.hpp, contraction off).
+template <typename T, typename TN, bool KF>
 __device__ __forceinline__ void finish_block(const typename Acc<T>::type (&acc)[4][3],
                                              const T (&x0)[3], const TN* __restrict__ nodes,
                                              const uint32_t* __restrict__ row, uint32_t k,
                                              const double* __restrict__ tab64,
+                                             const T* __restrict__ ref_normals,
                                              const FactorParams<T>& p, uint64_t t, uint32_t qb,
-                                             T* __restrict__ stage, gptr<T> ox, gptr<T> oj,
-                                             gptr<T> od, uint32_t lane) {
+                                             T* __restrict__ stage, const FactorOutputs& o,
+                                             uint32_t lane) {
   constexpr bool kShift = sizeof(T) == 4;
   T* __restrict__ sj = stage;
   T* __restrict__ sx = stage;
   T* __restrict__ sd = stage + 3 * 256;
+  T* __restrict__ sn = stage + 4 * 256;
   const uint32_t cell = lane & 15u, kg = lane >> 4;
   const uint32_t mask = p.mask;
   const uint32_t nv = p.nq - qb * 16u < 16u ? p.nq - qb * 16u : 16u;  // rows in the block
@@ -143,26 +152,24 @@ __device__ __forceinline__ void finish_block(const typename Acc<T>::type (&acc)[
   const auto qrow = [kg](int i) -> uint32_t {
     return sizeof(T) == 4 ? kg * 4u + static_cast<uint32_t>(i) : kg + 4u * static_cast<uint32_t>(i);
   };
+  const auto jac_of = [&acc](int i, T (&j)[9]) {  // column-major: J(g, d) at d * 3 + g
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+#pragma unroll
+      for (int g = 0; g < 3; ++g) j[d * 3 + g] = acc[1 + d][g][i];
+  };
   T det[4];
   uint32_t near = 0;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-#pragma clang fp contract(off)
-    T jm[3][3];  // jm[d][g] = J(g, d)
-#pragma unroll
-    for (int d = 0; d < 3; ++d)
-#pragma unroll
-      for (int g = 0; g < 3; ++g) jm[d][g] = acc[1 + d][g][i];
-    det[i] = jm[0][0] * (jm[1][1] * jm[2][2] - jm[2][1] * jm[1][2]) -
-             jm[1][0] * (jm[0][1] * jm[2][2] - jm[2][1] * jm[0][2]) +
-             jm[2][0] * (jm[0][1] * jm[1][2] - jm[1][1] * jm[0][2]);
+    T j[9];
+    jac_of(i, j);
+    det[i] = jac::det_of<T, 3, 3>(j);
     if constexpr (sizeof(T) == 4) {
       if (p.fixup) {
         float n2 = 0.f;
 #pragma unroll
-        for (int d = 0; d < 3; ++d)
-#pragma unroll
-          for (int g = 0; g < 3; ++g) n2 = __builtin_fmaf(jm[d][g], jm[d][g], n2);
+        for (int e = 0; e < 9; ++e) n2 = __builtin_fmaf(j[e], j[e], n2);
         if (__builtin_fabsf(det[i]) < static_cast<float>(p.tau) * n2 * __builtin_sqrtf(n2))
           near |= 1u << i;
       }
@@ -172,23 +179,56 @@ __device__ __forceinline__ void finish_block(const typename Acc<T>::type (&acc)[
   if (mask & kFJac) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const uint32_t qq = qrow(i);
+      T j[9];
+      jac_of(i, j);
 #pragma unroll
-      for (int d = 0; d < 3; ++d)
-#pragma unroll
-        for (int g = 0; g < 3; ++g) sj[(qq * 9 + d * 3 + g) * 16 + cell] = acc[1 + d][g][i];
+      for (int e = 0; e < 9; ++e) sj[(qrow(i) * 9 + e) * 16 + cell] = j[e];
     }
     sf::wave_sync();
-    flush(sj, oj + ((r0 * 9) << 4), nv * 144u, lane);
-    sf::wave_sync();  // J is read; X and det J may overwrite it
+    flush(sj, gp(static_cast<T*>(o.j)) + ((r0 * 9) << 4), nv * 144u, lane);
+    sf::wave_sync();  // J is read; the stage is free
+  }
+  T fn[4][3];
+  if (KF && (mask & (kFPinv | kFFacetNormals))) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+#pragma clang fp contract(off)
+      T j[9], kk[9];
+      jac_of(i, j);
+      jac::pinv_of<T, 3, 3>(j, kk);  // K(t, g) at g * 3 + t
+      if (mask & kFPinv) {
+#pragma unroll
+        for (int e = 0; e < 9; ++e) sj[(qrow(i) * 9 + e) * 16 + cell] = kk[e];
+      }
+      if (mask & kFFacetNormals) {
+        const uint32_t q = qb * 16u + qrow(i) < p.nq ? qb * 16u + qrow(i) : p.nq - 1;
+        const T* __restrict__ rn = ref_normals + q * 3;
+        T s2 = T(0);
+#pragma unroll
+        for (int g = 0; g < 3; ++g) {
+          T a = kk[g * 3] * gp(rn)[0];
+#pragma unroll
+          for (int tt = 1; tt < 3; ++tt) a += kk[g * 3 + tt] * gp(rn)[tt];
+          fn[i][g] = a;
+          s2 = g == 0 ? a * a : s2 + a * a;
+        }
+        const T nrm = jac::jsqrt(s2);
+#pragma unroll
+        for (int g = 0; g < 3; ++g) fn[i][g] = fn[i][g] / nrm;
+      }
+    }
+    if (mask & kFPinv) {
+      sf::wave_sync();
+      flush(sj, gp(static_cast<T*>(o.pinv)) + ((r0 * 9) << 4), nv * 144u, lane);
+      sf::wave_sync();  // K is read; the stage is free
+    }
   }
   if (mask & kFPoints) {
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      const uint32_t qq = qrow(i);
 #pragma unroll
       for (int g = 0; g < 3; ++g)
-        sx[(qq * 3 + g) * 16 + cell] =
+        sx[(qrow(i) * 3 + g) * 16 + cell] =
             kShift ? (x0[g] + acc[0][g][i]) + p.origin[g] : acc[0][g][i] + p.origin[g];
     }
   }
@@ -203,17 +243,27 @@ __device__ __forceinline__ void finish_block(const typename Acc<T>::type (&acc)[
 #pragma unroll
     for (int i = 0; i < 4; ++i) sd[qrow(i) * 16 + cell] = det[i];
   }
+  if (KF && (mask & kFFacetNormals)) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int g = 0; g < 3; ++g) sn[(qrow(i) * 3 + g) * 16 + cell] = fn[i][g];
+  }
   sf::wave_sync();
-  if (mask & kFPoints) flush(sx, ox + ((r0 * 3) << 4), nv * 48u, lane);
-  if (mask & kFDetJ) flush(sd, od + (r0 << 4), nv * 16u, lane);
+  if (mask & kFPoints) flush(sx, gp(static_cast<T*>(o.x)) + ((r0 * 3) << 4), nv * 48u, lane);
+  if (mask & kFDetJ) flush(sd, gp(static_cast<T*>(o.detj)) + (r0 << 4), nv * 16u, lane);
+  if (KF && (mask & kFFacetNormals))
+    flush(sn, gp(static_cast<T*>(o.fnormals)) + ((r0 * 3) << 4), nv * 48u, lane);
   sf::wave_sync();  // the stage is read; the next block may overwrite it
 }
 
-template <int K, typename TN>
+// KF: the form that also forms K and the facet normals (their registers would cost the other
+// calls a wavefront per SIMD).
+template <int K, typename TN, bool KF>
 __global__ void __launch_bounds__(64 * kWaves)
     dense_mfma(const TN* __restrict__ nodes, const uint32_t* __restrict__ dofmap,
                const float* __restrict__ taba, const double* __restrict__ tab64,
-               FactorParams<float> p, FactorOutputs o) {
+               const float* __restrict__ ref_normals, FactorParams<float> p, FactorOutputs o) {
   constexpr int S = (K + 3) / 4;  // node steps of four (and gather rounds of four lanes a cell)
   __shared__ __align__(16) float lds[kWaves * kStageElems];
   const uint32_t lane = threadIdx.x & 63u;
@@ -231,9 +281,6 @@ __global__ void __launch_bounds__(64 * kWaves)
 
   const uint32_t cell = lane & 15u, kg = lane >> 4;
   const uint32_t mask = p.mask;
-  gptr<float> ox = gp(static_cast<float*>(o.x));
-  gptr<float> oj = gp(static_cast<float*>(o.j));
-  gptr<float> od = gp(static_cast<float*>(o.detj));
   for (uint64_t t = lo + (static_cast<uint64_t>(j) * kWaves + w); t < hi;
        t += static_cast<uint64_t>(nx) * kWaves) {
     const uint64_t c0 = t * 16 + cell;
@@ -286,13 +333,14 @@ __global__ void __launch_bounds__(64 * kWaves)
 #pragma unroll
         for (int vv = 0; vv < 4; ++vv) {
           if (vv == 0 && !(mask & kFPoints)) continue;
-          if (vv > 0 && !(mask & (kFJac | kFDetJ))) continue;
+          if (vv > 0 && !(mask & kNeedJ)) continue;
           const float a = ta[(vv * S + s) * 64];
 #pragma unroll
           for (int g = 0; g < 3; ++g) acc[vv][g] = mfma4(a, b[s][g], acc[vv][g]);
         }
       }
-      finish_block<float>(acc, x0, nodes, row, K, tab64, p, t, qb, stage, ox, oj, od, lane);
+      finish_block<float, TN, KF>(acc, x0, nodes, row, K, tab64, ref_normals, p, t, qb, stage, o,
+                                  lane);
     }
   }
 }
@@ -313,11 +361,11 @@ constexpr int rt_chunk() {
   return sizeof(T) == 4 ? 8 : 4;  // steps of four nodes per chunk of B operands
 }
 
-template <typename T, typename TN>
+template <typename T, typename TN, bool KF>
 __global__ void __launch_bounds__(64 * rt_waves<T>())
     dense_mfma_rt(const TN* __restrict__ nodes, const uint32_t* __restrict__ dofmap,
                   const T* __restrict__ taba, const double* __restrict__ tab64,
-                  FactorParams<T> p, FactorOutputs o) {
+                  const T* __restrict__ ref_normals, FactorParams<T> p, FactorOutputs o) {
   constexpr int kW = rt_waves<T>(), kC = rt_chunk<T>();
   constexpr bool kShift = sizeof(T) == 4;
   typedef typename Acc<T>::type V;
@@ -334,9 +382,6 @@ __global__ void __launch_bounds__(64 * rt_waves<T>())
   const uint64_t hi = lo + per + (xcd < rem ? 1u : 0u);
   const uint32_t cell = lane & 15u, kg = lane >> 4;
   const uint32_t mask = p.mask;
-  gptr<T> ox = gp(static_cast<T*>(o.x));
-  gptr<T> oj = gp(static_cast<T*>(o.j));
-  gptr<T> od = gp(static_cast<T*>(o.detj));
   for (uint64_t it = lo + (static_cast<uint64_t>(j) * kW + w); it < hi;
        it += static_cast<uint64_t>(nx) * kW) {
     const uint64_t t = it / nqb;
@@ -390,20 +435,20 @@ __global__ void __launch_bounds__(64 * rt_waves<T>())
 #pragma unroll
         for (int vv = 0; vv < 4; ++vv) {
           if (vv == 0 && !(mask & kFPoints)) continue;
-          if (vv > 0 && !(mask & (kFJac | kFDetJ))) continue;
+          if (vv > 0 && !(mask & kNeedJ)) continue;
           const T a = ta[(vv * S + s0 + i) * 64];
 #pragma unroll
           for (int g = 0; g < 3; ++g) acc[vv][g] = mfma4(a, b[i][g], acc[vv][g]);
         }
       }
     }
-    finish_block<T>(acc, x0, nodes, row, K, tab64, p, t, qb, stage, ox, oj, od, lane);
+    finish_block<T, TN, KF>(acc, x0, nodes, row, K, tab64, ref_normals, p, t, qb, stage, o, lane);
   }
 }
 
 template <typename T, typename TN>
-using DenseFn = void (*)(const TN*, const uint32_t*, const T*, const double*, FactorParams<T>,
-                         FactorOutputs);
+using DenseFn = void (*)(const TN*, const uint32_t*, const T*, const double*, const T*,
+                         FactorParams<T>, FactorOutputs);
 
 // 3D cells of 20 to 84 nodes. Measured on MI355X against the kernels they replace (X + J + det
 // J, FP32): P3 tetrahedra 1.25x (the register kernel), Q2 hexahedra 1.11x and Q3 hexahedra 2.9x
@@ -411,27 +456,27 @@ using DenseFn = void (*)(const TN*, const uint32_t*, const T*, const double*, Fa
 // tetrahedra 12x, P3 prisms 51x (the runtime-K kernel), P5 and P6 tetrahedra 14x and 9x, P4
 // prisms 7.7x (the cooperative kernel). P2 tetrahedra (10 nodes), P2 pyramids (14) and P2
 // prisms (18) measured slower here than on their register kernels and keep them.
-template <typename TN>
+template <typename TN, bool KF>
 DenseFn<float, TN> pick(uint32_t k) {
   switch (k) {
     case 20:  // P3 tetrahedra
-      return dense_mfma<20, TN>;
+      return dense_mfma<20, TN, KF>;
     case 27:  // Q2 hexahedra (point sets the sum-factorised kernels do not take)
-      return dense_mfma<27, TN>;
+      return dense_mfma<27, TN, KF>;
     case 30:  // P3 pyramids
-      return dense_mfma<30, TN>;
+      return dense_mfma<30, TN, KF>;
     case 35:  // P4 tetrahedra
-      return dense_mfma<35, TN>;
+      return dense_mfma<35, TN, KF>;
     case 40:  // P3 prisms
-      return dense_mfma<40, TN>;
+      return dense_mfma<40, TN, KF>;
     case 56:  // P5 tetrahedra
-      return dense_mfma<56, TN>;
+      return dense_mfma<56, TN, KF>;
     case 64:  // Q3 hexahedra (as for Q2)
-      return dense_mfma<64, TN>;
+      return dense_mfma<64, TN, KF>;
     case 75:  // P4 prisms
-      return dense_mfma<75, TN>;
+      return dense_mfma<75, TN, KF>;
     case 84:  // P6 tetrahedra
-      return dense_mfma<84, TN>;
+      return dense_mfma<84, TN, KF>;
     default:
       return nullptr;
   }
@@ -444,14 +489,16 @@ constexpr uint32_t kF64MinNodes = 20;
 
 template <typename T, typename TN>
 hipError_t launch(uint32_t k, const TN* nodes, const uint32_t* dofmap, const T* taba,
-                  const double* tab64, const FactorParams<T>& p, const FactorOutputs& out,
-                  hipStream_t stream) {
+                  const double* tab64, const T* ref_normals, const FactorParams<T>& p,
+                  const FactorOutputs& out, hipStream_t stream) {
+  const bool kf = (p.mask & (kFPinv | kFFacetNormals)) != 0;
   DenseFn<T, TN> fn = nullptr;
-  if constexpr (sizeof(T) == 4) fn = pick<TN>(k);
+  if constexpr (sizeof(T) == 4) fn = kf ? pick<TN, true>(k) : pick<TN, false>(k);
   const bool rt = fn == nullptr && k >= (sizeof(T) == 4 ? kRtMinNodes : kF64MinNodes);
-  if (rt) fn = dense_mfma_rt<T, TN>;
+  if (rt) fn = kf ? dense_mfma_rt<T, TN, true> : dense_mfma_rt<T, TN, false>;
   const int waves = rt ? rt_waves<T>() : kWaves;
-  if (!fn || p.tile_log2 != 4 || p.nq == 0 || p.nq > kDenseMfmaMaxPoints || p.k != k)
+  if (!fn || p.tile_log2 != 4 || p.nq == 0 || p.nq > kDenseMfmaMaxPoints || p.k != k ||
+      ((p.mask & kFFacetNormals) && ref_normals == nullptr))
     return hipErrorNotSupported;
   int dev = 0, cus = 0;
   hipError_t e = hipGetDevice(&dev);
@@ -476,14 +523,14 @@ hipError_t launch(uint32_t k, const TN* nodes, const uint32_t* dofmap, const T* 
   grid = (grid < need8 ? grid : need8) / 8 * 8;
   if (grid < 8) grid = 8;
   hipLaunchKernelGGL(fn, dim3(static_cast<uint32_t>(grid)), dim3(64 * waves), 0, stream, nodes,
-                     dofmap, taba, tab64, p, out);
+                     dofmap, taba, tab64, ref_normals, p, out);
   return hipGetLastError();
 }
 
 }  // namespace
 
 bool has_dense_mfma(int tdim, int gdim, uint32_t k) {
-  return tdim == 3 && gdim == 3 && (pick<float>(k) != nullptr || k >= kRtMinNodes);
+  return tdim == 3 && gdim == 3 && (pick<float, false>(k) != nullptr || k >= kRtMinNodes);
 }
 
 bool has_dense_mfma_fp64(int tdim, int gdim, uint32_t k) {
@@ -496,23 +543,24 @@ size_t dense_mfma_table_elems(uint32_t k, uint32_t nq) {
 }
 
 hipError_t launch_dense_mfma(uint32_t k, const float* nodes, const uint32_t* dofmap,
-                             const float* taba, const double* tab64, const FactorParams<float>& p,
-                             const FactorOutputs& out, hipStream_t stream) {
-  return launch<float, float>(k, nodes, dofmap, taba, tab64, p, out, stream);
+                             const float* taba, const double* tab64, const float* ref_normals,
+                             const FactorParams<float>& p, const FactorOutputs& out,
+                             hipStream_t stream) {
+  return launch<float, float>(k, nodes, dofmap, taba, tab64, ref_normals, p, out, stream);
 }
 
 hipError_t launch_dense_mfma_f64_nodes(uint32_t k, const double* nodes, const uint32_t* dofmap,
                                        const float* taba, const double* tab64,
-                                       const FactorParams<float>& p, const FactorOutputs& out,
-                                       hipStream_t stream) {
-  return launch<float, double>(k, nodes, dofmap, taba, tab64, p, out, stream);
+                                       const float* ref_normals, const FactorParams<float>& p,
+                                       const FactorOutputs& out, hipStream_t stream) {
+  return launch<float, double>(k, nodes, dofmap, taba, tab64, ref_normals, p, out, stream);
 }
 
 hipError_t launch_dense_mfma_fp64(uint32_t k, const double* nodes, const uint32_t* dofmap,
                                   const double* taba, const double* tab64,
-                                  const FactorParams<double>& p, const FactorOutputs& out,
-                                  hipStream_t stream) {
-  return launch<double, double>(k, nodes, dofmap, taba, tab64, p, out, stream);
+                                  const double* ref_normals, const FactorParams<double>& p,
+                                  const FactorOutputs& out, hipStream_t stream) {
+  return launch<double, double>(k, nodes, dofmap, taba, tab64, ref_normals, p, out, stream);
 }
 
 }  // namespace tg

@@ -168,32 +168,6 @@ hipError_t launch_sumfact_f64_nodes(int n1, int m, const double* nodes, const ui
                                     const FactorParams<float>& p, const FactorOutputs& out,
                                     hipStream_t stream);
 
-// Dense FP32 node sums on the matrix cores for 3D cells of k nodes (P3 tetrahedra, k = 20;
-// dense_mfma.hip): X, J and det J, tiles of 16, at most kDenseMfmaMaxPoints points. taba is
-// the table in operand order, [ceil(nq / 16)][4 kinds][ceil(k / 4)][64 lanes]: lane l of block
-// qb, kind v, step s holds T_v[16 qb + l % 16][4 s + l / 16] (0 outside nq and k), with
-// T_0 = phi and T_d = d phi / d xi_d, rounded once from FP64. hipErrorNotSupported for other k.
-constexpr uint32_t kDenseMfmaMaxPoints = 128;
-bool has_dense_mfma(int tdim, int gdim, uint32_t k);
-size_t dense_mfma_table_elems(uint32_t k, uint32_t nq);
-hipError_t launch_dense_mfma(uint32_t k, const float* nodes, const uint32_t* dofmap,
-                             const float* taba, const double* tab64, const FactorParams<float>& p,
-                             const FactorOutputs& out, hipStream_t stream);
-// The same on FP64-stored nodes: each offset from node 0 taken in FP64 and rounded to FP32 (as
-// the register kernels on FP64 nodes take it), the FP64 recomputation from the FP64 nodes.
-hipError_t launch_dense_mfma_f64_nodes(uint32_t k, const double* nodes, const uint32_t* dofmap,
-                                       const float* taba, const double* tab64,
-                                       const FactorParams<float>& p, const FactorOutputs& out,
-                                       hipStream_t stream);
-// FP64 geometries (v_mfma_f64_16x16x4f64): 3D cells of k >= 20 nodes, the table in the same
-// operand order in FP64, sums of the absolute coordinates from node 0 as the FP64 kernels take
-// them.
-bool has_dense_mfma_fp64(int tdim, int gdim, uint32_t k);
-hipError_t launch_dense_mfma_fp64(uint32_t k, const double* nodes, const uint32_t* dofmap,
-                                  const double* taba, const double* tab64,
-                                  const FactorParams<double>& p, const FactorOutputs& out,
-                                  hipStream_t stream);
-
 // The cooperative strategy (k up to 256 nodes): L lanes per cell, L the largest power of two not
 // above min(32, k), leaves added by a pairwise tree. hipErrorNotSupported for other shapes.
 template <typename T>

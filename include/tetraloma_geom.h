@@ -127,7 +127,7 @@ enum {
    * faster: degree 2 at 3^3 points, degree 3 at 4^3 points; under TG_OPT_SUMFACT_ALWAYS
    * wherever they apply), and the 3D cells of 20 nodes and more (P3 and higher tetrahedra,
    * pyramids and prisms, Q2 and higher hexahedra off the sum-factorised point sets; FP32 and
-   * FP64, at most 128 points, X, J and det J) run the register-resident, runtime-K or
+   * FP64, at most 128 points, X, J, det J, K, facet normals) run the register-resident, runtime-K or
    * cooperative kernel instead of the dense matrix-core kernel, whose values equal the
    * register and runtime-K kernels' bit for bit. */
   TG_OPT_NO_MFMA = 1u << 12,

@@ -275,3 +275,45 @@ def test_every_shape_against_oracle_and_node_order_kernels(cuda, idx, dtype):
         assert err[n] <= tol[n], (label, n, err[n], tol[n])
     for n in ("points", "detj"):
         assert np.array_equal(gpu_factors(g, 0, ps, [n])[n], got[n]), (label, n)
+
+
+@pytest.mark.parametrize("idx", range(len(SHAPES)))
+@pytest.mark.parametrize("dtype", ["f32", "f64"])
+def test_pinv_and_facet_normals_on_every_shape(cuda, idx, dtype):
+    """K with X, J and det J, K alone, and facet normals at the facet Gauss points, on the matrix
+    cores:
+    bit for bit the runtime-K kernel's (and the register kernel's where there is one), and
+    against the oracle within the reference's scaled tolerances (K with kappa(J),
+    tests/test_gpu_coop.py; 1e-12 / 1e-5 base for FP64 / FP32; the degree-4 pyramid's tables as
+    above), the facet normals with K's tolerance (1.3e-5 measured for FP32 P8 tetrahedra, whose
+    K tolerance is 3e-5 there)."""
+    import torch
+
+    from tests.test_gpu_coop import scaled_tolerances
+
+    label, cell, p, make, has_reg = SHAPES[idx]
+    f64 = dtype == "f64"
+    mesh = volume_mesh(cell, p, n=2)
+    g = api.geometry_from_host(mesh, dtype=torch.float64 if f64 else torch.float32)
+    names = NAMES + ["pinv"]
+    fps = api.PointSet.facet_gauss(cell, 2)
+    for ps, fac in ((make(), names), (make(), ["pinv"]), (fps, ["facet_normals"])):
+        got = gpu_factors(g, 0, ps, fac)
+        gen = gpu_factors(g, 0, ps, fac, options=api.OPT_FORCE_GENERIC)
+        for n in fac:
+            assert np.array_equal(got[n], gen[n]), (label, n)
+        if has_reg:
+            reg = gpu_factors(g, 0, ps, fac, options=api.OPT_NO_MFMA)
+            for n in fac:
+                assert np.array_equal(got[n], reg[n]), (label, n)
+        want = oracle_factors(g, mesh, 0, ps, fac)
+        err = rel_errs(got, want)
+        base = 1e-12 if f64 else 1e-5
+        jw = oracle_factors(g, mesh, 0, ps, ["jacobians"])["jacobians"]
+        tol = scaled_tolerances(cell, p, ps.points, base, jw)
+        tol["facet_normals"] = tol["pinv"]  # K^T n / |K^T n|: K's tolerance
+        if f64 and (cell, p) == (PYR, 4):  # the tables' digit, as in the test above
+            tol = {n: 1e-9 if n in ("pinv", "facet_normals") else 1e-10 for n in tol}
+        print(f"{label} {dtype} {fac}: {err}")
+        for n in fac:
+            assert err[n] <= tol[n], (label, n, err[n], tol[n])
