@@ -127,9 +127,9 @@ enum {
    * faster: degree 2 at 3^3 points, degree 3 at 4^3 points; under TG_OPT_SUMFACT_ALWAYS
    * wherever they apply), and the 3D cells of 20 nodes and more (P3 and higher tetrahedra,
    * pyramids and prisms, Q2 and higher hexahedra off the sum-factorised point sets; FP32 and
-   * FP64, at most 128 points, X, J, det J, K, facet normals) run the register-resident, runtime-K or
-   * cooperative kernel instead of the dense matrix-core kernel, whose values equal the
-   * register and runtime-K kernels' bit for bit. */
+   * FP64, at most 128 points, X, J, det J, K, facet normals) run the register-resident,
+   * runtime-K or cooperative kernel instead of the dense matrix-core kernel, whose values equal
+   * the register and runtime-K kernels' bit for bit. */
   TG_OPT_NO_MFMA = 1u << 12,
   /* The cooperative strategy (tetraloma-geometry/src/cooperative/factors.rs:100-183) for any
    * shape: each cell's node sums split among L lanes (L the largest power of two not above
