@@ -317,3 +317,30 @@ def test_pinv_and_facet_normals_on_every_shape(cuda, idx, dtype):
         print(f"{label} {dtype} {fac}: {err}")
         for n in fac:
             assert err[n] <= tol[n], (label, n, err[n], tol[n])
+
+
+@pytest.mark.parametrize("cell,p,deg", [(TET, 3, 6), (TET, 7, 8), (PRISM, 5, 6)])
+@pytest.mark.parametrize("dtype", ["f32", "f64"])
+def test_ragged_and_clamped_blocks(cuda, cell, p, deg, dtype):
+    """The register-operand form (P3 tetrahedra) and the runtime-k form (P7 tetrahedra, P5 prisms;
+    FP64 always) on blocks of 1, 17 and 37 cells (the last tile, and the last (tile, block of
+    points) item, part-full) with dofmap entries past the last node (read as the last node,
+    evector.rs:187-191): bit for bit the runtime-K kernel, which clamps the same way."""
+    import torch
+
+    mesh = volume_mesh(cell, p, n=2)
+    b = mesh.blocks[0]
+    cells = b.cells.copy()
+    cells[3, 5] = 10_000_000
+    cells[min(20, len(cells) - 1), 0] = 0xFFFFFFFF
+    ps = api.PointSet.gauss(cell, deg)
+    assert len(ps) <= 128
+    tdt = torch.float64 if dtype == "f64" else torch.float32
+    for n in (1, 17, 37):
+        m = M.HostMesh(mesh.gdim, mesh.nodes, [M.HostBlock(b.cell, b.degree, cells[:n])])
+        g = api.geometry_from_host(m, dtype=tdt)
+        names = NAMES + ["pinv"]
+        got = gpu_factors(g, 0, ps, names)
+        gen = gpu_factors(g, 0, ps, names, options=api.OPT_FORCE_GENERIC)
+        for nm in names:
+            assert np.array_equal(got[nm], gen[nm]), (n, nm)
