@@ -77,6 +77,74 @@ __device__ __forceinline__ gptr<T> gp(T* p) {
   return (gptr<T>)p;
 }
 
+// acc[v][g] += sum over node steps s0 ... s0 + N - 1 (those below S) of the products of the A
+// operands T_v (table row (v * S + s) * 64 from ta) with the B operands b[s - s0][g], in step
+// order, for the kinds the call asks for (WX: v = 0, X; WJ: v = 1..3, J). The kinds are fixed at
+// compile time so that the table loads carry no branches: they go in groups of two steps, each
+// group's loads issued before the previous group's products, so a block waits about one round
+// trip to the cache for its table instead of one a step.
+template <bool WX, bool WJ, int N, typename T, typename V>
+__device__ __forceinline__ void step_products(gptr<const T> ta, uint32_t S, uint32_t s0,
+                                              const T (&b)[N][3], V (&acc)[4][3]) {
+  constexpr int G = 2, NG = (N + G - 1) / G;
+  T a[2][4][G];
+  const auto load = [&](int gi) {
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+      // Steps past S (and past N) load the last step's row: in bounds, never used.
+      const uint32_t s0i = s0 + static_cast<uint32_t>(gi * G + i < N ? gi * G + i : N - 1);
+      const uint32_t s = s0i < S ? s0i : S - 1;
+#pragma unroll
+      for (int vv = 0; vv < 4; ++vv)
+        if (vv == 0 ? WX : WJ) a[gi & 1][vv][i] = ta[(vv * S + s) * 64];
+    }
+  };
+  load(0);
+#pragma unroll
+  for (int gi = 0; gi < NG; ++gi) {
+    if (gi + 1 < NG) load(gi + 1);
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+      const int si = gi * G + i;
+      if (si >= N || s0 + static_cast<uint32_t>(si) >= S) break;  // uniform
+#pragma unroll
+      for (int vv = 0; vv < 4; ++vv)
+        if (vv == 0 ? WX : WJ)
+#pragma unroll
+          for (int g = 0; g < 3; ++g) acc[vv][g] = mfma4(a[gi & 1][vv][i], b[si][g], acc[vv][g]);
+    }
+  }
+}
+
+// BATCH false: one table load per step and kind, each waited for before its products (the
+// kinds tested at run time; fewer registers).
+template <bool BATCH, int N, typename T, typename V>
+__device__ __forceinline__ void products(uint32_t mask, gptr<const T> ta, uint32_t S, uint32_t s0,
+                                         const T (&b)[N][3], V (&acc)[4][3]) {
+  if constexpr (!BATCH) {
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+      if (s0 + i >= S) break;  // uniform
+#pragma unroll
+      for (int vv = 0; vv < 4; ++vv) {
+        if (vv == 0 && !(mask & kFPoints)) continue;
+        if (vv > 0 && !(mask & kNeedJ)) continue;
+        const T a = ta[(vv * S + s0 + i) * 64];
+#pragma unroll
+        for (int g = 0; g < 3; ++g) acc[vv][g] = mfma4(a, b[i][g], acc[vv][g]);
+      }
+    }
+    return;
+  }
+  const bool wx = (mask & kFPoints) != 0, wj = (mask & kNeedJ) != 0;
+  if (wx && wj)
+    step_products<true, true>(ta, S, s0, b, acc);
+  else if (wx)
+    step_products<true, false>(ta, S, s0, b, acc);
+  else
+    step_products<false, true>(ta, S, s0, b, acc);
+}
+
 // Elements of a wavefront's LDS: the stage of one block of 16 points, J [16][9][16], then, over
 // it once J is stored, X [16][3][16] and det J [16][16] (36 KiB a workgroup: four per CU).
 constexpr int kStageElems = 9 * 256;
@@ -258,8 +326,11 @@ __device__ __forceinline__ void finish_block(const typename Acc<T>::type (&acc)[
 }
 
 // KF: the form that also forms K and the facet normals (their registers would cost the other
-// calls a wavefront per SIMD).
-template <int K, typename TN, bool KF>
+// calls a wavefront per SIMD). SCHED, the order of a tile's loads: 0, a table load per step and
+// kind; 1, the table loads in groups ahead of the products; 2, as 1, and the next tile's dofmap
+// entries loaded during this tile's products and its coordinates during this tile's output (more
+// registers: fewer wavefronts per SIMD).
+template <int K, typename TN, bool KF, int SCHED>
 __global__ void __launch_bounds__(64 * kWaves)
     dense_mfma(const TN* __restrict__ nodes, const uint32_t* __restrict__ dofmap,
                const float* __restrict__ taba, const double* __restrict__ tab64,
@@ -281,37 +352,45 @@ __global__ void __launch_bounds__(64 * kWaves)
 
   const uint32_t cell = lane & 15u, kg = lane >> 4;
   const uint32_t mask = p.mask;
-  for (uint64_t t = lo + (static_cast<uint64_t>(j) * kWaves + w); t < hi;
-       t += static_cast<uint64_t>(nx) * kWaves) {
-    const uint64_t c0 = t * 16 + cell;
-    const uint64_t c = c0 < p.num_cells ? c0 : p.num_cells - 1;
-    const uint32_t* __restrict__ row = dofmap + c * K;
-    // Gather: lane (cell, kg) loads nodes kg, kg + 4, ... of its cell (past k: node k - 1
-    // again, so the loads go together) and node 0.
-    TN x0n[3];
-    float x0[3];
-    {
-      const uint32_t r0 = gp(row)[0];
-      const uint32_t n0 = r0 < p.last_node ? r0 : p.last_node;
-#pragma unroll
-      for (int g = 0; g < 3; ++g) {
-        x0n[g] = gp(nodes)[static_cast<uint64_t>(n0) * 3 + g];
-        x0[g] = static_cast<float>(x0n[g]);
-      }
-    }
-    // B operands, formed in registers: the node lane (cell, kg) loads at step s is node
-    // 4 s + kg, whose offset from node 0 (taken in the nodes' type, then rounded to FP32) is the
-    // operand (zero for node 0 and past k).
-    TN v[S][3];
+  const uint64_t step = static_cast<uint64_t>(nx) * kWaves;
+  uint64_t t = lo + (static_cast<uint64_t>(j) * kWaves + w);
+  if (t >= hi || nqb == 0) return;
+  const auto row_of = [&](uint64_t tt) -> const uint32_t* {
+    const uint64_t c0 = tt * 16 + cell;
+    return dofmap + (c0 < p.num_cells ? c0 : p.num_cells - 1) * K;
+  };
+  // Gather: lane (cell, kg) loads node 0 and nodes kg, kg + 4, ... of its cell (past k: node
+  // k - 1 again, so the loads go together). The dofmap entries (rr[0]: node 0, rr[1 + s]: node
+  // 4 s + kg) and then the coordinates of the next tile are loaded while this tile's products
+  // and output run, so a tile starts with its operands in registers.
+  uint32_t rr[S + 1];
+  TN x0n[3], v[S][3];
+  const auto rows = [&](uint64_t tt) {
+    const uint32_t* __restrict__ row = row_of(tt);
+    rr[0] = gp(row)[0];
 #pragma unroll
     for (int s = 0; s < S; ++s) {
       const uint32_t k = 4u * s + kg;
-      const uint32_t r = gp(row)[k < static_cast<uint32_t>(K) ? k : K - 1];
-      const uint32_t n = r < p.last_node ? r : p.last_node;
+      rr[1 + s] = gp(row)[k < static_cast<uint32_t>(K) ? k : K - 1];
+    }
+  };
+  const auto coords = [&]() {
+    const uint32_t n0 = rr[0] < p.last_node ? rr[0] : p.last_node;
+#pragma unroll
+    for (int g = 0; g < 3; ++g) x0n[g] = gp(nodes)[static_cast<uint64_t>(n0) * 3 + g];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      const uint32_t n = rr[1 + s] < p.last_node ? rr[1 + s] : p.last_node;
 #pragma unroll
       for (int g = 0; g < 3; ++g) v[s][g] = gp(nodes)[static_cast<uint64_t>(n) * 3 + g];
     }
-    float b[S][3];
+  };
+  // B operands, formed in registers: the node lane (cell, kg) loads at step s is node 4 s + kg,
+  // whose offset from node 0 (taken in the nodes' type, then rounded to FP32) is the operand
+  // (zero for node 0 and past k).
+  const auto offsets = [&](float (&x0)[3], float (&b)[S][3]) {
+#pragma unroll
+    for (int g = 0; g < 3; ++g) x0[g] = static_cast<float>(x0n[g]);
 #pragma unroll
     for (int s = 0; s < S; ++s) {
       const uint32_t k = 4u * s + kg;
@@ -320,28 +399,49 @@ __global__ void __launch_bounds__(64 * kWaves)
         b[s][g] = (k > 0 && k < static_cast<uint32_t>(K)) ? static_cast<float>(v[s][g] - x0n[g])
                                                           : 0.f;
     }
-    for (uint32_t qb = 0; qb < nqb; ++qb) {
-      const v4f zero = {0.f, 0.f, 0.f, 0.f};
-      v4f acc[4][3];
+  };
+  const auto block_products = [&](uint32_t qb, const float (&b)[S][3], v4f (&acc)[4][3]) {
+    const v4f zero = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-      for (int vv = 0; vv < 4; ++vv)
+    for (int vv = 0; vv < 4; ++vv)
 #pragma unroll
-        for (int g = 0; g < 3; ++g) acc[vv][g] = zero;
-      gptr<const float> ta = gp(taba) + qb * (4u * S * 64u) + lane;
-#pragma unroll
-      for (int s = 0; s < S; ++s) {
-#pragma unroll
-        for (int vv = 0; vv < 4; ++vv) {
-          if (vv == 0 && !(mask & kFPoints)) continue;
-          if (vv > 0 && !(mask & kNeedJ)) continue;
-          const float a = ta[(vv * S + s) * 64];
-#pragma unroll
-          for (int g = 0; g < 3; ++g) acc[vv][g] = mfma4(a, b[s][g], acc[vv][g]);
-        }
+      for (int g = 0; g < 3; ++g) acc[vv][g] = zero;
+    products<(SCHED > 0)>(mask, gp(taba) + qb * (4u * S * 64u) + lane, S, 0u, b, acc);
+  };
+  if constexpr (SCHED < 2) {
+    for (; t < hi; t += step) {
+      const uint32_t* __restrict__ row = row_of(t);
+      rows(t);
+      coords();
+      float x0[3], b[S][3];
+      offsets(x0, b);
+      for (uint32_t qb = 0; qb < nqb; ++qb) {
+        v4f acc[4][3];
+        block_products(qb, b, acc);
+        finish_block<float, TN, KF>(acc, x0, nodes, row, K, tab64, ref_normals, p, t, qb, stage,
+                                    o, lane);
       }
+    }
+    return;
+  }
+  rows(t);
+  coords();
+  for (; t < hi; t += step) {
+    const uint32_t* __restrict__ row = row_of(t);
+    float x0[3], b[S][3];
+    offsets(x0, b);
+    // The last tile fetches its own entries again (in bounds, unused).
+    rows(t + step < hi ? t + step : t);
+    v4f acc[4][3];
+    for (uint32_t qb = 0; qb + 1 < nqb; ++qb) {
+      block_products(qb, b, acc);
       finish_block<float, TN, KF>(acc, x0, nodes, row, K, tab64, ref_normals, p, t, qb, stage, o,
                                   lane);
     }
+    block_products(nqb - 1, b, acc);
+    coords();  // the next tile's; b is dead from here
+    finish_block<float, TN, KF>(acc, x0, nodes, row, K, tab64, ref_normals, p, t, nqb - 1, stage,
+                                o, lane);
   }
 }
 
@@ -429,18 +529,7 @@ __global__ void __launch_bounds__(64 * rt_waves<T>())
             b[i][g] = k < K ? static_cast<T>(v[i][g]) : T(0);
         }
       }
-#pragma unroll
-      for (int i = 0; i < kC; ++i) {
-        if (s0 + i >= S) break;  // uniform
-#pragma unroll
-        for (int vv = 0; vv < 4; ++vv) {
-          if (vv == 0 && !(mask & kFPoints)) continue;
-          if (vv > 0 && !(mask & kNeedJ)) continue;
-          const T a = ta[(vv * S + s0 + i) * 64];
-#pragma unroll
-          for (int g = 0; g < 3; ++g) acc[vv][g] = mfma4(a, b[i][g], acc[vv][g]);
-        }
-      }
+      products<false>(mask, ta, S, s0, b, acc);
     }
     finish_block<T, TN, KF>(acc, x0, nodes, row, K, tab64, ref_normals, p, t, qb, stage, o, lane);
   }
@@ -456,27 +545,40 @@ using DenseFn = void (*)(const TN*, const uint32_t*, const T*, const double*, co
 // tetrahedra 12x, P3 prisms 51x (the runtime-K kernel), P5 and P6 tetrahedra 14x and 9x, P4
 // prisms 7.7x (the cooperative kernel). P2 tetrahedra (10 nodes), P2 pyramids (14) and P2
 // prisms (18) measured slower here than on their register kernels and keep them.
+// The load schedule of each node count (SCHED above), from X + J + det J in FP32 on MI355X
+// against the table load per step (us per launch, one host): batched table loads for 56 nodes
+// and more (P5 tetrahedra 82.2 -> 78.9, Q3 hexahedra at 100 points 1055 -> 833, P4 prisms 99.3
+// -> 65.2, P6 tetrahedra 74.0 -> 61.6); with the prefetch for P3 tetrahedra (14 points 120.0 ->
+// 112.9; 64 points 410 against 413) and P3 prisms (147.3 -> 125.3), in the plain form on FP32
+// nodes only (the prefetch's registers); the table load per step for 27, 30 and 35 nodes (Q2
+// hexahedra at 40 points 272 against 343 batched).
+template <int K, typename TN, bool KF>
+constexpr int sched_of() {
+  constexpr int s = K >= 56 ? 1 : (K == 20 || K == 40) ? 2 : 0;
+  return s == 2 && (KF || sizeof(TN) != 4) ? 1 : s;
+}
+
 template <typename TN, bool KF>
 DenseFn<float, TN> pick(uint32_t k) {
   switch (k) {
     case 20:  // P3 tetrahedra
-      return dense_mfma<20, TN, KF>;
+      return dense_mfma<20, TN, KF, sched_of<20, TN, KF>()>;
     case 27:  // Q2 hexahedra (point sets the sum-factorised kernels do not take)
-      return dense_mfma<27, TN, KF>;
+      return dense_mfma<27, TN, KF, sched_of<27, TN, KF>()>;
     case 30:  // P3 pyramids
-      return dense_mfma<30, TN, KF>;
+      return dense_mfma<30, TN, KF, sched_of<30, TN, KF>()>;
     case 35:  // P4 tetrahedra
-      return dense_mfma<35, TN, KF>;
+      return dense_mfma<35, TN, KF, sched_of<35, TN, KF>()>;
     case 40:  // P3 prisms
-      return dense_mfma<40, TN, KF>;
+      return dense_mfma<40, TN, KF, sched_of<40, TN, KF>()>;
     case 56:  // P5 tetrahedra
-      return dense_mfma<56, TN, KF>;
+      return dense_mfma<56, TN, KF, sched_of<56, TN, KF>()>;
     case 64:  // Q3 hexahedra (as for Q2)
-      return dense_mfma<64, TN, KF>;
+      return dense_mfma<64, TN, KF, sched_of<64, TN, KF>()>;
     case 75:  // P4 prisms
-      return dense_mfma<75, TN, KF>;
+      return dense_mfma<75, TN, KF, sched_of<75, TN, KF>()>;
     case 84:  // P6 tetrahedra
-      return dense_mfma<84, TN, KF>;
+      return dense_mfma<84, TN, KF, sched_of<84, TN, KF>()>;
     default:
       return nullptr;
   }
