@@ -613,9 +613,10 @@ hipError_t launch(uint32_t k, const TN* nodes, const uint32_t* dofmap, const T* 
                                                    64 * waves, 0);
   if (e != hipSuccess) return e;
   // Tiles of more than three blocks of points are bound by their write stream, which a
-  // narrower band of tiles in flight writes faster: three workgroups a CU there (P3 tetrahedra
-  // at 64 points 413.8 against 425.5 us with four; at 14 points four measured faster, 120.8
-  // against 125.0 us, and Q2 hexahedra at 40 points 272.6 against 285.1 us).
+  // narrower band of tiles in flight writes faster: three workgroups a CU there (measured with
+  // the table load per step, 116 VGPRs: P3 tetrahedra at 64 points 413.8 against 425.5 us with
+  // four; at 14 points four measured faster, 120.8 against 125.0 us, and Q2 hexahedra at 40
+  // points 272.6 against 285.1 us).
   if (!rt && (p.nq + 15) / 16 > 3 && per_cu > 3) per_cu = 3;
   // Work items: tiles, or (tile, block of 16 points) for the runtime-k form.
   const uint64_t items = (p.padded_cells >> 4) * (rt ? (p.nq + 15) / 16 : 1);
