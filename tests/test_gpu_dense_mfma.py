@@ -278,6 +278,29 @@ def test_every_shape_against_oracle_and_node_order_kernels(cuda, idx, dtype):
 
 
 @pytest.mark.parametrize("idx", range(len(SHAPES)))
+def test_every_shape_on_fp64_stored_nodes(cuda, idx):
+    """FP32 geometries on FP64-stored nodes, every cell the kernel takes: each offset from node 0
+    taken in FP64 and rounded once, so X, J and det J equal the runtime-K kernel's on the same
+    nodes (TG_OPT_FORCE_GENERIC) bit for bit; X alone and det J alone (the forms whose table
+    loads leave out the other kind) equal the full call; and the values are not those of the
+    FP32-node geometry (the FP64 nodes were read)."""
+    import torch
+
+    label, cell, p, make, _ = SHAPES[idx]
+    ps = make()
+    mesh = volume_mesh(cell, p, n=2)
+    g = api.geometry_from_host(mesh, dtype=torch.float32, nodes_f64=True)
+    got = gpu_factors(g, 0, ps, NAMES)
+    gen = gpu_factors(g, 0, ps, NAMES, options=api.OPT_FORCE_GENERIC)
+    for n in NAMES:
+        assert np.array_equal(got[n], gen[n]), (label, n)
+    for n in ("points", "detj"):
+        assert np.array_equal(gpu_factors(g, 0, ps, [n])[n], got[n]), (label, n)
+    f32 = gpu_factors(api.geometry_from_host(mesh, dtype=torch.float32), 0, ps, NAMES)
+    assert not np.array_equal(got["jacobians"], f32["jacobians"]), label
+
+
+@pytest.mark.parametrize("idx", range(len(SHAPES)))
 @pytest.mark.parametrize("dtype", ["f32", "f64"])
 def test_pinv_and_facet_normals_on_every_shape(cuda, idx, dtype):
     """K with X, J and det J, K alone, and facet normals at the facet Gauss points, on the matrix
